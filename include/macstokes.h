@@ -1,0 +1,107 @@
+/*
+ * macstokes.h -- C ABI of the MI355X (gfx950) MAC shifted-Stokes stage solve.
+ *
+ * The library solves, for each right-hand-side column f,
+ *     (mass*I + viscosity*K) u + G p = f,   D u = 0,   mean(p) = 0
+ * on a uniform no-slip MAC grid (the reference's `TensorStokes.solve`,
+ * solver/mac/tensor_stokes.py:245-364): the velocity Helmholtz blocks are inverted
+ * exactly by 2-D DST transforms written as MFMA GEMMs, and the pressure Schur
+ * complement D L^-1 D^T is solved by preconditioned CG with the preconditioner
+ * mass*(D D^T)^+ + viscosity*I (2-D DCT-II), all on the GPU.
+ *
+ * Layouts (the reference's, solver/mac/grid.py:52-65 and kernels.py:55):
+ *   velocity column: size = (nx-1)*ny + nx*(ny-1) doubles,
+ *                    [u interior faces (ny, nx-1) row-major ; v interior faces (ny-1, nx)]
+ *   pressure column: np = nx*ny doubles, cell (j, i) at j*nx + i
+ *   k columns: column c starts at base + c*ld (ld >= size or >= np).
+ *
+ * Every call is synchronous: it returns after the GPU work is done, and keeps no
+ * pointer it was given.  Pointers are host memory unless `device_pointers` is 1,
+ * then they are device memory of the handle's device.  One handle must not be used
+ * from two threads at once.
+ *
+ * Return codes (mapped by the Python binding to the reference's exceptions,
+ * tensor_stokes.py:137-147, 246-257, 322-323, 345-351):
+ */
+#ifndef MACSTOKES_H
+#define MACSTOKES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAC_OK 0
+#define MAC_INVALID_ARGUMENT 1 /* ValueError: invalid controls, shifts or RHS    */
+#define MAC_NOT_CONVERGED 2    /* RuntimeError: Schur CG hit max_iterations       */
+#define MAC_GATE_FAILED 3      /* RuntimeError: residual / divergence gate failed */
+#define MAC_HIP_ERROR 4        /* RuntimeError: HIP runtime failure               */
+
+#define MAC_PRECISION_FP64 0   /* FP64 MFMA transforms (the plain GPU solve)       */
+#define MAC_PRECISION_MIXED 1  /* BF16 MFMA transforms, FP32 accumulation, inside
+                                  an FP64 defect-correction loop                   */
+
+typedef struct mac_stokes_opts {
+  double tolerance;        /* residual / divergence gate (reference default 1e-9)    */
+  double krylov_tolerance; /* Schur-CG rtol, ||r|| < rtol*||b|| (default 1e-12)      */
+  int32_t max_iterations;  /* Schur-CG maxiter (default 100)                         */
+  int32_t precision;       /* MAC_PRECISION_*                                        */
+  double refine_tolerance; /* MIXED: stop when residual and divergence/(1+max|u|)
+                              are both <= this (default 1e-12)                       */
+  double inner_tolerance;  /* MIXED: inner Schur-CG rtol per sweep (default 1e-3)    */
+  int32_t max_sweeps;      /* MIXED: outer defect-correction sweep cap (default 30)  */
+  int32_t device;          /* HIP device ordinal                                      */
+} mac_stokes_opts;
+
+typedef struct mac_solve_report {
+  double residual;        /* max|mass u + visc K u + G p - f| / (1 + max|f|)      */
+  double divergence_inf;  /* max|D u|                                              */
+  double velocity_max;    /* max|u| (the divergence gate's scale)                  */
+  double seconds;         /* wall time of the call, device-synchronised            */
+  int32_t sweeps;         /* outer defect-correction sweeps (1 for FP64)           */
+  int32_t total_iterations; /* Schur-CG iterations summed over columns and sweeps  */
+} mac_solve_report;
+
+/* Fills `opts` with the reference defaults (tensor_stokes.py:128-136). */
+void mac_stokes_default_opts(mac_stokes_opts* opts);
+
+/* Replaces `TensorStokes.__init__` (tensor_stokes.py:128-174): validates controls,
+ * uploads the DST/DCT matrices and eigenvalues of an nx x ny grid on [0,lx]x[0,ly].
+ * Returns NULL on failure; mac_stokes_create_error() then describes why. */
+void* mac_stokes_create(int32_t nx, int32_t ny, double lx, double ly, const mac_stokes_opts* opts);
+const char* mac_stokes_create_error(void);
+void mac_stokes_destroy(void* handle);
+/* Message of the last failing call on this handle ("" if none). */
+const char* mac_stokes_last_error(void* handle);
+
+/* Replaces `TensorStokes.solve(rhs, mass=, viscosity=)` (tensor_stokes.py:245-364).
+ * rhs: ncols columns of `size`; velocity out: ncols columns of `size`; pressure
+ * out: ncols columns of `np` (mean zero).  `pressure_guess` (nullable, ncols columns
+ * of np) starts the Schur CG from a guess instead of zero.  iterations (nullable)
+ * receives ncols Schur-CG counts (FP64: as scipy's cg callback counts them). */
+int32_t mac_stokes_solve(void* handle, const double* rhs, int64_t ld_rhs, int32_t ncols, double mass,
+                         double viscosity, double* velocity, int64_t ld_velocity, double* pressure,
+                         int64_t ld_pressure, const double* pressure_guess, int64_t ld_guess,
+                         int32_t* iterations, mac_solve_report* report, int32_t device_pointers);
+
+/* ---- single operators, device pointers only (parity tests and host callers) ---- */
+
+/* D: packed velocity columns -> cell divergence (kernels.py:46-87). */
+int32_t mac_apply_divergence(void* handle, const double* velocity, int64_t ld_velocity, int32_t ncols,
+                             double* out, int64_t ld_out);
+/* D^T: pressure columns -> packed faces (kernels.py:89-132). */
+int32_t mac_apply_divergence_transpose(void* handle, const double* pressure, int64_t ld_pressure,
+                                       int32_t ncols, double* out, int64_t ld_out);
+/* (mass*I + viscosity*K)^-1 on packed velocity columns (tensor_stokes.py:200-215). */
+int32_t mac_velocity_inverse(void* handle, const double* in, int64_t ld_in, int32_t ncols, double mass,
+                             double viscosity, double* out, int64_t ld_out);
+/* (D D^T)^+ on pressure columns (tensor_stokes.py:217-243). */
+int32_t mac_pressure_poisson_inverse(void* handle, const double* in, int64_t ld_in, int32_t ncols,
+                                     double* out, int64_t ld_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MACSTOKES_H */

@@ -1,0 +1,114 @@
+// Internal declarations shared by the gfx950 kernels and the C-ABI host code.
+// Nothing here is part of the public interface (include/macstokes.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace macs {
+
+// ----------------------------------------------------------------- GEMM batch
+// One entry computes C = epi(A . Bt^T): A is [M][K] row-major (lda), Bt is [N][K]
+// row-major (ldb), so B[k][n] = Bt[n][k].  C is stored [M][N] (ldc) or, with
+// trans_c, as its transpose [N][M] (ldc).  Every 2-D DST/DCT pass is one such
+// product with the transform matrix as Bt (see transforms in stokes.hip).
+enum Epilogue : int {
+  EPI_STORE = 0,    // C = acc
+  EPI_SHIFT = 1,    // C = acc / (s0 + s1 * (ea[m] + eb[n]))        velocity shift
+  EPI_LAMBDA = 2,   // lam = ea[m] + eb[n]; C = lam > thresh ? acc/lam : 0   (D D^T)^+
+  EPI_COMBINE = 3,  // C = s0 * acc + s1 * aux                          preconditioner
+};
+enum DType : int { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2 };
+
+struct GemmEntry {
+  const void* A;
+  const void* Bt;
+  void* C;
+  const void* aux;     // EPI_COMBINE: same layout/type as C
+  const double* ea;    // eigenvalues indexed by m
+  const double* eb;    // eigenvalues indexed by n
+  int M, N, K;
+  int lda, ldb, ldc;
+  int tiles_m, tiles_n, tile_begin;
+};
+
+constexpr int kMaxEntries = 8;
+struct GemmBatch {
+  GemmEntry e[kMaxEntries];
+  int count;
+  int total_tiles;
+  int epi;
+  int trans_c;
+  double s0, s1, thresh;
+};
+
+// Tile edge used by the launchers (entries are tiled by the launcher itself).
+void gemm_f64(GemmBatch& b, hipStream_t s);
+// a_type: DT_F64 / DT_F32 / DT_BF16 for A and aux/C per c_type; Bt is always bf16.
+void gemm_bf16(GemmBatch& b, int a_type, int c_type, hipStream_t s);
+
+// ------------------------------------------------------------- stencil kernels
+// Internal velocity layout: u block [ny][ldu], v block [ny-1][ldv] (ld >= row length),
+// at offsets 0 and voff of each column; column stride vstride.  The reference's
+// packed layout is the special case ldu = nx-1, ldv = nx, voff = nu.
+struct VelLayout {
+  int ldu, ldv;
+  int64_t voff, stride;
+};
+
+struct GridDims {
+  int nx, ny;
+  double hx, hy;
+};
+
+// Per-column Schur-CG scalars, resident in device memory so the whole recurrence
+// stays on the GPU; the host only reads the `active` flags once per iteration.
+struct CgState {
+  double bnorm, tol, rr, rho, rho_prev, alpha, beta, pad0;
+  int active, iters, converged, pad1;
+};
+
+constexpr int kThreads = 256;
+// Blocks per column used by every reduction (fixed, so summation order is fixed).
+int reduce_blocks(int64_t n);
+
+enum Reduce : int { RED_NONE = 0, RED_SUM = 1, RED_MAXABS = 2 };
+
+// q[c] = sign * D vel[c]; optional per-block partial sums / max|.| of q.
+void divergence(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
+                int64_t out_stride, double sign, int ncols, int reduce, double* partials,
+                const CgState* st, hipStream_t s);
+// out[c] = add[c] + D^T (p[c] - mean[c]) (add / mean optional), out in layout L.
+void divergence_transpose(const double* p, int64_t p_stride, const double* mean_sums, const double* add,
+                          int64_t add_stride, int out_type, void* out, const VelLayout& L,
+                          const GridDims& g, int ncols, const CgState* st, hipStream_t s);
+// partials of sum(x) (mode RED_SUM, y == null), sum(x*y) (RED_SUM, y != null), max|x|.
+void reduce(const double* x, const double* y, int64_t n, int64_t stride, int ncols, int mode,
+            double* partials, const CgState* st, hipStream_t s);
+// out[c] = sum / max over the partials of column c (fixed order).
+void finalize(const double* partials, int nblk, int ncols, int mode, double* out, hipStream_t s);
+// x[c] -= sums[c]/n; optional partial sums of y*x afterwards.
+void sub_mean_dot(double* x, int64_t n, int64_t stride, int ncols, const double* sums, const double* y,
+                  double* partials, const CgState* st, hipStream_t s);
+void fill(double* x, int64_t n, double v, hipStream_t s);
+// out = a - b column-wise (skips inactive columns when st is given).
+void sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t stride, int ncols,
+              const CgState* st, hipStream_t s);
+void copy_cols(const double* src, int64_t src_stride, double* dst, int64_t dst_stride, int64_t n, int ncols,
+               hipStream_t s);
+
+// CG scalar steps (one thread per column).
+void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s);
+void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s);
+void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s);
+void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s);
+void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,
+                 hipStream_t s);
+void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_t n, int64_t stride,
+                  int ncols, const CgState* st, double* partials, hipStream_t s);
+
+// Final-solve checks over all columns: partial max|defect|, max|f|, max|vel| where
+// defect = mass*vel + visc*K vel - D^T p - f (vel, f packed; p mean-zero).
+void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols,
+                double mass, double visc, double* partials3, hipStream_t s);
+
+}  // namespace macs

@@ -1,0 +1,418 @@
+// MAC stencils and Schur-CG vector kernels (HBM-bound), gfx950.
+//
+// D (cell divergence) and D^T follow the reference's matrix-free kernels term by term
+// (solver/mac/kernels.py:46-132): each face value is divided by the spacing before it
+// is added, so the FP64 results are bitwise those of the reference loops.  Reductions
+// are two-level with a block count fixed by the vector length, so every sum has one
+// fixed order and a solve is bitwise reproducible run to run.
+#include "internal.h"
+
+namespace macs {
+namespace {
+
+__device__ inline double ld_d(const double* p, int64_t i) { return p[i]; }
+__device__ inline double ld_d(const float* p, int64_t i) { return (double)p[i]; }
+__device__ inline double ld_d(const __bf16* p, int64_t i) { return (double)(float)p[i]; }
+__device__ inline void st_d(double* p, int64_t i, double v) { p[i] = v; }
+__device__ inline void st_d(float* p, int64_t i, double v) { p[i] = (float)v; }
+__device__ inline void st_d(__bf16* p, int64_t i, double v) { p[i] = (__bf16)(float)v; }
+// |x| with NaN mapped to +inf, so max-norm checks catch non-finite values.
+__device__ inline double absn(double x) { return x != x ? (double)INFINITY : fabs(x); }
+
+// 256-thread block reduction in a fixed order; result valid in thread 0.
+template <int MODE>
+__device__ inline double block_reduce(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(v, off, 64);
+    v = MODE == RED_MAXABS ? fmax(v, o) : v + o;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = MODE == RED_MAXABS ? fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]))
+                           : (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  }
+  return v;
+}
+
+template <typename TIN, int MODE>
+__global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, VelLayout L, GridDims g,
+                                                  double* __restrict__ out, int64_t out_stride, double sign,
+                                                  double* partials, const CgState* st) {
+  const int c = blockIdx.y;
+  if (st != nullptr && !st[c].active) return;
+  __shared__ double sh[4];
+  const TIN* u = vel + (int64_t)c * L.stride;
+  const TIN* v = u + L.voff;
+  double* o = out + (int64_t)c * out_stride;
+  const int nx = g.nx, ny = g.ny;
+  const int64_t np = (int64_t)nx * ny;
+  double accum = 0.0;
+  for (int64_t cell = (int64_t)blockIdx.x * kThreads + threadIdx.x; cell < np;
+       cell += (int64_t)gridDim.x * kThreads) {
+    const int j = (int)(cell / nx), i = (int)(cell - (int64_t)j * nx);
+    double value = 0.0;
+    if (i < nx - 1) value += ld_d(u, (int64_t)j * L.ldu + i) / g.hx;
+    if (i > 0) value -= ld_d(u, (int64_t)j * L.ldu + i - 1) / g.hx;
+    if (j < ny - 1) value += ld_d(v, (int64_t)j * L.ldv + i) / g.hy;
+    if (j > 0) value -= ld_d(v, (int64_t)(j - 1) * L.ldv + i) / g.hy;
+    value = sign * value;
+    o[cell] = value;
+    if (MODE == RED_SUM) accum += value;
+    if (MODE == RED_MAXABS) accum = fmax(accum, absn(value));
+  }
+  if (MODE != RED_NONE) {
+    accum = block_reduce<MODE>(accum, sh);
+    if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
+  }
+}
+
+template <typename TOUT>
+__global__ __launch_bounds__(kThreads) void k_dt(const double* __restrict__ p, int64_t p_stride,
+                                                 const double* mean_sums, const double* __restrict__ add,
+                                                 int64_t add_stride, TOUT* __restrict__ out, VelLayout L,
+                                                 GridDims g, const CgState* st) {
+  const int c = blockIdx.y;
+  if (st != nullptr && !st[c].active) return;
+  const int nx = g.nx, ny = g.ny;
+  const int64_t nu = (int64_t)(nx - 1) * ny, nv = (int64_t)nx * (ny - 1);
+  const int64_t f = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (f >= nu + nv) return;
+  const double* pc = p + (int64_t)c * p_stride;
+  const double mean = mean_sums ? mean_sums[c] / (double)((int64_t)nx * ny) : 0.0;
+  TOUT* oc = out + (int64_t)c * L.stride;
+  double value;
+  int64_t dst;
+  if (f < nu) {
+    const int j = (int)(f / (nx - 1)), i = (int)(f - (int64_t)j * (nx - 1));
+    const int64_t left = (int64_t)j * nx + i;
+    value = mean_sums ? ((pc[left] - mean) - (pc[left + 1] - mean)) / g.hx : (pc[left] - pc[left + 1]) / g.hx;
+    dst = (int64_t)j * L.ldu + i;
+  } else {
+    const int64_t e = f - nu;
+    const int j = (int)(e / nx), i = (int)(e - (int64_t)j * nx);
+    const int64_t lower = (int64_t)j * nx + i;
+    value = mean_sums ? ((pc[lower] - mean) - (pc[lower + nx] - mean)) / g.hy : (pc[lower] - pc[lower + nx]) / g.hy;
+    dst = L.voff + (int64_t)j * L.ldv + i;
+  }
+  if (add != nullptr) value = add[(int64_t)c * add_stride + f] + value;
+  st_d(oc, dst, value);
+}
+
+template <int MODE, bool DOT>
+__global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ x, const double* __restrict__ y,
+                                                     int64_t n, int64_t stride, double* partials,
+                                                     const CgState* st) {
+  const int c = blockIdx.y;
+  if (st != nullptr && !st[c].active) return;
+  __shared__ double sh[4];
+  const double* xc = x + (int64_t)c * stride;
+  const double* yc = DOT ? y + (int64_t)c * stride : nullptr;
+  double accum = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    if (MODE == RED_MAXABS) accum = fmax(accum, absn(xc[i]));
+    else accum += DOT ? xc[i] * yc[i] : xc[i];
+  }
+  accum = block_reduce<MODE>(accum, sh);
+  if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_finalize(const double* partials, int nblk, double* out) {
+  const int c = blockIdx.x;
+  __shared__ double sh[4];
+  double accum = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kThreads) {
+    const double v = partials[(int64_t)c * nblk + i];
+    accum = MODE == RED_MAXABS ? fmax(accum, v) : accum + v;
+  }
+  accum = block_reduce<MODE>(accum, sh);
+  if (threadIdx.x == 0) out[c] = accum;
+}
+
+template <bool DOT>
+__global__ __launch_bounds__(kThreads) void k_sub_mean_dot(double* x, int64_t n, int64_t stride,
+                                                           const double* sums, const double* y,
+                                                           double* partials, const CgState* st) {
+  const int c = blockIdx.y;
+  if (st != nullptr && !st[c].active) return;
+  __shared__ double sh[4];
+  double* xc = x + (int64_t)c * stride;
+  const double* yc = DOT ? y + (int64_t)c * stride : nullptr;
+  const double mean = sums[c] / (double)n;
+  double accum = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double v = xc[i] - mean;
+    xc[i] = v;
+    if (DOT) accum += yc[i] * v;
+  }
+  if (DOT) {
+    accum = block_reduce<RED_SUM>(accum, sh);
+    if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
+  }
+}
+
+__global__ void k_sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t stride,
+                           const CgState* st) {
+  const int c = blockIdx.y;
+  if (st != nullptr && !st[c].active) return;
+  const int64_t off = (int64_t)c * stride;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    out[off + i] = a[off + i] - b[off + i];
+}
+
+__global__ void k_fill(double* x, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) x[i] = v;
+}
+
+__global__ void k_copy_cols(const double* src, int64_t ss, double* dst, int64_t ds, int64_t n) {
+  const int c = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    dst[(int64_t)c * ds + i] = src[(int64_t)c * ss + i];
+}
+
+// ---- CG scalars (scipy 1.15.3 cg loop: stop when ||r|| < rtol*||b||, tested at the
+// top of each iteration; rho = (r,z); beta = rho/rho_prev; alpha = rho/(p,q)).
+__global__ void k_cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols) {
+  const int c = threadIdx.x;
+  if (c >= ncols) return;
+  CgState s{};
+  s.bnorm = sqrt(bb[c]);
+  s.tol = rtol * s.bnorm;
+  s.rr = bb[c];
+  s.active = (bmax[c] != 0.0 && s.bnorm != 0.0) ? 1 : 0;
+  s.converged = s.active ? 0 : 1;
+  s.iters = 0;
+  st[c] = s;
+}
+
+__global__ void k_cg_check(CgState* st, const double* rr, int* flags, int ncols) {
+  const int c = threadIdx.x;
+  if (c >= ncols) return;
+  if (st[c].active) {
+    st[c].rr = rr[c];
+    if (sqrt(rr[c]) < st[c].tol) {
+      st[c].active = 0;
+      st[c].converged = 1;
+    }
+  }
+  flags[c] = st[c].active;
+}
+
+__global__ void k_cg_rho(CgState* st, const double* rz, int ncols) {
+  const int c = threadIdx.x;
+  if (c >= ncols || !st[c].active) return;
+  st[c].rho = rz[c];
+  st[c].beta = st[c].iters > 0 ? rz[c] / st[c].rho_prev : 0.0;
+}
+
+__global__ void k_cg_alpha(CgState* st, const double* pq, int ncols) {
+  const int c = threadIdx.x;
+  if (c >= ncols || !st[c].active) return;
+  st[c].alpha = st[c].rho / pq[c];
+  st[c].rho_prev = st[c].rho;
+  st[c].iters += 1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_update_p(double* __restrict__ p, const double* __restrict__ z,
+                                                       int64_t n, int64_t stride, const CgState* st) {
+  const int c = blockIdx.y;
+  if (!st[c].active) return;
+  const bool first = st[c].iters == 0;
+  const double beta = st[c].beta;
+  double* pc = p + (int64_t)c * stride;
+  const double* zc = z + (int64_t)c * stride;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    pc[i] = first ? zc[i] : __dadd_rn(__dmul_rn(pc[i], beta), zc[i]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_update_xr(double* __restrict__ x, double* __restrict__ r,
+                                                        const double* __restrict__ p, const double* __restrict__ q,
+                                                        int64_t n, int64_t stride, const CgState* st,
+                                                        double* partials) {
+  const int c = blockIdx.y;
+  if (!st[c].active) return;
+  __shared__ double sh[4];
+  const double alpha = st[c].alpha;
+  const int64_t off = (int64_t)c * stride;
+  double accum = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    x[off + i] = __dadd_rn(x[off + i], __dmul_rn(alpha, p[off + i]));
+    const double rv = __dsub_rn(r[off + i], __dmul_rn(alpha, q[off + i]));
+    r[off + i] = rv;
+    accum += rv * rv;
+  }
+  accum = block_reduce<RED_SUM>(accum, sh);
+  if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
+}
+
+// defect = mass*vel + visc*(K vel) - D^T p - f on every free face (operators.py:6-17 for
+// K: 5-point, tangential wall diagonal 3, normal diagonal 2), max-abs partials of the
+// defect, of f and of vel (three per block).
+__global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ vel, const double* __restrict__ f,
+                                                     const double* __restrict__ p, GridDims g, int ncols,
+                                                     double mass, double visc, double* partials) {
+  __shared__ double sh[4];
+  const int nx = g.nx, ny = g.ny;
+  const int64_t nu = (int64_t)(nx - 1) * ny, nv = (int64_t)nx * (ny - 1), size = nu + nv;
+  const int64_t np = (int64_t)nx * ny;
+  const double hx2 = g.hx * g.hx, hy2 = g.hy * g.hy;
+  const double ox = -1.0 / hx2, oy = -1.0 / hy2;
+  double md = 0.0, mf = 0.0, mv = 0.0;
+  const int64_t total = size * ncols;
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total; t += (int64_t)gridDim.x * kThreads) {
+    const int c = (int)(t / size);
+    const int64_t fidx = t - (int64_t)c * size;
+    const double* vc = vel + (int64_t)c * size;
+    const double* pc = p + (int64_t)c * np;
+    double kv, lift;
+    if (fidx < nu) {
+      const int w = nx - 1;
+      const int j = (int)(fidx / w), i = (int)(fidx - (int64_t)j * w);
+      const double* u = vc;
+      const double diag = 2.0 / hx2 + ((j == 0 || j == ny - 1) ? 3.0 : 2.0) / hy2;
+      kv = 0.0;
+      if (j > 0) kv += oy * u[fidx - w];
+      if (i > 0) kv += ox * u[fidx - 1];
+      kv += diag * u[fidx];
+      if (i < w - 1) kv += ox * u[fidx + 1];
+      if (j < ny - 1) kv += oy * u[fidx + w];
+      const int64_t left = (int64_t)j * nx + i;
+      lift = (pc[left] - pc[left + 1]) / g.hx;
+    } else {
+      const int64_t e = fidx - nu;
+      const int j = (int)(e / nx), i = (int)(e - (int64_t)j * nx);
+      const double* v = vc + nu;
+      const double diag = ((i == 0 || i == nx - 1) ? 3.0 : 2.0) / hx2 + 2.0 / hy2;
+      kv = 0.0;
+      if (j > 0) kv += oy * v[e - nx];
+      if (i > 0) kv += ox * v[e - 1];
+      kv += diag * v[e];
+      if (i < nx - 1) kv += ox * v[e + 1];
+      if (j < ny - 2) kv += oy * v[e + nx];
+      const int64_t lower = (int64_t)j * nx + i;
+      lift = (pc[lower] - pc[lower + nx]) / g.hy;
+    }
+    const double fv = f[(int64_t)c * size + fidx];
+    const double vv = vc[fidx];
+    const double d = mass * vv + visc * kv - lift - fv;
+    md = fmax(md, absn(d));
+    mf = fmax(mf, absn(fv));
+    mv = fmax(mv, absn(vv));
+  }
+  md = block_reduce<RED_MAXABS>(md, sh);
+  __syncthreads();
+  mf = block_reduce<RED_MAXABS>(mf, sh);
+  __syncthreads();
+  mv = block_reduce<RED_MAXABS>(mv, sh);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = md;
+    partials[gridDim.x + blockIdx.x] = mf;
+    partials[2 * gridDim.x + blockIdx.x] = mv;
+  }
+}
+
+inline dim3 col_grid(int64_t n, int ncols) { return dim3((unsigned)reduce_blocks(n), (unsigned)ncols); }
+
+}  // namespace
+
+int reduce_blocks(int64_t n) {
+  int64_t b = (n + (int64_t)kThreads * 8 - 1) / ((int64_t)kThreads * 8);
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return (int)b;
+}
+
+void divergence(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
+                int64_t out_stride, double sign, int ncols, int reduce, double* partials, const CgState* st,
+                hipStream_t s) {
+  const dim3 grid = col_grid((int64_t)g.nx * g.ny, ncols);
+#define MACS_DIV(T)                                                                                          \
+  switch (reduce) {                                                                                          \
+    case RED_SUM: k_div<T, RED_SUM><<<grid, kThreads, 0, s>>>((const T*)vel, L, g, out, out_stride, sign,   \
+                                                                partials, st); break;                        \
+    case RED_MAXABS: k_div<T, RED_MAXABS><<<grid, kThreads, 0, s>>>((const T*)vel, L, g, out, out_stride,   \
+                                                                      sign, partials, st); break;            \
+    default: k_div<T, RED_NONE><<<grid, kThreads, 0, s>>>((const T*)vel, L, g, out, out_stride, sign,       \
+                                                            partials, st); break;                            \
+  }
+  if (in_type == DT_F64) { MACS_DIV(double) }
+  else if (in_type == DT_F32) { MACS_DIV(float) }
+  else { MACS_DIV(__bf16) }
+#undef MACS_DIV
+}
+
+void divergence_transpose(const double* p, int64_t p_stride, const double* mean_sums, const double* add,
+                          int64_t add_stride, int out_type, void* out, const VelLayout& L, const GridDims& g,
+                          int ncols, const CgState* st, hipStream_t s) {
+  const int64_t faces = (int64_t)(g.nx - 1) * g.ny + (int64_t)g.nx * (g.ny - 1);
+  const dim3 grid((unsigned)((faces + kThreads - 1) / kThreads), (unsigned)ncols);
+  if (out_type == DT_F64)
+    k_dt<double><<<grid, kThreads, 0, s>>>(p, p_stride, mean_sums, add, add_stride, (double*)out, L, g, st);
+  else if (out_type == DT_F32)
+    k_dt<float><<<grid, kThreads, 0, s>>>(p, p_stride, mean_sums, add, add_stride, (float*)out, L, g, st);
+  else
+    k_dt<__bf16><<<grid, kThreads, 0, s>>>(p, p_stride, mean_sums, add, add_stride, (__bf16*)out, L, g, st);
+}
+
+void reduce(const double* x, const double* y, int64_t n, int64_t stride, int ncols, int mode, double* partials,
+            const CgState* st, hipStream_t s) {
+  const dim3 grid = col_grid(n, ncols);
+  if (mode == RED_MAXABS) k_reduce<RED_MAXABS, false><<<grid, kThreads, 0, s>>>(x, y, n, stride, partials, st);
+  else if (y != nullptr) k_reduce<RED_SUM, true><<<grid, kThreads, 0, s>>>(x, y, n, stride, partials, st);
+  else k_reduce<RED_SUM, false><<<grid, kThreads, 0, s>>>(x, y, n, stride, partials, st);
+}
+
+void finalize(const double* partials, int nblk, int ncols, int mode, double* out, hipStream_t s) {
+  if (mode == RED_MAXABS) k_finalize<RED_MAXABS><<<ncols, kThreads, 0, s>>>(partials, nblk, out);
+  else k_finalize<RED_SUM><<<ncols, kThreads, 0, s>>>(partials, nblk, out);
+}
+
+void sub_mean_dot(double* x, int64_t n, int64_t stride, int ncols, const double* sums, const double* y,
+                  double* partials, const CgState* st, hipStream_t s) {
+  const dim3 grid = col_grid(n, ncols);
+  if (y != nullptr) k_sub_mean_dot<true><<<grid, kThreads, 0, s>>>(x, n, stride, sums, y, partials, st);
+  else k_sub_mean_dot<false><<<grid, kThreads, 0, s>>>(x, n, stride, sums, y, partials, st);
+}
+
+void sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t stride, int ncols,
+              const CgState* st, hipStream_t s) {
+  k_sub_cols<<<col_grid(n, ncols), kThreads, 0, s>>>(a, b, out, n, stride, st);
+}
+
+void fill(double* x, int64_t n, double v, hipStream_t s) {
+  k_fill<<<reduce_blocks(n), kThreads, 0, s>>>(x, n, v);
+}
+
+void copy_cols(const double* src, int64_t ss, double* dst, int64_t ds, int64_t n, int ncols, hipStream_t s) {
+  k_copy_cols<<<col_grid(n, ncols), kThreads, 0, s>>>(src, ss, dst, ds, n);
+}
+
+void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s) {
+  k_cg_init<<<1, 64, 0, s>>>(st, bb, bmax, rtol, ncols);
+}
+void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s) {
+  k_cg_check<<<1, 64, 0, s>>>(st, rr, flags, ncols);
+}
+void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s) { k_cg_rho<<<1, 64, 0, s>>>(st, rz, ncols); }
+void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s) {
+  k_cg_alpha<<<1, 64, 0, s>>>(st, pq, ncols);
+}
+void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,
+                 hipStream_t s) {
+  k_update_p<<<col_grid(n, ncols), kThreads, 0, s>>>(p, z, n, stride, st);
+}
+void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_t n, int64_t stride, int ncols,
+                  const CgState* st, double* partials, hipStream_t s) {
+  k_update_xr<<<col_grid(n, ncols), kThreads, 0, s>>>(x, r, p, q, n, stride, st, partials);
+}
+
+void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols, double mass,
+                double visc, double* partials3, hipStream_t s) {
+  const int64_t size = (int64_t)(g.nx - 1) * g.ny + (int64_t)g.nx * (g.ny - 1);
+  k_defect<<<reduce_blocks(size * ncols), kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3);
+}
+
+}  // namespace macs

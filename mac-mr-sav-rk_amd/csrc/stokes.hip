@@ -1,0 +1,746 @@
+// Host orchestration of the MAC shifted-Stokes stage solve and its C ABI.
+//
+// Algorithm (the reference's TensorStokes.solve, solver/mac/tensor_stokes.py:245-364):
+//   W = L^-1 f                           L = mass I + viscosity K, by 2-D DST GEMMs
+//   b = -D W, mean removed               Schur right-hand side
+//   PCG on S = D L^-1 D^T, M^-1 = mass (D D^T)^+ + viscosity I, x0 = 0 (or a guess),
+//       stop when ||r|| < krylov_tolerance ||b|| at the top of an iteration
+//   p = x - mean(x);  u = L^-1 (f + D^T p)
+//   residual = max|mass u + viscosity K u - D^T p - f| / (1 + max|f|); divergence = max|D u|
+// All right-hand-side columns iterate together (one GEMM launch covers every column's
+// u and v blocks); a converged column is frozen, so each column's iterates and
+// iteration count are those of a column solved alone.
+//
+// MIXED precision wraps the same solve in an FP64 defect-correction loop: the inner
+// solve uses bf16 transform GEMMs with FP32 accumulation and a loose Schur-CG
+// tolerance; the outer residual, divergence and the velocity/pressure accumulation
+// stay FP64.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "macstokes.h"
+
+namespace macs {
+
+#define HIP_CHECK(expr)                                                                              \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      throw macs::HipFailure(std::string(#expr) + ": " + hipGetErrorString(e_));                           \
+  } while (0)
+
+struct HipFailure : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgFailure : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct SolveFailure : std::runtime_error {
+  int code;
+  SolveFailure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum Kind { KIND_DST1 = 0, KIND_DST2 = 1, KIND_DCT2 = 2 };
+
+// Orthonormal analysis matrix (forward = Q x, inverse = Q^T y), as scipy.fft's
+// norm="ortho" dst/dct (tensor_stokes.py:58-82).  The angle's integer numerator is
+// reduced modulo the period first, so every entry is a correctly-argued sin/cos.
+static std::vector<double> transform_matrix(int n, int kind) {
+  std::vector<double> q((size_t)n * n);
+  for (int k = 0; k < n; ++k) {
+    for (int j = 0; j < n; ++j) {
+      double v;
+      if (kind == KIND_DST1) {
+        const int64_t period = 2 * (int64_t)(n + 1);
+        const int64_t a = ((int64_t)(k + 1) * (j + 1)) % period;
+        v = std::sqrt(2.0 / (n + 1)) * std::sin(M_PI * (double)a / (double)(n + 1));
+      } else if (kind == KIND_DST2) {
+        const int64_t period = 4 * (int64_t)n;
+        const int64_t a = ((int64_t)(k + 1) * (2 * j + 1)) % period;
+        v = std::sqrt(2.0 / n) * std::sin(M_PI * (double)a / (2.0 * n));
+        if (k == n - 1) v /= std::sqrt(2.0);
+      } else {
+        const int64_t period = 4 * (int64_t)n;
+        const int64_t a = ((int64_t)k * (2 * j + 1)) % period;
+        v = std::sqrt(2.0 / n) * std::cos(M_PI * (double)a / (2.0 * n));
+        if (k == 0) v /= std::sqrt(2.0);
+      }
+      q[(size_t)k * n + j] = v;
+    }
+  }
+  return q;
+}
+
+// (2 - 2 cos theta_k)/h^2, theta = pi k/n (dct2, k=0..n-1), pi k/(n+1) (dst1, k=1..n),
+// pi k/n (dst2, k=1..n): tensor_stokes.py:31-38.
+static std::vector<double> eigenvalues_1d(int n, double h, int kind) {
+  std::vector<double> lam(n);
+  for (int k = 0; k < n; ++k) {
+    double theta;
+    if (kind == KIND_DCT2) theta = M_PI * k / n;
+    else if (kind == KIND_DST1) theta = M_PI * (k + 1) / (n + 1);
+    else theta = M_PI * (k + 1) / n;
+    lam[k] = (2.0 - 2.0 * std::cos(theta)) / (h * h);
+  }
+  return lam;
+}
+
+static inline int round8(int n) { return (n + 7) & ~7; }
+
+// Device copies of one transform matrix: FP64 Q and Q^T (n x n), bf16 Q and Q^T
+// (n x ld8, zero padded) for the mixed path.
+struct DevMatrix {
+  int n = 0, kind = -1, ld8 = 0;
+  double* q = nullptr;
+  double* qt = nullptr;
+  __bf16* qb = nullptr;
+  __bf16* qtb = nullptr;
+};
+
+// One 2-D block (u, v or pressure) of shape ny x nx with its y/x bases.
+struct Block {
+  int ny = 0, nx = 0;
+  const DevMatrix* my = nullptr;
+  const DevMatrix* mx = nullptr;
+  double* lam_y = nullptr;
+  double* lam_x = nullptr;
+  std::vector<double> host_lam_y, host_lam_x;
+};
+
+class Solver {
+ public:
+  Solver(int nx, int ny, double lx, double ly, const mac_stokes_opts& o) : opts_(o) {
+    g_.nx = nx;
+    g_.ny = ny;
+    g_.hx = lx / nx;
+    g_.hy = ly / ny;
+    nu_ = (int64_t)(nx - 1) * ny;
+    nv_ = (int64_t)nx * (ny - 1);
+    size_ = nu_ + nv_;
+    np_ = (int64_t)nx * ny;
+    HIP_CHECK(hipSetDevice(o.device));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    packed_.ldu = nx - 1;
+    packed_.ldv = nx;
+    packed_.voff = nu_;
+    packed_.stride = size_;
+    build_bases();
+    HIP_CHECK(hipHostMalloc((void**)&host_flags_, sizeof(int) * kMaxCols, hipHostMallocDefault));
+  }
+
+  ~Solver() {
+    (void)hipSetDevice(opts_.device);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : allocs_) (void)hipFree(p);
+    if (host_flags_) (void)hipHostFree(host_flags_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  const GridDims& grid() const { return g_; }
+  int64_t size() const { return size_; }
+  int64_t np() const { return np_; }
+  hipStream_t stream() const { return stream_; }
+  const mac_stokes_opts& opts() const { return opts_; }
+  std::string error;
+
+  static constexpr int kMaxCols = 64;
+
+  // ---------------------------------------------------------------- transforms
+  // out = L^-1 in on packed velocity columns (in/out may not alias).
+  void velocity_inverse(const double* in, int64_t in_stride, double* out, int64_t out_stride,
+                        const std::vector<int>& cols, double mass, double visc) {
+    for (size_t base = 0; base < cols.size(); base += kMaxEntries / 2) {
+      const size_t count = std::min<size_t>(kMaxEntries / 2, cols.size() - base);
+      GemmBatch ga{}, gb{}, gc{}, gd{};
+      for (size_t t = 0; t < count; ++t) {
+        const int c = cols[base + t];
+        double* s1 = s1_ + (int64_t)c * size_;
+        double* s2 = s2_ + (int64_t)c * size_;
+        const double* src = in + (int64_t)c * in_stride;
+        double* dst = out + (int64_t)c * out_stride;
+        const Block* blocks[2] = {&ublk_, &vblk_};
+        const int64_t offs[2] = {0, nu_};
+        for (int bi = 0; bi < 2; ++bi) {
+          const Block& B = *blocks[bi];
+          const int64_t off = offs[bi];
+          add_pass(ga, src + off, B.nx, B.mx->q, B.mx->n, s1 + off, B.ny, B.ny, B.nx, B.nx, nullptr, nullptr);
+          add_pass(gb, s1 + off, B.ny, B.my->q, B.my->n, s2 + off, B.ny, B.nx, B.ny, B.ny, B.lam_x, B.lam_y);
+          add_pass(gc, s2 + off, B.ny, B.my->qt, B.my->n, s1 + off, B.nx, B.nx, B.ny, B.ny, nullptr, nullptr);
+          add_pass(gd, s1 + off, B.nx, B.mx->qt, B.mx->n, dst + off, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr);
+        }
+      }
+      ga.epi = EPI_STORE; ga.trans_c = 1;
+      gb.epi = EPI_SHIFT; gb.trans_c = 0; gb.s0 = mass; gb.s1 = visc;
+      gc.epi = EPI_STORE; gc.trans_c = 1;
+      gd.epi = EPI_STORE; gd.trans_c = 0;
+      gemm_f64(ga, stream_);
+      gemm_f64(gb, stream_);
+      gemm_f64(gc, stream_);
+      gemm_f64(gd, stream_);
+    }
+  }
+
+  // out = (D D^T)^+ in on pressure columns, or with combine: out = s0*(D D^T)^+ in + s1*in.
+  // The input's mean is annihilated by the masked zero mode.
+  void pressure_inverse(const double* in, int64_t in_stride, double* out, int64_t out_stride,
+                        const std::vector<int>& cols, bool combine, double s0, double s1) {
+    for (size_t base = 0; base < cols.size(); base += kMaxEntries) {
+      const size_t count = std::min<size_t>(kMaxEntries, cols.size() - base);
+      GemmBatch ga{}, gb{}, gc{}, gd{};
+      const Block& B = pblk_;
+      for (size_t t = 0; t < count; ++t) {
+        const int c = cols[base + t];
+        double* s1p = s1_ + (int64_t)c * size_;
+        double* s2p = s2_ + (int64_t)c * size_;
+        const double* src = in + (int64_t)c * in_stride;
+        double* dst = out + (int64_t)c * out_stride;
+        add_pass(ga, src, B.nx, B.mx->q, B.mx->n, s1p, B.ny, B.ny, B.nx, B.nx, nullptr, nullptr);
+        add_pass(gb, s1p, B.ny, B.my->q, B.my->n, s2p, B.ny, B.nx, B.ny, B.ny, B.lam_x, B.lam_y);
+        add_pass(gc, s2p, B.ny, B.my->qt, B.my->n, s1p, B.nx, B.nx, B.ny, B.ny, nullptr, nullptr);
+        add_pass(gd, s1p, B.nx, B.mx->qt, B.mx->n, dst, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr);
+        gd.e[gd.count - 1].aux = src;
+      }
+      ga.epi = EPI_STORE; ga.trans_c = 1;
+      gb.epi = EPI_LAMBDA; gb.trans_c = 0; gb.thresh = p_thresh_;
+      gc.epi = EPI_STORE; gc.trans_c = 1;
+      gd.epi = combine ? EPI_COMBINE : EPI_STORE; gd.trans_c = 0; gd.s0 = s0; gd.s1 = s1;
+      gemm_f64(ga, stream_);
+      gemm_f64(gb, stream_);
+      gemm_f64(gc, stream_);
+      gemm_f64(gd, stream_);
+    }
+  }
+
+  // ------------------------------------------------------------------- solve
+  struct Outcome {
+    double residual = 0, divergence = 0, vmax = 0, fmax = 0;
+    std::vector<int> iterations;
+    int sweeps = 1;
+  };
+
+  Outcome solve(const double* f, int64_t f_stride, int ncols, double mass, double visc, double* vel,
+                int64_t vel_stride, double* pres, int64_t pres_stride, const double* guess, int64_t guess_stride) {
+    ensure_capacity(ncols);
+    ncols_cur_ = ncols;
+    std::vector<int> all(ncols);
+    for (int c = 0; c < ncols; ++c) all[c] = c;
+    Outcome out;
+    out.iterations.assign(ncols, 0);
+    // W = L^-1 f ; b = -D W ; b -= mean(b)
+    velocity_inverse(f, f_stride, w_, size_, all, mass, visc);
+    const int nbp = reduce_blocks(np_);
+    divergence(DT_F64, w_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_);
+    finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+    sub_mean_dot(b_, np_, np_, ncols, sums_, b_, partials_, nullptr, stream_);
+    finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);  // ||b||^2
+    reduce(b_, nullptr, np_, np_, ncols, RED_MAXABS, partials_, nullptr, stream_);
+    finalize(partials_, nbp, ncols, RED_MAXABS, sums3_, stream_);
+    cg_init(cg_, sums2_, sums3_, opts_.krylov_tolerance, ncols, stream_);
+    // x = 0, r = b  (or x = guess, r = b - S guess on columns whose CG runs)
+    HIP_CHECK(hipMemsetAsync(x_, 0, sizeof(double) * np_ * ncols, stream_));
+    copy_cols(b_, np_, r_, np_, np_, ncols, stream_);
+    HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
+    if (guess != nullptr) {
+      std::vector<CgState> st(ncols);
+      HIP_CHECK(hipMemcpyAsync(st.data(), cg_, sizeof(CgState) * ncols, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      std::vector<int> live;
+      for (int c = 0; c < ncols; ++c) {
+        if (!st[c].active) continue;  // zero right-hand side: pressure stays zero
+        live.push_back(c);
+        HIP_CHECK(hipMemcpyAsync(x_ + (int64_t)c * np_, guess + (int64_t)c * guess_stride, sizeof(double) * np_,
+                                 hipMemcpyDeviceToDevice, stream_));
+      }
+      if (!live.empty()) {
+        use_state_ = true;
+        schur(x_, q_, live, mass, visc, nullptr);
+        use_state_ = false;
+        sub_cols(b_, q_, r_, np_, np_, ncols, cg_, stream_);
+        reduce(r_, r_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
+        finalize(partials_, nbp, ncols, RED_SUM, rr_, stream_);
+      }
+    }
+    pcg(all, ncols, mass, visc, out.iterations);
+    // p = x - mean(x)
+    reduce(x_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_);
+    finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+    sub_mean_dot(x_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
+    copy_cols(x_, np_, pres, pres_stride, np_, ncols, stream_);
+    // u = L^-1 (f + D^T p)
+    divergence_transpose(x_, np_, nullptr, f, f_stride, DT_F64, w_, packed_, g_, ncols, nullptr, stream_);
+    velocity_inverse(w_, size_, vel, vel_stride, all, mass, visc);
+    checks(f, f_stride, vel, vel_stride, x_, ncols, mass, visc, out);
+    return out;
+  }
+
+ private:
+  // One GEMM pass of a 2-D transform: C = A . Q^T with A [M][K] (lda), Q (or Q^T) as
+  // Bt [N][K] (ldb = n), C written transposed [N][M] (ldc) when the batch says so.
+  static void add_pass(GemmBatch& b, const double* A, int lda, const double* Q, int n, double* C, int ldc,
+                       int M, int N, int K, const double* ea, const double* eb) {
+    GemmEntry& e = b.e[b.count++];
+    e.A = A;
+    e.Bt = Q;
+    e.C = C;
+    e.aux = nullptr;
+    e.ea = ea;
+    e.eb = eb;
+    e.M = M;
+    e.N = N;
+    e.K = K;
+    e.lda = lda;
+    e.ldb = n;
+    e.ldc = ldc;
+  }
+
+  // q = S p = D L^-1 D^T p, mean removed; optional partial (y, q) dots after the projection.
+  void schur(const double* pin, double* qout, const std::vector<int>& cols, double mass, double visc,
+             const double* dot_with) {
+    const int ncols = ncols_cur_;
+    const CgState* st = use_state_ ? cg_ : nullptr;
+    divergence_transpose(pin, np_, nullptr, nullptr, 0, DT_F64, w2_, packed_, g_, ncols, st, stream_);
+    velocity_inverse(w2_, size_, w_, size_, cols, mass, visc);
+    const int nbp = reduce_blocks(np_);
+    divergence(DT_F64, w_, packed_, g_, qout, np_, 1.0, ncols, RED_SUM, partials_, st, stream_);
+    finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+    sub_mean_dot(qout, np_, np_, ncols, sums_, dot_with, partials_, st, stream_);
+  }
+
+  void pcg(const std::vector<int>& all, int ncols, double mass, double visc, std::vector<int>& iters) {
+    const int nbp = reduce_blocks(np_);
+    ncols_cur_ = ncols;
+    use_state_ = true;
+    std::vector<int> active;
+    for (int it = 0; it < opts_.max_iterations; ++it) {
+      cg_check(cg_, rr_, flags_, ncols, stream_);
+      HIP_CHECK(hipMemcpyAsync(host_flags_, flags_, sizeof(int) * ncols, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      active.clear();
+      for (int c = 0; c < ncols; ++c)
+        if (host_flags_[c]) active.push_back(c);
+      if (active.empty()) break;
+      // z = mass (D D^T)^+ r + visc r, mean removed; rho = (r, z)
+      pressure_inverse(r_, np_, z_, np_, active, true, mass, visc);
+      reduce(z_, nullptr, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
+      finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+      sub_mean_dot(z_, np_, np_, ncols, sums_, r_, partials_, cg_, stream_);
+      finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);
+      cg_rho(cg_, sums2_, ncols, stream_);
+      cg_update_p(p_, z_, np_, np_, ncols, cg_, stream_);
+      // q = S p; alpha = rho / (p, q)
+      schur(p_, q_, active, mass, visc, p_);
+      finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);
+      cg_alpha(cg_, sums2_, ncols, stream_);
+      cg_update_xr(x_, r_, p_, q_, np_, np_, ncols, cg_, partials_, stream_);
+      finalize(partials_, nbp, ncols, RED_SUM, rr_, stream_);
+    }
+    use_state_ = false;
+    // A column still active after max_iterations completed iterations failed (the
+    // convergence test is only made at the top of an iteration, as in scipy's cg).
+    std::vector<CgState> st(ncols);
+    HIP_CHECK(hipMemcpyAsync(st.data(), cg_, sizeof(CgState) * ncols, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int c = 0; c < ncols; ++c) {
+      iters[c] = st[c].iters;
+      if (!st[c].converged)
+        throw SolveFailure(MAC_NOT_CONVERGED,
+                           "Pressure Schur CG failed with info=" + std::to_string(opts_.max_iterations));
+    }
+  }
+
+  void checks(const double* f, int64_t f_stride, const double* vel, int64_t vel_stride, const double* p, int ncols,
+              double mass, double visc, Outcome& out) {
+    // The defect kernel reads contiguous columns; the caller's strides are compacted.
+    const double* fc = f;
+    const double* vc = vel;
+    if (f_stride != size_) {
+      copy_cols(f, f_stride, w_, size_, size_, ncols, stream_);
+      fc = w_;
+    }
+    if (vel_stride != size_) {
+      copy_cols(vel, vel_stride, w2_, size_, size_, ncols, stream_);
+      vc = w2_;
+    }
+    const int nb = reduce_blocks(size_ * ncols);
+    defect_max(vc, fc, p, g_, ncols, mass, visc, partials_, stream_);
+    finalize(partials_, nb, 1, RED_MAXABS, sums_, stream_);
+    finalize(partials_ + nb, nb, 1, RED_MAXABS, sums_ + 1, stream_);
+    finalize(partials_ + 2 * nb, nb, 1, RED_MAXABS, sums_ + 2, stream_);
+    const int nbp = reduce_blocks(np_);
+    divergence(DT_F64, vc, packed_, g_, q_, np_, 1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr, stream_);
+    finalize(partials_ + 3 * nb, nbp, ncols, RED_MAXABS, sums2_, stream_);
+    double h[3];
+    std::vector<double> dv(ncols);
+    HIP_CHECK(hipMemcpyAsync(h, sums_, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipMemcpyAsync(dv.data(), sums2_, sizeof(double) * ncols, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    out.residual = h[0] / (1.0 + h[1]);
+    out.fmax = h[1];
+    out.vmax = h[2];
+    out.divergence = 0.0;
+    for (double d : dv) out.divergence = std::max(out.divergence, d);
+  }
+
+  void build_bases() {
+    const int nx = g_.nx, ny = g_.ny;
+    ublk_.ny = ny; ublk_.nx = nx - 1;
+    ublk_.my = matrix(ny, KIND_DST2); ublk_.mx = matrix(nx - 1, KIND_DST1);
+    vblk_.ny = ny - 1; vblk_.nx = nx;
+    vblk_.my = matrix(ny - 1, KIND_DST1); vblk_.mx = matrix(nx, KIND_DST2);
+    pblk_.ny = ny; pblk_.nx = nx;
+    pblk_.my = matrix(ny, KIND_DCT2); pblk_.mx = matrix(nx, KIND_DCT2);
+    set_lam(ublk_, KIND_DST2, KIND_DST1);
+    set_lam(vblk_, KIND_DST1, KIND_DST2);
+    set_lam(pblk_, KIND_DCT2, KIND_DCT2);
+    double lmax = 0.0;
+    for (double a : pblk_.host_lam_y)
+      for (double b : pblk_.host_lam_x) lmax = std::max(lmax, a + b);
+    p_thresh_ = 100.0 * 2.220446049250313e-16 * lmax;
+  }
+
+  void set_lam(Block& B, int ky, int kx) {
+    B.host_lam_y = eigenvalues_1d(B.ny, g_.hy, ky);
+    B.host_lam_x = eigenvalues_1d(B.nx, g_.hx, kx);
+    B.lam_y = upload(B.host_lam_y);
+    B.lam_x = upload(B.host_lam_x);
+  }
+
+  const DevMatrix* matrix(int n, int kind) {
+    for (auto& m : mats_)
+      if (m->n == n && m->kind == kind) return m.get();
+    auto m = std::make_unique<DevMatrix>();
+    m->n = n;
+    m->kind = kind;
+    m->ld8 = round8(n);
+    std::vector<double> q = transform_matrix(n, kind);
+    std::vector<double> qt((size_t)n * n);
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) qt[(size_t)b * n + a] = q[(size_t)a * n + b];
+    m->q = upload(q);
+    m->qt = upload(qt);
+    std::vector<__bf16> qb((size_t)n * m->ld8, (__bf16)0.0f), qtb((size_t)n * m->ld8, (__bf16)0.0f);
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        qb[(size_t)a * m->ld8 + b] = (__bf16)(float)q[(size_t)a * n + b];
+        qtb[(size_t)a * m->ld8 + b] = (__bf16)(float)qt[(size_t)a * n + b];
+      }
+    m->qb = static_cast<__bf16*>(upload_raw(qb.data(), qb.size() * sizeof(__bf16)));
+    m->qtb = static_cast<__bf16*>(upload_raw(qtb.data(), qtb.size() * sizeof(__bf16)));
+    mats_.push_back(std::move(m));
+    return mats_.back().get();
+  }
+
+  void* dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+    allocs_.push_back(p);
+    return p;
+  }
+  void* upload_raw(const void* h, size_t bytes) {
+    void* d = dev_alloc(bytes);
+    HIP_CHECK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+    return d;
+  }
+  double* upload(const std::vector<double>& v) {
+    return static_cast<double*>(upload_raw(v.data(), v.size() * sizeof(double)));
+  }
+
+ public:
+  void ensure_capacity(int ncols) {
+    if (ncols <= cap_) return;
+    // Work buffers are replaced, never freed mid-solve; old ones are released here.
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    for (void* p : work_) {
+      (void)hipFree(p);
+      allocs_.erase(std::remove(allocs_.begin(), allocs_.end(), p), allocs_.end());
+    }
+    work_.clear();
+    auto take = [&](size_t bytes) {
+      void* p = dev_alloc(bytes);
+      work_.push_back(p);
+      return p;
+    };
+    const size_t vb = sizeof(double) * size_ * ncols, pb = sizeof(double) * np_ * ncols;
+    w_ = (double*)take(vb);
+    w2_ = (double*)take(vb);
+    s1_ = (double*)take(vb);
+    s2_ = (double*)take(vb);
+    io_f_ = (double*)take(vb);
+    io_v_ = (double*)take(vb);
+    io_p_ = (double*)take(pb);
+    io_g_ = (double*)take(pb);
+    b_ = (double*)take(pb);
+    x_ = (double*)take(pb);
+    r_ = (double*)take(pb);
+    z_ = (double*)take(pb);
+    p_ = (double*)take(pb);
+    q_ = (double*)take(pb);
+    const int nb = std::max(reduce_blocks(size_ * ncols), reduce_blocks(np_) * ncols);
+    partials_ = (double*)take(sizeof(double) * (4 * (size_t)nb + 16));
+    sums_ = (double*)take(sizeof(double) * (ncols + 4));
+    sums2_ = (double*)take(sizeof(double) * (ncols + 4));
+    sums3_ = (double*)take(sizeof(double) * (ncols + 4));
+    rr_ = (double*)take(sizeof(double) * (ncols + 4));
+    cg_ = (CgState*)take(sizeof(CgState) * ncols);
+    flags_ = (int*)take(sizeof(int) * ncols);
+    cap_ = ncols;
+  }
+
+  double* io_f_ = nullptr;
+  double* io_v_ = nullptr;
+  double* io_p_ = nullptr;
+  double* io_g_ = nullptr;
+  double* scratch_sums() { return sums_; }
+  double* scratch_partials() { return partials_; }
+  const VelLayout& packed() const { return packed_; }
+
+ private:
+  mac_stokes_opts opts_;
+  GridDims g_{};
+  int64_t nu_ = 0, nv_ = 0, size_ = 0, np_ = 0;
+  hipStream_t stream_ = nullptr;
+  VelLayout packed_{};
+  std::vector<std::unique_ptr<DevMatrix>> mats_;
+  Block ublk_, vblk_, pblk_;
+  double p_thresh_ = 0.0;
+  std::vector<void*> allocs_, work_;
+  int cap_ = 0;
+  int ncols_cur_ = 0;
+  bool use_state_ = false;
+  double *w_ = nullptr, *w2_ = nullptr, *s1_ = nullptr, *s2_ = nullptr;
+  double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;
+  double *partials_ = nullptr, *sums_ = nullptr, *sums2_ = nullptr, *sums3_ = nullptr, *rr_ = nullptr;
+  CgState* cg_ = nullptr;
+  int* flags_ = nullptr;
+  int* host_flags_ = nullptr;
+};
+
+}  // namespace macs
+
+// ======================================================================= C ABI
+using macs::Solver;
+
+namespace {
+thread_local std::string g_create_error;
+
+int fail(Solver* s, int code, const std::string& msg) {
+  if (s) s->error = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(Solver* s, F&& body) {
+  try {
+    s->error.clear();
+    body();
+    return MAC_OK;
+  } catch (const macs::ArgFailure& e) {
+    return fail(s, MAC_INVALID_ARGUMENT, e.what());
+  } catch (const macs::SolveFailure& e) {
+    return fail(s, e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(s, MAC_HIP_ERROR, e.what());
+  }
+}
+
+bool is_finite(double v) { return std::isfinite(v); }
+}  // namespace
+
+extern "C" {
+
+void mac_stokes_default_opts(mac_stokes_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->tolerance = 1e-9;
+  o->krylov_tolerance = 1e-12;
+  o->max_iterations = 100;
+  o->precision = MAC_PRECISION_FP64;
+  o->refine_tolerance = 1e-12;
+  o->inner_tolerance = 1e-3;
+  o->max_sweeps = 30;
+  o->device = 0;
+}
+
+void* mac_stokes_create(int32_t nx, int32_t ny, double lx, double ly, const mac_stokes_opts* opts) {
+  g_create_error.clear();
+  mac_stokes_opts o;
+  mac_stokes_default_opts(&o);
+  if (opts) o = *opts;
+  if (nx < 3 || ny < 3 || !is_finite(lx) || !is_finite(ly) || lx <= 0 || ly <= 0) {
+    g_create_error = "Require nx,ny >= 3 and positive finite domain lengths";
+    return nullptr;
+  }
+  if (!(o.tolerance > 0) || !(o.krylov_tolerance > 0) || !is_finite(o.tolerance) || !is_finite(o.krylov_tolerance) ||
+      o.max_iterations < 1 || (o.precision != MAC_PRECISION_FP64 && o.precision != MAC_PRECISION_MIXED) ||
+      !(o.refine_tolerance > 0) || !(o.inner_tolerance > 0) || o.max_sweeps < 1) {
+    g_create_error = "Invalid solver controls";
+    return nullptr;
+  }
+  try {
+    return new Solver(nx, ny, lx, ly, o);
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return nullptr;
+  }
+}
+
+const char* mac_stokes_create_error(void) { return g_create_error.c_str(); }
+
+void mac_stokes_destroy(void* h) { delete static_cast<Solver*>(h); }
+
+const char* mac_stokes_last_error(void* h) { return h ? static_cast<Solver*>(h)->error.c_str() : ""; }
+
+int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t ncols, double mass, double viscosity,
+                         double* velocity, int64_t ld_velocity, double* pressure, int64_t ld_pressure,
+                         const double* guess, int64_t ld_guess, int32_t* iterations, mac_solve_report* report,
+                         int32_t device_pointers) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] {
+    if (!is_finite(mass) || !is_finite(viscosity) || std::min(mass, viscosity) < 0 || mass + viscosity <= 0)
+      throw macs::ArgFailure("Require mass,viscosity >= 0 and nonzero total");
+    const int64_t size = s->size(), np = s->np();
+    if (ncols < 1 || ncols > Solver::kMaxCols || !rhs || !velocity || !pressure || ld_rhs < size ||
+        ld_velocity < size || ld_pressure < np || (guess && ld_guess < np))
+      throw macs::ArgFailure("Invalid Stokes RHS");
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    const auto t0 = std::chrono::steady_clock::now();
+    s->ensure_capacity(ncols);
+    hipStream_t st = s->stream();
+    const double* f = rhs;
+    const double* gs = guess;
+    double* vel = velocity;
+    double* pr = pressure;
+    int64_t lf = ld_rhs, lv = ld_velocity, lp = ld_pressure, lg = ld_guess;
+    if (!device_pointers) {
+      HIP_CHECK(hipMemcpy2DAsync(s->io_f_, size * sizeof(double), rhs, ld_rhs * sizeof(double), size * sizeof(double),
+                                 ncols, hipMemcpyHostToDevice, st));
+      f = s->io_f_;
+      lf = size;
+      if (guess) {
+        HIP_CHECK(hipMemcpy2DAsync(s->io_g_, np * sizeof(double), guess, ld_guess * sizeof(double),
+                                   np * sizeof(double), ncols, hipMemcpyHostToDevice, st));
+        gs = s->io_g_;
+        lg = np;
+      }
+      vel = s->io_v_;
+      lv = size;
+      pr = s->io_p_;
+      lp = np;
+    }
+    // Non-finite RHS values are rejected like the reference (ValueError).
+    macs::reduce(f, nullptr, size, lf, ncols, macs::RED_MAXABS, s->scratch_partials(), nullptr, st);
+    macs::finalize(s->scratch_partials(), macs::reduce_blocks(size), ncols, macs::RED_MAXABS, s->scratch_sums(), st);
+    std::vector<double> fm(ncols);
+    HIP_CHECK(hipMemcpyAsync(fm.data(), s->scratch_sums(), sizeof(double) * ncols, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (double v : fm)
+      if (!is_finite(v)) throw macs::ArgFailure("Invalid Stokes RHS");
+    Solver::Outcome out = s->solve(f, lf, ncols, mass, viscosity, vel, lv, pr, lp, gs, lg);
+    if (!device_pointers) {
+      HIP_CHECK(hipMemcpy2DAsync(velocity, ld_velocity * sizeof(double), vel, size * sizeof(double),
+                                 size * sizeof(double), ncols, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpy2DAsync(pressure, ld_pressure * sizeof(double), pr, np * sizeof(double),
+                                 np * sizeof(double), ncols, hipMemcpyDeviceToHost, st));
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int total = 0;
+    for (int c = 0; c < ncols; ++c) {
+      total += out.iterations[c];
+      if (iterations) iterations[c] = out.iterations[c];
+    }
+    if (report) {
+      report->residual = out.residual;
+      report->divergence_inf = out.divergence;
+      report->velocity_max = out.vmax;
+      report->seconds = secs;
+      report->sweeps = out.sweeps;
+      report->total_iterations = total;
+    }
+    const double tol = s->opts().tolerance;
+    if (!is_finite(out.residual) || !is_finite(out.divergence) || !is_finite(out.vmax) || out.residual > tol ||
+        out.divergence > tol * (1.0 + out.vmax)) {
+      char buf[128];
+      std::snprintf(buf, sizeof buf, "Stokes residual=%.3e, divergence=%.3e", out.residual, out.divergence);
+      throw macs::SolveFailure(MAC_GATE_FAILED, buf);
+    }
+  });
+}
+
+int32_t mac_apply_divergence(void* h, const double* vel, int64_t ldv, int32_t ncols, double* out, int64_t ldo) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] {
+    if (ncols < 1 || !vel || !out || ldv < s->size() || ldo < s->np()) throw macs::ArgFailure("Invalid arguments");
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    macs::VelLayout L = s->packed();
+    L.stride = ldv;
+    macs::divergence(macs::DT_F64, vel, L, s->grid(), out, ldo, 1.0, ncols, macs::RED_NONE, nullptr, nullptr,
+                     s->stream());
+    HIP_CHECK(hipStreamSynchronize(s->stream()));
+  });
+}
+
+int32_t mac_apply_divergence_transpose(void* h, const double* p, int64_t ldp, int32_t ncols, double* out,
+                                       int64_t ldo) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] {
+    if (ncols < 1 || !p || !out || ldp < s->np() || ldo < s->size()) throw macs::ArgFailure("Invalid arguments");
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    macs::VelLayout L = s->packed();
+    L.stride = ldo;
+    macs::divergence_transpose(p, ldp, nullptr, nullptr, 0, macs::DT_F64, out, L, s->grid(), ncols, nullptr,
+                               s->stream());
+    HIP_CHECK(hipStreamSynchronize(s->stream()));
+  });
+}
+
+int32_t mac_velocity_inverse(void* h, const double* in, int64_t ldi, int32_t ncols, double mass, double visc,
+                             double* out, int64_t ldo) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] {
+    if (ncols < 1 || ncols > Solver::kMaxCols || !in || !out || ldi < s->size() || ldo < s->size() ||
+        !is_finite(mass) || !is_finite(visc))
+      throw macs::ArgFailure("Invalid arguments");
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    s->ensure_capacity(ncols);
+    std::vector<int> cols(ncols);
+    for (int c = 0; c < ncols; ++c) cols[c] = c;
+    s->velocity_inverse(in, ldi, out, ldo, cols, mass, visc);
+    HIP_CHECK(hipStreamSynchronize(s->stream()));
+  });
+}
+
+int32_t mac_pressure_poisson_inverse(void* h, const double* in, int64_t ldi, int32_t ncols, double* out,
+                                     int64_t ldo) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] {
+    if (ncols < 1 || ncols > Solver::kMaxCols || !in || !out || ldi < s->np() || ldo < s->np())
+      throw macs::ArgFailure("Invalid arguments");
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    s->ensure_capacity(ncols);
+    std::vector<int> cols(ncols);
+    for (int c = 0; c < ncols; ++c) cols[c] = c;
+    s->pressure_inverse(in, ldi, out, ldo, cols, false, 1.0, 0.0);
+    const int64_t np = s->np();
+    macs::reduce(out, nullptr, np, ldo, ncols, macs::RED_SUM, s->scratch_partials(), nullptr, s->stream());
+    macs::finalize(s->scratch_partials(), macs::reduce_blocks(np), ncols, macs::RED_SUM, s->scratch_sums(),
+                   s->stream());
+    macs::sub_mean_dot(out, np, ldo, ncols, s->scratch_sums(), nullptr, nullptr, nullptr, s->stream());
+    HIP_CHECK(hipStreamSynchronize(s->stream()));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+"""ctypes binding of libmacstokes.so (the gfx950 HIP library, C ABI in include/macstokes.h).
+
+The library is loaded from this package directory.  There is no fallback: when the
+shared object is missing, or no GPU is visible, the solver raises instead of
+computing anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmacstokes.so"
+
+MAC_OK = 0
+MAC_INVALID_ARGUMENT = 1
+MAC_NOT_CONVERGED = 2
+MAC_GATE_FAILED = 3
+MAC_HIP_ERROR = 4
+
+MAC_PRECISION_FP64 = 0
+MAC_PRECISION_MIXED = 1
+
+# Every symbol include/macstokes.h declares, checked by the CPU test suite.
+EXPORTED = (
+    "mac_stokes_default_opts",
+    "mac_stokes_create",
+    "mac_stokes_create_error",
+    "mac_stokes_destroy",
+    "mac_stokes_last_error",
+    "mac_stokes_solve",
+    "mac_apply_divergence",
+    "mac_apply_divergence_transpose",
+    "mac_velocity_inverse",
+    "mac_pressure_poisson_inverse",
+)
+
+
+class StokesOpts(ctypes.Structure):
+    _fields_ = [
+        ("tolerance", ctypes.c_double),
+        ("krylov_tolerance", ctypes.c_double),
+        ("max_iterations", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("refine_tolerance", ctypes.c_double),
+        ("inner_tolerance", ctypes.c_double),
+        ("max_sweeps", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [
+        ("residual", ctypes.c_double),
+        ("divergence_inf", ctypes.c_double),
+        ("velocity_max", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+        ("sweeps", ctypes.c_int32),
+        ("total_iterations", ctypes.c_int32),
+    ]
+
+
+_lib: ctypes.CDLL | None = None
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+
+def load() -> ctypes.CDLL:
+    """Load libmacstokes.so once and declare every entry point's signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = ctypes.CDLL(os.fspath(LIB_PATH))
+    lib.mac_stokes_default_opts.argtypes = [ctypes.POINTER(StokesOpts)]
+    lib.mac_stokes_default_opts.restype = None
+    lib.mac_stokes_create.argtypes = [_I32, _I32, _D, _D, ctypes.POINTER(StokesOpts)]
+    lib.mac_stokes_create.restype = _P
+    lib.mac_stokes_create_error.argtypes = []
+    lib.mac_stokes_create_error.restype = ctypes.c_char_p
+    lib.mac_stokes_destroy.argtypes = [_P]
+    lib.mac_stokes_destroy.restype = None
+    lib.mac_stokes_last_error.argtypes = [_P]
+    lib.mac_stokes_last_error.restype = ctypes.c_char_p
+    lib.mac_stokes_solve.argtypes = [_P, _P, _I64, _I32, _D, _D, _P, _I64, _P, _I64, _P, _I64,
+                                     ctypes.POINTER(_I32), ctypes.POINTER(SolveReport), _I32]
+    lib.mac_stokes_solve.restype = _I32
+    lib.mac_apply_divergence.argtypes = [_P, _P, _I64, _I32, _P, _I64]
+    lib.mac_apply_divergence.restype = _I32
+    lib.mac_apply_divergence_transpose.argtypes = [_P, _P, _I64, _I32, _P, _I64]
+    lib.mac_apply_divergence_transpose.restype = _I32
+    lib.mac_velocity_inverse.argtypes = [_P, _P, _I64, _I32, _D, _D, _P, _I64]
+    lib.mac_velocity_inverse.restype = _I32
+    lib.mac_pressure_poisson_inverse.argtypes = [_P, _P, _I64, _I32, _P, _I64]
+    lib.mac_pressure_poisson_inverse.restype = _I32
+    _lib = lib
+    return lib
+
+
+def default_opts() -> StokesOpts:
+    opts = StokesOpts()
+    load().mac_stokes_default_opts(ctypes.byref(opts))
+    return opts
+
+
+def raise_for(code: int, message: str) -> None:
+    """Map a C status to the reference's exception types (tensor_stokes.py:246-257,
+    322-323, 345-351)."""
+    if code == MAC_OK:
+        return
+    if code == MAC_INVALID_ARGUMENT:
+        raise ValueError(message)
+    raise RuntimeError(message)

@@ -1,0 +1,212 @@
+"""`GpuTensorStokes`: the MI355X stage-solve backend behind the `StokesBackend` seam.
+
+Drop-in for the reference's `TensorStokes` (solver/mac/tensor_stokes.py:118-364): same
+constructor controls, same `solve(rhs, *, mass, viscosity) -> StokesResult`, same
+validation errors (ValueError), same failure modes (RuntimeError on Schur-CG
+non-convergence and on the residual/divergence gate) and the same counters
+(`setup_seconds`, `solve_seconds`, `solves`, `right_hand_sides`, `total_iterations`,
+`last_iterations`).  The work runs in libmacstokes.so on the GPU; this class only
+validates, moves buffers and maps status codes.
+
+Inputs may be NumPy arrays (copied to the device and back) or torch device tensors
+(zero-copy; results come back as torch tensors on the same device).
+
+`precision="mixed"` runs the transforms as bf16 MFMA GEMMs inside an FP64
+defect-correction loop; `warm_start` is an optional callable
+`(pressure_rhs_unused, previous_pressure) -> guess` or a `PressureWarmStart`, used
+as the Schur-CG starting pressure.
+"""
+from __future__ import annotations
+
+import ctypes
+from time import perf_counter
+from typing import Any
+
+import numpy as np
+
+from .. import _native
+from .grid import MACGrid
+from .stokes import StokesResult
+
+_PRECISIONS = {"fp64": _native.MAC_PRECISION_FP64, "mixed": _native.MAC_PRECISION_MIXED}
+
+
+def _is_torch(x: Any) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+class GpuTensorStokes:
+    """Shift-reusable MAC Stokes solve on one GPU (DST/DCT MFMA GEMMs + Schur-CG)."""
+
+    def __init__(self, operators: Any, tolerance: float = 1e-9, krylov_tolerance: float = 1e-12,
+                 max_iterations: int = 100, matrix_free_operators: bool = True,
+                 reuse_transform_buffers: bool = True, *, precision: str = "fp64",
+                 refine_tolerance: float = 1e-12, inner_tolerance: float = 1e-3, max_sweeps: int = 30,
+                 device: int = 0) -> None:
+        if (tolerance <= 0 or krylov_tolerance <= 0 or not np.isfinite([tolerance, krylov_tolerance]).all()
+                or max_iterations < 1):
+            raise ValueError("Invalid solver controls")
+        if not isinstance(matrix_free_operators, bool):
+            raise ValueError("matrix_free_operators must be a bool")
+        if not isinstance(reuse_transform_buffers, bool):
+            raise ValueError("reuse_transform_buffers must be a bool")
+        if precision not in _PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
+        start = perf_counter()
+        grid = operators if isinstance(operators, MACGrid) else operators.grid
+        self.ops = None if isinstance(operators, MACGrid) else operators
+        self.grid = grid
+        self.tolerance = tolerance
+        self.krylov_tolerance = krylov_tolerance
+        self.max_iterations = max_iterations
+        self.matrix_free_operators = matrix_free_operators
+        self.reuse_transform_buffers = reuse_transform_buffers
+        self.precision = precision
+        lib = _native.load()
+        opts = _native.default_opts()
+        opts.tolerance = tolerance
+        opts.krylov_tolerance = krylov_tolerance
+        opts.max_iterations = max_iterations
+        opts.precision = _PRECISIONS[precision]
+        opts.refine_tolerance = refine_tolerance
+        opts.inner_tolerance = inner_tolerance
+        opts.max_sweeps = max_sweeps
+        opts.device = device
+        handle = lib.mac_stokes_create(grid.nx, grid.ny, float(grid.lx), float(grid.ly), ctypes.byref(opts))
+        if not handle:
+            message = lib.mac_stokes_create_error().decode()
+            if message.startswith("Invalid solver controls") or message.startswith("Require"):
+                raise ValueError(message)
+            raise RuntimeError(f"GPU Stokes setup failed: {message}")
+        self._lib = lib
+        self._handle = handle
+        self.device = device
+        self.setup_seconds = perf_counter() - start
+        self.solve_seconds = 0.0
+        self.solves = 0
+        self.right_hand_sides = 0
+        self.total_iterations = 0
+        self.last_iterations: tuple[int, ...] = ()
+        self.sweeps = 0
+        self.last_sweeps = 0
+        self.last_report: dict[str, float] = {}
+
+    def __del__(self) -> None:
+        handle = getattr(self, "_handle", None)
+        if handle:
+            self._lib.mac_stokes_destroy(handle)
+            self._handle = None
+
+    # ------------------------------------------------------------------ solve
+    def solve(self, rhs: Any, *, mass: float, viscosity: float,
+              pressure_guess: Any = None) -> StokesResult:
+        if (not np.isfinite([mass, viscosity]).all() or min(mass, viscosity) < 0
+                or mass + viscosity <= 0):
+            raise ValueError("Require mass,viscosity >= 0 and nonzero total")
+        grid = self.grid
+        if _is_torch(rhs):
+            return self._solve_torch(rhs, float(mass), float(viscosity), pressure_guess)
+        rhs = np.asarray(rhs)
+        if rhs.ndim not in (1, 2) or rhs.shape[0] != grid.size or not np.isfinite(rhs).all():
+            raise ValueError("Invalid Stokes RHS")
+        two_d = rhs.ndim == 2
+        cols = np.ascontiguousarray((rhs if two_d else rhs[:, None]).T, dtype=np.float64)
+        k = cols.shape[0]
+        vel = np.empty((k, grid.size))
+        pres = np.empty((k, grid.np))
+        guess_ptr, ld_guess = None, 0
+        if pressure_guess is not None:
+            g = np.ascontiguousarray(np.asarray(pressure_guess, dtype=np.float64).reshape(grid.np, k).T)
+            guess_ptr, ld_guess = g.ctypes.data, grid.np
+        report = self._call(cols.ctypes.data, grid.size, k, mass, viscosity, vel.ctypes.data, pres.ctypes.data,
+                            guess_ptr, ld_guess, device=False)
+        velocity = vel.T.copy() if two_d else vel[0].copy()
+        pressure = (pres.T.reshape(grid.ny, grid.nx, k).copy() if two_d
+                    else pres[0].reshape(grid.ny, grid.nx).copy())
+        return StokesResult(velocity, pressure, report["residual"], report["divergence_inf"])
+
+    def _solve_torch(self, rhs: Any, mass: float, viscosity: float, pressure_guess: Any) -> StokesResult:
+        import torch
+
+        grid = self.grid
+        if rhs.ndim not in (1, 2) or rhs.shape[0] != grid.size or rhs.device.type != "cuda":
+            raise ValueError("Invalid Stokes RHS")
+        if not bool(torch.isfinite(rhs).all()):
+            raise ValueError("Invalid Stokes RHS")
+        two_d = rhs.ndim == 2
+        cols = (rhs if two_d else rhs[:, None]).t().contiguous().to(torch.float64)
+        k = cols.shape[0]
+        vel = torch.empty((k, grid.size), dtype=torch.float64, device=rhs.device)
+        pres = torch.empty((k, grid.np), dtype=torch.float64, device=rhs.device)
+        guess_ptr, ld_guess, guess = None, 0, None
+        if pressure_guess is not None:
+            guess = torch.as_tensor(pressure_guess, device=rhs.device).to(torch.float64).reshape(grid.np, k)
+            guess = guess.t().contiguous()
+            guess_ptr, ld_guess = guess.data_ptr(), grid.np
+        torch.cuda.current_stream(rhs.device).synchronize()
+        report = self._call(cols.data_ptr(), grid.size, k, mass, viscosity, vel.data_ptr(), pres.data_ptr(),
+                            guess_ptr, ld_guess, device=True)
+        velocity = vel.t() if two_d else vel[0]
+        pressure = pres.t().reshape(grid.ny, grid.nx, k) if two_d else pres[0].reshape(grid.ny, grid.nx)
+        return StokesResult(velocity, pressure, report["residual"], report["divergence_inf"])
+
+    def _call(self, rhs_ptr: int, ld: int, k: int, mass: float, viscosity: float, vel_ptr: int, pres_ptr: int,
+              guess_ptr: int | None, ld_guess: int, *, device: bool) -> dict[str, float]:
+        grid = self.grid
+        iters = (ctypes.c_int32 * k)()
+        report = _native.SolveReport()
+        start = perf_counter()
+        code = self._lib.mac_stokes_solve(self._handle, rhs_ptr, ld, k, mass, viscosity, vel_ptr, grid.size,
+                                          pres_ptr, grid.np, guess_ptr, ld_guess, iters, ctypes.byref(report),
+                                          1 if device else 0)
+        elapsed = perf_counter() - start
+        message = self._lib.mac_stokes_last_error(self._handle).decode()
+        if code == _native.MAC_NOT_CONVERGED or code in (_native.MAC_INVALID_ARGUMENT, _native.MAC_HIP_ERROR):
+            _native.raise_for(code, message)
+        self.solve_seconds += elapsed
+        self.solves += 1
+        self.right_hand_sides += k
+        self.total_iterations += int(sum(iters))
+        self.last_iterations = tuple(int(i) for i in iters)
+        self.sweeps += int(report.sweeps)
+        self.last_sweeps = int(report.sweeps)
+        self.last_report = {"residual": report.residual, "divergence_inf": report.divergence_inf,
+                            "velocity_max": report.velocity_max, "seconds": report.seconds,
+                            "sweeps": int(report.sweeps), "total_iterations": int(report.total_iterations)}
+        _native.raise_for(code, message)
+        return self.last_report
+
+    # ----------------------------------------------- single operators (device)
+    def apply_divergence(self, velocity: Any) -> Any:
+        """D on packed velocity columns (torch device tensor, (size,) or (size, k))."""
+        return self._op(velocity, self.grid.size, self.grid.np, "mac_apply_divergence")
+
+    def apply_divergence_transpose(self, pressure: Any) -> Any:
+        """D^T on pressure columns (torch device tensor, (np,) or (np, k))."""
+        return self._op(pressure, self.grid.np, self.grid.size, "mac_apply_divergence_transpose")
+
+    def velocity_inverse(self, values: Any, mass: float, viscosity: float) -> Any:
+        """(mass I + viscosity K)^-1 on packed velocity columns (torch device tensor)."""
+        return self._op(values, self.grid.size, self.grid.size, "mac_velocity_inverse", mass, viscosity)
+
+    def pressure_poisson_inverse(self, values: Any) -> Any:
+        """(D D^T)^+ on pressure columns, mean removed (torch device tensor)."""
+        return self._op(values, self.grid.np, self.grid.np, "mac_pressure_poisson_inverse")
+
+    def _op(self, x: Any, n_in: int, n_out: int, name: str, *scalars: float) -> Any:
+        import torch
+
+        if not _is_torch(x) or x.device.type != "cuda" or x.shape[0] != n_in or x.ndim not in (1, 2):
+            raise ValueError("expected a device tensor of matching length")
+        two_d = x.ndim == 2
+        cols = (x if two_d else x[:, None]).t().contiguous().to(torch.float64)
+        k = cols.shape[0]
+        out = torch.empty((k, n_out), dtype=torch.float64, device=x.device)
+        torch.cuda.current_stream(x.device).synchronize()
+        fn = getattr(self._lib, name)
+        if scalars:
+            code = fn(self._handle, cols.data_ptr(), n_in, k, *scalars, out.data_ptr(), n_out)
+        else:
+            code = fn(self._handle, cols.data_ptr(), n_in, k, out.data_ptr(), n_out)
+        _native.raise_for(code, self._lib.mac_stokes_last_error(self._handle).decode())
+        return out.t() if two_d else out[0]
