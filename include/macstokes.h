@@ -47,11 +47,15 @@ typedef struct mac_stokes_opts {
   double krylov_tolerance; /* Schur-CG rtol, ||r|| < rtol*||b|| (default 1e-12)      */
   int32_t max_iterations;  /* Schur-CG maxiter (default 100)                         */
   int32_t precision;       /* MAC_PRECISION_*                                        */
-  double refine_tolerance; /* MIXED: stop when residual and divergence/(1+max|u|)
-                              are both <= this (default 1e-12)                       */
+  double refine_tolerance; /* MIXED: stop when the residual and max|D u| are both
+                              <= this (default 1e-10)                               */
   double inner_tolerance;  /* MIXED: inner Schur-CG rtol per sweep (default 1e-3)    */
   int32_t max_sweeps;      /* MIXED: outer defect-correction sweep cap (default 30)  */
   int32_t device;          /* HIP device ordinal                                      */
+  int32_t mixed_split;     /* MIXED: bit 0 -> velocity inverses, bit 1 -> pressure
+                              preconditioner use the split bf16x3 GEMM (hi/lo parts
+                              of data and matrix, ~16-bit operands; default 3)      */
+  int32_t pad0;
 } mac_stokes_opts;
 
 typedef struct mac_solve_report {
@@ -85,6 +89,19 @@ int32_t mac_stokes_solve(void* handle, const double* rhs, int64_t ld_rhs, int32_
                          int64_t ld_pressure, const double* pressure_guess, int64_t ld_guess,
                          int32_t* iterations, mac_solve_report* report, int32_t device_pointers);
 
+/* ---- measurement: HIP-event timing of the transform GEMM launches ---- */
+typedef struct mac_kernel_profile {
+  double gemm_ms;        /* summed device time of the transform-GEMM launches       */
+  double gemm_flops;     /* their algorithmic flops, 2*M*N*K per product            */
+  double gemm_bytes;     /* their algorithmic bytes: A, Bt and C elements once each  */
+  int64_t gemm_launches; /* number of launches timed                                */
+} mac_kernel_profile;
+
+/* No counterpart in the reference (its counters are wall-clock only,
+ * tensor_stokes.py:169-174): switches per-launch event timing on/off and resets it. */
+void mac_stokes_set_profiling(void* handle, int32_t on);
+int32_t mac_stokes_get_profile(void* handle, mac_kernel_profile* out);
+
 /* ---- single operators, device pointers only (parity tests and host callers) ---- */
 
 /* D: packed velocity columns -> cell divergence (kernels.py:46-87). */
@@ -93,7 +110,9 @@ int32_t mac_apply_divergence(void* handle, const double* velocity, int64_t ld_ve
 /* D^T: pressure columns -> packed faces (kernels.py:89-132). */
 int32_t mac_apply_divergence_transpose(void* handle, const double* pressure, int64_t ld_pressure,
                                        int32_t ncols, double* out, int64_t ld_out);
-/* (mass*I + viscosity*K)^-1 on packed velocity columns (tensor_stokes.py:200-215). */
+/* (mass*I + viscosity*K)^-1 on packed velocity columns (tensor_stokes.py:200-215).
+ * On a MAC_PRECISION_MIXED handle this and the next entry point use the bf16
+ * transforms (FP32 accumulation), as the mixed solve's inner iterations do. */
 int32_t mac_velocity_inverse(void* handle, const double* in, int64_t ld_in, int32_t ncols, double mass,
                              double viscosity, double* out, int64_t ld_out);
 /* (D D^T)^+ on pressure columns (tensor_stokes.py:217-243). */

@@ -16,12 +16,14 @@ enum Epilogue : int {
   EPI_SHIFT = 1,    // C = acc / (s0 + s1 * (ea[m] + eb[n]))        velocity shift
   EPI_LAMBDA = 2,   // lam = ea[m] + eb[n]; C = lam > thresh ? acc/lam : 0   (D D^T)^+
   EPI_COMBINE = 3,  // C = s0 * acc + s1 * aux                          preconditioner
+  EPI_ACCUM = 4,    // C = C + acc                                      defect correction
 };
 enum DType : int { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2 };
 
 struct GemmEntry {
   const void* A;
   const void* Bt;
+  const void* Bt_lo;   // split mode: low bf16 part of the matrix (same layout as Bt)
   void* C;
   const void* aux;     // EPI_COMBINE: same layout/type as C
   const double* ea;    // eigenvalues indexed by m
@@ -44,7 +46,8 @@ struct GemmBatch {
 // Tile edge used by the launchers (entries are tiled by the launcher itself).
 void gemm_f64(GemmBatch& b, hipStream_t s);
 // a_type: DT_F64 / DT_F32 / DT_BF16 for A and aux/C per c_type; Bt is always bf16.
-void gemm_bf16(GemmBatch& b, int a_type, int c_type, hipStream_t s);
+// split: three-product hi/lo form (A must be DT_F32 or DT_F64, Bt_lo set).
+void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s);
 
 // ------------------------------------------------------------- stencil kernels
 // Internal velocity layout: u block [ny][ldu], v block [ny-1][ldv] (ld >= row length),
@@ -73,10 +76,11 @@ int reduce_blocks(int64_t n);
 
 enum Reduce : int { RED_NONE = 0, RED_SUM = 1, RED_MAXABS = 2 };
 
-// q[c] = sign * D vel[c]; optional per-block partial sums / max|.| of q.
+// q[c] = add[c] + sign * D vel[c] (add optional); optional per-block partial sums /
+// max|.| of q.
 void divergence(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
                 int64_t out_stride, double sign, int ncols, int reduce, double* partials,
-                const CgState* st, hipStream_t s);
+                const CgState* st, hipStream_t s, const double* add = nullptr);
 // out[c] = add[c] + D^T (p[c] - mean[c]) (add / mean optional), out in layout L.
 void divergence_transpose(const double* p, int64_t p_stride, const double* mean_sums, const double* add,
                           int64_t add_stride, int out_type, void* out, const VelLayout& L,
@@ -109,6 +113,8 @@ void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_
 // Final-solve checks over all columns: partial max|defect|, max|f|, max|vel| where
 // defect = mass*vel + visc*K vel - D^T p - f (vel, f packed; p mean-zero).
 void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols,
-                double mass, double visc, double* partials3, hipStream_t s);
+                double mass, double visc, double* partials3, hipStream_t s, double* residual_out = nullptr);
+// x[c] += y[c] (FP64, contiguous columns).
+void add_cols(double* x, const double* y, int64_t n, int ncols, hipStream_t s);
 
 }  // namespace macs

@@ -40,7 +40,7 @@ __device__ inline double block_reduce(double v, double* sh) {
 template <typename TIN, int MODE>
 __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, VelLayout L, GridDims g,
                                                   double* __restrict__ out, int64_t out_stride, double sign,
-                                                  double* partials, const CgState* st) {
+                                                  double* partials, const CgState* st, const double* add) {
   const int c = blockIdx.y;
   if (st != nullptr && !st[c].active) return;
   __shared__ double sh[4];
@@ -59,6 +59,7 @@ __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, V
     if (j < ny - 1) value += ld_d(v, (int64_t)j * L.ldv + i) / g.hy;
     if (j > 0) value -= ld_d(v, (int64_t)(j - 1) * L.ldv + i) / g.hy;
     value = sign * value;
+    if (add != nullptr) value = add[(int64_t)c * out_stride + cell] + value;
     o[cell] = value;
     if (MODE == RED_SUM) accum += value;
     if (MODE == RED_MAXABS) accum = fmax(accum, absn(value));
@@ -253,7 +254,8 @@ __global__ __launch_bounds__(kThreads) void k_update_xr(double* __restrict__ x, 
 // defect, of f and of vel (three per block).
 __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ vel, const double* __restrict__ f,
                                                      const double* __restrict__ p, GridDims g, int ncols,
-                                                     double mass, double visc, double* partials) {
+                                                     double mass, double visc, double* partials,
+                                                     double* __restrict__ rout) {
   __shared__ double sh[4];
   const int nx = g.nx, ny = g.ny;
   const int64_t nu = (int64_t)(nx - 1) * ny, nv = (int64_t)nx * (ny - 1), size = nu + nv;
@@ -298,6 +300,7 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
     const double fv = f[(int64_t)c * size + fidx];
     const double vv = vc[fidx];
     const double d = mass * vv + visc * kv - lift - fv;
+    if (rout != nullptr) rout[(int64_t)c * size + fidx] = -d;
     md = fmax(md, absn(d));
     mf = fmax(mf, absn(fv));
     mv = fmax(mv, absn(vv));
@@ -327,16 +330,16 @@ int reduce_blocks(int64_t n) {
 
 void divergence(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
                 int64_t out_stride, double sign, int ncols, int reduce, double* partials, const CgState* st,
-                hipStream_t s) {
+                hipStream_t s, const double* add) {
   const dim3 grid = col_grid((int64_t)g.nx * g.ny, ncols);
 #define MACS_DIV(T)                                                                                          \
   switch (reduce) {                                                                                          \
     case RED_SUM: k_div<T, RED_SUM><<<grid, kThreads, 0, s>>>((const T*)vel, L, g, out, out_stride, sign,   \
-                                                                partials, st); break;                        \
+                                                                partials, st, add); break;                   \
     case RED_MAXABS: k_div<T, RED_MAXABS><<<grid, kThreads, 0, s>>>((const T*)vel, L, g, out, out_stride,   \
-                                                                      sign, partials, st); break;            \
+                                                                      sign, partials, st, add); break;       \
     default: k_div<T, RED_NONE><<<grid, kThreads, 0, s>>>((const T*)vel, L, g, out, out_stride, sign,       \
-                                                            partials, st); break;                            \
+                                                            partials, st, add); break;                       \
   }
   if (in_type == DT_F64) { MACS_DIV(double) }
   else if (in_type == DT_F32) { MACS_DIV(float) }
@@ -410,9 +413,19 @@ void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_
 }
 
 void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols, double mass,
-                double visc, double* partials3, hipStream_t s) {
+                double visc, double* partials3, hipStream_t s, double* residual_out) {
   const int64_t size = (int64_t)(g.nx - 1) * g.ny + (int64_t)g.nx * (g.ny - 1);
-  k_defect<<<reduce_blocks(size * ncols), kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3);
+  k_defect<<<reduce_blocks(size * ncols), kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3,
+                                                             residual_out);
+}
+
+__global__ void k_add_cols(double* __restrict__ x, const double* __restrict__ y, int64_t total) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kThreads)
+    x[i] = x[i] + y[i];
+}
+
+void add_cols(double* x, const double* y, int64_t n, int ncols, hipStream_t s) {
+  k_add_cols<<<reduce_blocks(n * ncols), kThreads, 0, s>>>(x, y, n * ncols);
 }
 
 }  // namespace macs

@@ -103,6 +103,8 @@ struct DevMatrix {
   double* qt = nullptr;
   __bf16* qb = nullptr;
   __bf16* qtb = nullptr;
+  __bf16* qb_lo = nullptr;   // bf16(q - qb): the split form's second term
+  __bf16* qtb_lo = nullptr;
 };
 
 // One 2-D block (u, v or pressure) of shape ny x nx with its y/x bases.
@@ -133,12 +135,27 @@ class Solver {
     packed_.voff = nu_;
     packed_.stride = size_;
     build_bases();
+    // bf16 scratch per column: u region then v region, each large enough for the
+    // natural [ny'][round8(nx')] and transposed [nx'][round8(ny')] forms.
+    {
+      auto r8 = [](int64_t v) { return (v + 7) & ~int64_t(7); };
+      const int64_t ureg = std::max<int64_t>((int64_t)ny * round8(nx - 1), (int64_t)(nx - 1) * round8(ny));
+      const int64_t vreg = std::max<int64_t>((int64_t)(ny - 1) * round8(nx), (int64_t)nx * round8(ny - 1));
+      const int64_t preg = std::max<int64_t>((int64_t)ny * round8(nx), (int64_t)nx * round8(ny));
+      toff_v_ = r8(ureg);
+      tstride_ = std::max(r8(toff_v_ + vreg), r8(preg));
+      lay_b_.ldu = round8(nx - 1);
+      lay_b_.ldv = round8(nx);
+      lay_b_.voff = toff_v_;
+      lay_b_.stride = tstride_;
+    }
     HIP_CHECK(hipHostMalloc((void**)&host_flags_, sizeof(int) * kMaxCols, hipHostMallocDefault));
   }
 
   ~Solver() {
     (void)hipSetDevice(opts_.device);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (hipEvent_t e : all_events_) (void)hipEventDestroy(e);
     for (void* p : allocs_) (void)hipFree(p);
     if (host_flags_) (void)hipHostFree(host_flags_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -181,10 +198,10 @@ class Solver {
       gb.epi = EPI_SHIFT; gb.trans_c = 0; gb.s0 = mass; gb.s1 = visc;
       gc.epi = EPI_STORE; gc.trans_c = 1;
       gd.epi = EPI_STORE; gd.trans_c = 0;
-      gemm_f64(ga, stream_);
-      gemm_f64(gb, stream_);
-      gemm_f64(gc, stream_);
-      gemm_f64(gd, stream_);
+      run_gemm(ga);
+      run_gemm(gb);
+      run_gemm(gc);
+      run_gemm(gd);
     }
   }
 
@@ -212,13 +229,188 @@ class Solver {
       gb.epi = EPI_LAMBDA; gb.trans_c = 0; gb.thresh = p_thresh_;
       gc.epi = EPI_STORE; gc.trans_c = 1;
       gd.epi = combine ? EPI_COMBINE : EPI_STORE; gd.trans_c = 0; gd.s0 = s0; gd.s1 = s1;
-      gemm_f64(ga, stream_);
-      gemm_f64(gb, stream_);
-      gemm_f64(gc, stream_);
-      gemm_f64(gd, stream_);
+      run_gemm(ga);
+      run_gemm(gb);
+      run_gemm(gc);
+      run_gemm(gd);
     }
   }
 
+  // ------------------------------------------------------ bf16 transforms
+  // Mixed precision: the same four passes with bf16 operands (FP32 accumulation).
+  // Intermediates are bf16 in padded scratch (leading dimensions rounded to 8 so
+  // every row starts 16-byte aligned); the last pass writes FP32 (out_type DT_F32,
+  // natural packed layout) or adds into an FP64 packed velocity (DT_F64, EPI_ACCUM).
+  void velocity_inverse_bf16(const void* in, int in_type, const VelLayout& Lin, void* out, int out_type,
+                             int64_t out_stride, const std::vector<int>& cols, double mass, double visc) {
+    const bool sp = split_velocity();
+    const int mid = sp ? DT_F32 : DT_BF16;           // intermediate storage type
+    const size_t mid_bytes = sp ? 4 : 2;
+    char* t1base = sp ? reinterpret_cast<char*>(tf1_) : reinterpret_cast<char*>(tb1_);
+    char* t2base = sp ? reinterpret_cast<char*>(tf2_) : reinterpret_cast<char*>(tb2_);
+    for (size_t base = 0; base < cols.size(); base += kMaxEntries / 2) {
+      const size_t count = std::min<size_t>(kMaxEntries / 2, cols.size() - base);
+      GemmBatch ga{}, gb{}, gc{}, gd{};
+      const size_t in_bytes = in_type == DT_F64 ? 8 : (in_type == DT_F32 ? 4 : 2);
+      const size_t out_bytes = out_type == DT_F64 ? 8 : 4;
+      for (size_t t = 0; t < count; ++t) {
+        const int c = cols[base + t];
+        const Block* blocks[2] = {&ublk_, &vblk_};
+        const int64_t in_off[2] = {0, Lin.voff};
+        const int in_ld[2] = {Lin.ldu, Lin.ldv};
+        const int64_t t_off[2] = {0, toff_v_};
+        const int64_t out_off[2] = {0, nu_};
+        for (int bi = 0; bi < 2; ++bi) {
+          const Block& B = *blocks[bi];
+          const char* src = static_cast<const char*>(in) + ((int64_t)c * Lin.stride + in_off[bi]) * in_bytes;
+          char* t1 = t1base + ((int64_t)c * tstride_ + t_off[bi]) * mid_bytes;
+          char* t2 = t2base + ((int64_t)c * tstride_ + t_off[bi]) * mid_bytes;
+          char* dst = static_cast<char*>(out) + ((int64_t)c * out_stride + out_off[bi]) * out_bytes;
+          const int ry = round8(B.ny), rx = round8(B.nx);
+          add_pass_any(ga, src, in_ld[bi], B.mx->qb, B.mx->ld8, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr,
+                       sp ? B.mx->qb_lo : nullptr);
+          add_pass_any(gb, t1, ry, B.my->qb, B.my->ld8, t2, ry, B.nx, B.ny, B.ny, B.lam_x, B.lam_y,
+                       sp ? B.my->qb_lo : nullptr);
+          add_pass_any(gc, t2, ry, B.my->qtb, B.my->ld8, t1, rx, B.nx, B.ny, B.ny, nullptr, nullptr,
+                       sp ? B.my->qtb_lo : nullptr);
+          add_pass_any(gd, t1, rx, B.mx->qtb, B.mx->ld8, dst, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr,
+                       sp ? B.mx->qtb_lo : nullptr);
+        }
+      }
+      ga.epi = EPI_STORE; ga.trans_c = 1;
+      gb.epi = EPI_SHIFT; gb.trans_c = 0; gb.s0 = mass; gb.s1 = visc;
+      gc.epi = EPI_STORE; gc.trans_c = 1;
+      gd.epi = out_type == DT_F64 ? EPI_ACCUM : EPI_STORE; gd.trans_c = 0;
+      run_gemm(ga, in_type, mid, true, sp);
+      run_gemm(gb, mid, mid, true, sp);
+      run_gemm(gc, mid, mid, true, sp);
+      run_gemm(gd, mid, out_type, true, sp);
+    }
+  }
+
+  // z = s0 * (D D^T)^+ r + s1 * r with bf16 transforms; r and z FP64 (np per column).
+  void pressure_inverse_bf16(const double* in, double* out, const std::vector<int>& cols, double s0, double s1) {
+    const bool sp = split_pressure();
+    const int mid = sp ? DT_F32 : DT_BF16;
+    const size_t mid_bytes = sp ? 4 : 2;
+    char* t1base = sp ? reinterpret_cast<char*>(tf1_) : reinterpret_cast<char*>(tb1_);
+    char* t2base = sp ? reinterpret_cast<char*>(tf2_) : reinterpret_cast<char*>(tb2_);
+    for (size_t base = 0; base < cols.size(); base += kMaxEntries) {
+      const size_t count = std::min<size_t>(kMaxEntries, cols.size() - base);
+      GemmBatch ga{}, gb{}, gc{}, gd{};
+      const Block& B = pblk_;
+      const int ry = round8(B.ny), rx = round8(B.nx);
+      for (size_t t = 0; t < count; ++t) {
+        const int c = cols[base + t];
+        const double* src = in + (int64_t)c * np_;
+        double* dst = out + (int64_t)c * np_;
+        char* t1 = t1base + (int64_t)c * tstride_ * mid_bytes;
+        char* t2 = t2base + (int64_t)c * tstride_ * mid_bytes;
+        add_pass_any(ga, src, B.nx, B.mx->qb, B.mx->ld8, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr,
+                     sp ? B.mx->qb_lo : nullptr);
+        add_pass_any(gb, t1, ry, B.my->qb, B.my->ld8, t2, ry, B.nx, B.ny, B.ny, B.lam_x, B.lam_y,
+                     sp ? B.my->qb_lo : nullptr);
+        add_pass_any(gc, t2, ry, B.my->qtb, B.my->ld8, t1, rx, B.nx, B.ny, B.ny, nullptr, nullptr,
+                     sp ? B.my->qtb_lo : nullptr);
+        add_pass_any(gd, t1, rx, B.mx->qtb, B.mx->ld8, dst, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr,
+                     sp ? B.mx->qtb_lo : nullptr);
+        gd.e[gd.count - 1].aux = src;
+      }
+      ga.epi = EPI_STORE; ga.trans_c = 1;
+      gb.epi = EPI_LAMBDA; gb.trans_c = 0; gb.thresh = p_thresh_;
+      gc.epi = EPI_STORE; gc.trans_c = 1;
+      gd.epi = EPI_COMBINE; gd.trans_c = 0; gd.s0 = s0; gd.s1 = s1;
+      run_gemm(ga, DT_F64, mid, true, sp);
+      run_gemm(gb, mid, mid, true, sp);
+      run_gemm(gc, mid, mid, true, sp);
+      run_gemm(gd, mid, DT_F64, true, sp);
+    }
+  }
+
+  bool split_velocity() const { return (opts_.mixed_split & 1) != 0; }
+  bool split_pressure() const { return (opts_.mixed_split & 2) != 0; }
+
+  // ------------------------------------------------------------- profiling
+  // Opt-in HIP-event timing of every transform-GEMM launch on the solver's stream,
+  // with the launches' algorithmic flops and bytes (2MNK and the A, Bt, C elements).
+  void set_profiling(bool on) {
+    collect_profile();
+    profiling_ = on;
+    prof_ms_ = prof_flops_ = prof_bytes_ = 0.0;
+    prof_launches_ = 0;
+  }
+  void profile(double* ms, double* flops, double* bytes, int64_t* launches) {
+    collect_profile();
+    *ms = prof_ms_;
+    *flops = prof_flops_;
+    *bytes = prof_bytes_;
+    *launches = prof_launches_;
+  }
+
+  void run_gemm(GemmBatch& b, int a_type = DT_F64, int c_type = DT_F64, bool bf16 = false, bool split = false) {
+    if (b.count <= 0) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (profiling_) {
+      e0 = take_event();
+      e1 = take_event();
+      HIP_CHECK(hipEventRecord(e0, stream_));
+    }
+    if (bf16) gemm_bf16(b, a_type, c_type, split, stream_);
+    else gemm_f64(b, stream_);
+    HIP_CHECK(hipGetLastError());
+    if (profiling_) {
+      HIP_CHECK(hipEventRecord(e1, stream_));
+      const double in_a = a_type == DT_F64 ? 8 : (a_type == DT_F32 ? 4 : 2);
+      const double in_b = bf16 ? (split ? 4 : 2) : 8;
+      const double out_c = c_type == DT_F64 ? 8 : (c_type == DT_F32 ? 4 : 2);
+      double flops = 0, bytes = 0;
+      for (int i = 0; i < b.count; ++i) {
+        const GemmEntry& e = b.e[i];
+        flops += (split ? 6.0 : 2.0) * e.M * e.N * e.K;
+        bytes += in_a * e.M * e.K + in_b * (double)e.N * e.K + out_c * (double)e.M * e.N;
+      }
+      pending_.push_back({e0, e1, flops, bytes});
+    }
+  }
+
+ private:
+  struct Pending {
+    hipEvent_t e0, e1;
+    double flops, bytes;
+  };
+  hipEvent_t take_event() {
+    if (!event_pool_.empty()) {
+      hipEvent_t e = event_pool_.back();
+      event_pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    all_events_.push_back(e);
+    return e;
+  }
+  void collect_profile() {
+    if (pending_.empty()) return;
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    for (const Pending& p : pending_) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, p.e0, p.e1));
+      prof_ms_ += ms;
+      prof_flops_ += p.flops;
+      prof_bytes_ += p.bytes;
+      prof_launches_ += 1;
+      event_pool_.push_back(p.e0);
+      event_pool_.push_back(p.e1);
+    }
+    pending_.clear();
+  }
+  bool profiling_ = false;
+  double prof_ms_ = 0, prof_flops_ = 0, prof_bytes_ = 0;
+  int64_t prof_launches_ = 0;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> event_pool_, all_events_;
+
+ public:
   // ------------------------------------------------------------------- solve
   struct Outcome {
     double residual = 0, divergence = 0, vmax = 0, fmax = 0;
@@ -228,6 +420,9 @@ class Solver {
 
   Outcome solve(const double* f, int64_t f_stride, int ncols, double mass, double visc, double* vel,
                 int64_t vel_stride, double* pres, int64_t pres_stride, const double* guess, int64_t guess_stride) {
+    if (opts_.precision == MAC_PRECISION_MIXED)
+      return solve_mixed(f, f_stride, ncols, mass, visc, vel, vel_stride, pres, pres_stride, guess, guess_stride);
+    mixed_ = false;
     ensure_capacity(ncols);
     ncols_cur_ = ncols;
     std::vector<int> all(ncols);
@@ -268,7 +463,7 @@ class Solver {
         finalize(partials_, nbp, ncols, RED_SUM, rr_, stream_);
       }
     }
-    pcg(all, ncols, mass, visc, out.iterations);
+    pcg(ncols, mass, visc, out.iterations, true);
     // p = x - mean(x)
     reduce(x_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_);
     finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
@@ -281,14 +476,103 @@ class Solver {
     return out;
   }
 
+  // MIXED: FP64 defect correction around bf16-transform inner solves.
+  //   u = 0, p = guess (or 0)
+  //   repeat: r_u = f - (mass u + visc K u) + D^T p, r_c = -D u   (FP64)
+  //           stop when max|r_u|/(1+max|f|) <= refine_tol and max|D u| <= refine_tol
+  //           W = L~^-1 r_u; b = r_c - D W; dp = PCG(S~, b, inner_tol); p += dp
+  //           u += L~^-1 (r_u + D^T dp)
+  // (L~, S~: the velocity inverse and Schur operator with bf16 transforms.)
+  Outcome solve_mixed(const double* f, int64_t f_stride, int ncols, double mass, double visc, double* vel,
+                      int64_t vel_stride, double* pres, int64_t pres_stride, const double* guess,
+                      int64_t guess_stride) {
+    ensure_capacity(ncols);
+    ncols_cur_ = ncols;
+    std::vector<int> all(ncols);
+    for (int c = 0; c < ncols; ++c) all[c] = c;
+    Outcome out;
+    out.iterations.assign(ncols, 0);
+    out.sweeps = 0;
+    const double* fc = f;
+    if (f_stride != size_) {
+      copy_cols(f, f_stride, fcomp_, size_, size_, ncols, stream_);
+      fc = fcomp_;
+    }
+    HIP_CHECK(hipMemsetAsync(uacc_, 0, sizeof(double) * size_ * ncols, stream_));
+    if (guess) copy_cols(guess, guess_stride, pacc_, np_, np_, ncols, stream_);
+    else HIP_CHECK(hipMemsetAsync(pacc_, 0, sizeof(double) * np_ * ncols, stream_));
+    const int nbp = reduce_blocks(np_);
+    const int nb = reduce_blocks(size_ * ncols);
+    std::vector<int> inner(ncols);
+    for (int sweep = 0;; ++sweep) {
+      // FP64 residuals of the full system at the current iterate.
+      defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_);
+      finalize(partials_, nb, 1, RED_MAXABS, sums_, stream_);
+      finalize(partials_ + nb, nb, 1, RED_MAXABS, sums_ + 1, stream_);
+      finalize(partials_ + 2 * nb, nb, 1, RED_MAXABS, sums_ + 2, stream_);
+      divergence(DT_F64, uacc_, packed_, g_, rc_, np_, -1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr,
+                 stream_);
+      finalize(partials_ + 3 * nb, nbp, ncols, RED_MAXABS, sums2_, stream_);
+      double h[3];
+      std::vector<double> dv(ncols);
+      HIP_CHECK(hipMemcpyAsync(h, sums_, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipMemcpyAsync(dv.data(), sums2_, sizeof(double) * ncols, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      out.residual = h[0] / (1.0 + h[1]);
+      out.fmax = h[1];
+      out.vmax = h[2];
+      out.divergence = 0.0;
+      for (double d : dv) out.divergence = std::max(out.divergence, d);
+      const bool done = out.residual <= opts_.refine_tolerance && out.divergence <= opts_.refine_tolerance;
+      if (done || sweep >= opts_.max_sweeps || !std::isfinite(out.residual) || !std::isfinite(out.divergence))
+        break;
+      // Inner solve for the correction (dp, du).
+      mixed_ = true;
+      velocity_inverse_bf16(ru_, DT_F64, packed_, wf_, DT_F32, size_, all, mass, visc);
+      divergence(DT_F32, wf_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_);
+      finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+      sub_mean_dot(b_, np_, np_, ncols, sums_, b_, partials_, nullptr, stream_);
+      finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);
+      reduce(b_, nullptr, np_, np_, ncols, RED_MAXABS, partials_, nullptr, stream_);
+      finalize(partials_, nbp, ncols, RED_MAXABS, sums3_, stream_);
+      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_);
+      HIP_CHECK(hipMemsetAsync(x_, 0, sizeof(double) * np_ * ncols, stream_));
+      copy_cols(b_, np_, r_, np_, np_, ncols, stream_);
+      HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
+      pcg(ncols, mass, visc, inner, false);
+      for (int c = 0; c < ncols; ++c) out.iterations[c] += inner[c];
+      add_cols(pacc_, x_, np_, ncols, stream_);
+      {
+        const int dt_type = split_velocity() ? DT_F32 : DT_BF16;
+        void* dt_buf = split_velocity() ? static_cast<void*>(tfb_) : static_cast<void*>(tb_);
+        divergence_transpose(x_, np_, nullptr, ru_, size_, dt_type, dt_buf, lay_b_, g_, ncols, nullptr, stream_);
+        velocity_inverse_bf16(dt_buf, dt_type, lay_b_, uacc_, DT_F64, size_, all, mass, visc);
+      }
+      mixed_ = false;
+      out.sweeps = sweep + 1;
+    }
+    // p mean-zero; hand out u and p.
+    reduce(pacc_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_);
+    finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+    sub_mean_dot(pacc_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
+    copy_cols(pacc_, np_, pres, pres_stride, np_, ncols, stream_);
+    copy_cols(uacc_, size_, vel, vel_stride, size_, ncols, stream_);
+    return out;
+  }
+
  private:
   // One GEMM pass of a 2-D transform: C = A . Q^T with A [M][K] (lda), Q (or Q^T) as
   // Bt [N][K] (ldb = n), C written transposed [N][M] (ldc) when the batch says so.
   static void add_pass(GemmBatch& b, const double* A, int lda, const double* Q, int n, double* C, int ldc,
                        int M, int N, int K, const double* ea, const double* eb) {
+    add_pass_any(b, A, lda, Q, n, C, ldc, M, N, K, ea, eb);
+  }
+  static void add_pass_any(GemmBatch& b, const void* A, int lda, const void* Q, int n, void* C, int ldc,
+                           int M, int N, int K, const double* ea, const double* eb, const void* Qlo = nullptr) {
     GemmEntry& e = b.e[b.count++];
     e.A = A;
     e.Bt = Q;
+    e.Bt_lo = Qlo;
     e.C = C;
     e.aux = nullptr;
     e.ea = ea;
@@ -306,15 +590,25 @@ class Solver {
              const double* dot_with) {
     const int ncols = ncols_cur_;
     const CgState* st = use_state_ ? cg_ : nullptr;
-    divergence_transpose(pin, np_, nullptr, nullptr, 0, DT_F64, w2_, packed_, g_, ncols, st, stream_);
-    velocity_inverse(w2_, size_, w_, size_, cols, mass, visc);
     const int nbp = reduce_blocks(np_);
-    divergence(DT_F64, w_, packed_, g_, qout, np_, 1.0, ncols, RED_SUM, partials_, st, stream_);
+    if (mixed_) {
+      const int dt_type = split_velocity() ? DT_F32 : DT_BF16;
+      void* dt_buf = split_velocity() ? static_cast<void*>(tfb_) : static_cast<void*>(tb_);
+      divergence_transpose(pin, np_, nullptr, nullptr, 0, dt_type, dt_buf, lay_b_, g_, ncols, st, stream_);
+      velocity_inverse_bf16(dt_buf, dt_type, lay_b_, wf_, DT_F32, size_, cols, mass, visc);
+      divergence(DT_F32, wf_, packed_, g_, qout, np_, 1.0, ncols, RED_SUM, partials_, st, stream_);
+    } else {
+      divergence_transpose(pin, np_, nullptr, nullptr, 0, DT_F64, w2_, packed_, g_, ncols, st, stream_);
+      velocity_inverse(w2_, size_, w_, size_, cols, mass, visc);
+      divergence(DT_F64, w_, packed_, g_, qout, np_, 1.0, ncols, RED_SUM, partials_, st, stream_);
+    }
     finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
     sub_mean_dot(qout, np_, np_, ncols, sums_, dot_with, partials_, st, stream_);
   }
 
-  void pcg(const std::vector<int>& all, int ncols, double mass, double visc, std::vector<int>& iters) {
+  // Schur PCG on all columns at once (x, r, rr, cg_ prepared by the caller).  With
+  // `fail`, a column not converged after max_iterations raises MAC_NOT_CONVERGED.
+  void pcg(int ncols, double mass, double visc, std::vector<int>& iters, bool fail) {
     const int nbp = reduce_blocks(np_);
     ncols_cur_ = ncols;
     use_state_ = true;
@@ -328,7 +622,8 @@ class Solver {
         if (host_flags_[c]) active.push_back(c);
       if (active.empty()) break;
       // z = mass (D D^T)^+ r + visc r, mean removed; rho = (r, z)
-      pressure_inverse(r_, np_, z_, np_, active, true, mass, visc);
+      if (mixed_) pressure_inverse_bf16(r_, z_, active, mass, visc);
+      else pressure_inverse(r_, np_, z_, np_, active, true, mass, visc);
       reduce(z_, nullptr, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
       finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
       sub_mean_dot(z_, np_, np_, ncols, sums_, r_, partials_, cg_, stream_);
@@ -350,7 +645,7 @@ class Solver {
     HIP_CHECK(hipStreamSynchronize(stream_));
     for (int c = 0; c < ncols; ++c) {
       iters[c] = st[c].iters;
-      if (!st[c].converged)
+      if (fail && !st[c].converged)
         throw SolveFailure(MAC_NOT_CONVERGED,
                            "Pressure Schur CG failed with info=" + std::to_string(opts_.max_iterations));
     }
@@ -427,13 +722,20 @@ class Solver {
     m->q = upload(q);
     m->qt = upload(qt);
     std::vector<__bf16> qb((size_t)n * m->ld8, (__bf16)0.0f), qtb((size_t)n * m->ld8, (__bf16)0.0f);
+    std::vector<__bf16> qbl((size_t)n * m->ld8, (__bf16)0.0f), qtbl((size_t)n * m->ld8, (__bf16)0.0f);
     for (int a = 0; a < n; ++a)
       for (int b = 0; b < n; ++b) {
-        qb[(size_t)a * m->ld8 + b] = (__bf16)(float)q[(size_t)a * n + b];
-        qtb[(size_t)a * m->ld8 + b] = (__bf16)(float)qt[(size_t)a * n + b];
+        const double v = q[(size_t)a * n + b], vt = qt[(size_t)a * n + b];
+        const __bf16 h = (__bf16)(float)v, ht = (__bf16)(float)vt;
+        qb[(size_t)a * m->ld8 + b] = h;
+        qtb[(size_t)a * m->ld8 + b] = ht;
+        qbl[(size_t)a * m->ld8 + b] = (__bf16)(float)(v - (double)(float)h);
+        qtbl[(size_t)a * m->ld8 + b] = (__bf16)(float)(vt - (double)(float)ht);
       }
     m->qb = static_cast<__bf16*>(upload_raw(qb.data(), qb.size() * sizeof(__bf16)));
     m->qtb = static_cast<__bf16*>(upload_raw(qtb.data(), qtb.size() * sizeof(__bf16)));
+    m->qb_lo = static_cast<__bf16*>(upload_raw(qbl.data(), qbl.size() * sizeof(__bf16)));
+    m->qtb_lo = static_cast<__bf16*>(upload_raw(qtbl.data(), qtbl.size() * sizeof(__bf16)));
     mats_.push_back(std::move(m));
     return mats_.back().get();
   }
@@ -483,6 +785,18 @@ class Solver {
     z_ = (double*)take(pb);
     p_ = (double*)take(pb);
     q_ = (double*)take(pb);
+    fcomp_ = (double*)take(vb);
+    uacc_ = (double*)take(vb);
+    ru_ = (double*)take(vb);
+    pacc_ = (double*)take(pb);
+    rc_ = (double*)take(pb);
+    wf_ = (float*)take(sizeof(float) * size_ * ncols);
+    tb_ = (__bf16*)take(sizeof(__bf16) * tstride_ * ncols);
+    tb1_ = (__bf16*)take(sizeof(__bf16) * tstride_ * ncols);
+    tb2_ = (__bf16*)take(sizeof(__bf16) * tstride_ * ncols);
+    tf1_ = (float*)take(sizeof(float) * tstride_ * ncols);
+    tf2_ = (float*)take(sizeof(float) * tstride_ * ncols);
+    tfb_ = (float*)take(sizeof(float) * tstride_ * ncols);
     const int nb = std::max(reduce_blocks(size_ * ncols), reduce_blocks(np_) * ncols);
     partials_ = (double*)take(sizeof(double) * (4 * (size_t)nb + 16));
     sums_ = (double*)take(sizeof(double) * (ncols + 4));
@@ -517,6 +831,13 @@ class Solver {
   bool use_state_ = false;
   double *w_ = nullptr, *w2_ = nullptr, *s1_ = nullptr, *s2_ = nullptr;
   double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;
+  double *fcomp_ = nullptr, *uacc_ = nullptr, *ru_ = nullptr, *pacc_ = nullptr, *rc_ = nullptr;
+  float* wf_ = nullptr;
+  __bf16 *tb_ = nullptr, *tb1_ = nullptr, *tb2_ = nullptr;
+  float *tf1_ = nullptr, *tf2_ = nullptr, *tfb_ = nullptr;
+  int64_t tstride_ = 0, toff_v_ = 0;
+  VelLayout lay_b_{};
+  bool mixed_ = false;
   double *partials_ = nullptr, *sums_ = nullptr, *sums2_ = nullptr, *sums3_ = nullptr, *rr_ = nullptr;
   CgState* cg_ = nullptr;
   int* flags_ = nullptr;
@@ -563,9 +884,10 @@ void mac_stokes_default_opts(mac_stokes_opts* o) {
   o->krylov_tolerance = 1e-12;
   o->max_iterations = 100;
   o->precision = MAC_PRECISION_FP64;
-  o->refine_tolerance = 1e-12;
+  o->refine_tolerance = 1e-10;
   o->inner_tolerance = 1e-3;
   o->max_sweeps = 30;
+  o->mixed_split = 3;
   o->device = 0;
 }
 
@@ -580,7 +902,8 @@ void* mac_stokes_create(int32_t nx, int32_t ny, double lx, double ly, const mac_
   }
   if (!(o.tolerance > 0) || !(o.krylov_tolerance > 0) || !is_finite(o.tolerance) || !is_finite(o.krylov_tolerance) ||
       o.max_iterations < 1 || (o.precision != MAC_PRECISION_FP64 && o.precision != MAC_PRECISION_MIXED) ||
-      !(o.refine_tolerance > 0) || !(o.inner_tolerance > 0) || o.max_sweeps < 1) {
+      !(o.refine_tolerance > 0) || !(o.inner_tolerance > 0) || o.max_sweeps < 1 || o.mixed_split < 0 ||
+      o.mixed_split > 3) {
     g_create_error = "Invalid solver controls";
     return nullptr;
   }
@@ -676,6 +999,18 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
   });
 }
 
+void mac_stokes_set_profiling(void* h, int32_t on) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s) return;
+  (void)guarded(s, [&] { s->set_profiling(on != 0); });
+}
+
+int32_t mac_stokes_get_profile(void* h, mac_kernel_profile* out) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s || !out) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] { s->profile(&out->gemm_ms, &out->gemm_flops, &out->gemm_bytes, &out->gemm_launches); });
+}
+
 int32_t mac_apply_divergence(void* h, const double* vel, int64_t ldv, int32_t ncols, double* out, int64_t ldo) {
   Solver* s = static_cast<Solver*>(h);
   if (!s) return MAC_INVALID_ARGUMENT;
@@ -717,7 +1052,15 @@ int32_t mac_velocity_inverse(void* h, const double* in, int64_t ldi, int32_t nco
     s->ensure_capacity(ncols);
     std::vector<int> cols(ncols);
     for (int c = 0; c < ncols; ++c) cols[c] = c;
-    s->velocity_inverse(in, ldi, out, ldo, cols, mass, visc);
+    if (s->opts().precision == MAC_PRECISION_MIXED) {
+      // bf16 transforms, FP32 accumulation, added into a zeroed FP64 output.
+      HIP_CHECK(hipMemset2DAsync(out, ldo * sizeof(double), 0, s->size() * sizeof(double), ncols, s->stream()));
+      macs::VelLayout L = s->packed();
+      L.stride = ldi;
+      s->velocity_inverse_bf16(in, macs::DT_F64, L, out, macs::DT_F64, ldo, cols, mass, visc);
+    } else {
+      s->velocity_inverse(in, ldi, out, ldo, cols, mass, visc);
+    }
     HIP_CHECK(hipStreamSynchronize(s->stream()));
   });
 }
@@ -733,8 +1076,16 @@ int32_t mac_pressure_poisson_inverse(void* h, const double* in, int64_t ldi, int
     s->ensure_capacity(ncols);
     std::vector<int> cols(ncols);
     for (int c = 0; c < ncols; ++c) cols[c] = c;
-    s->pressure_inverse(in, ldi, out, ldo, cols, false, 1.0, 0.0);
     const int64_t np = s->np();
+    if (s->opts().precision == MAC_PRECISION_MIXED) {
+      double* tin = s->io_g_;
+      double* tout = s->io_p_;
+      macs::copy_cols(in, ldi, tin, np, np, ncols, s->stream());
+      s->pressure_inverse_bf16(tin, tout, cols, 1.0, 0.0);
+      macs::copy_cols(tout, np, out, ldo, np, ncols, s->stream());
+    } else {
+      s->pressure_inverse(in, ldi, out, ldo, cols, false, 1.0, 0.0);
+    }
     macs::reduce(out, nullptr, np, ldo, ncols, macs::RED_SUM, s->scratch_partials(), nullptr, s->stream());
     macs::finalize(s->scratch_partials(), macs::reduce_blocks(np), ncols, macs::RED_SUM, s->scratch_sums(),
                    s->stream());

@@ -29,6 +29,8 @@ EXPORTED = (
     "mac_stokes_destroy",
     "mac_stokes_last_error",
     "mac_stokes_solve",
+    "mac_stokes_set_profiling",
+    "mac_stokes_get_profile",
     "mac_apply_divergence",
     "mac_apply_divergence_transpose",
     "mac_velocity_inverse",
@@ -46,6 +48,8 @@ class StokesOpts(ctypes.Structure):
         ("inner_tolerance", ctypes.c_double),
         ("max_sweeps", ctypes.c_int32),
         ("device", ctypes.c_int32),
+        ("mixed_split", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
     ]
 
 
@@ -57,6 +61,15 @@ class SolveReport(ctypes.Structure):
         ("seconds", ctypes.c_double),
         ("sweeps", ctypes.c_int32),
         ("total_iterations", ctypes.c_int32),
+    ]
+
+
+class KernelProfile(ctypes.Structure):
+    _fields_ = [
+        ("gemm_ms", ctypes.c_double),
+        ("gemm_flops", ctypes.c_double),
+        ("gemm_bytes", ctypes.c_double),
+        ("gemm_launches", ctypes.c_int64),
     ]
 
 
@@ -91,6 +104,10 @@ def load() -> ctypes.CDLL:
     lib.mac_stokes_solve.argtypes = [_P, _P, _I64, _I32, _D, _D, _P, _I64, _P, _I64, _P, _I64,
                                      ctypes.POINTER(_I32), ctypes.POINTER(SolveReport), _I32]
     lib.mac_stokes_solve.restype = _I32
+    lib.mac_stokes_set_profiling.argtypes = [_P, _I32]
+    lib.mac_stokes_set_profiling.restype = None
+    lib.mac_stokes_get_profile.argtypes = [_P, ctypes.POINTER(KernelProfile)]
+    lib.mac_stokes_get_profile.restype = _I32
     lib.mac_apply_divergence.argtypes = [_P, _P, _I64, _I32, _P, _I64]
     lib.mac_apply_divergence.restype = _I32
     lib.mac_apply_divergence_transpose.argtypes = [_P, _P, _I64, _I32, _P, _I64]

@@ -41,8 +41,8 @@ class GpuTensorStokes:
     def __init__(self, operators: Any, tolerance: float = 1e-9, krylov_tolerance: float = 1e-12,
                  max_iterations: int = 100, matrix_free_operators: bool = True,
                  reuse_transform_buffers: bool = True, *, precision: str = "fp64",
-                 refine_tolerance: float = 1e-12, inner_tolerance: float = 1e-3, max_sweeps: int = 30,
-                 device: int = 0) -> None:
+                 refine_tolerance: float = 1e-10, inner_tolerance: float = 1e-3, max_sweeps: int = 30,
+                 mixed_split: int = 3, device: int = 0) -> None:
         if (tolerance <= 0 or krylov_tolerance <= 0 or not np.isfinite([tolerance, krylov_tolerance]).all()
                 or max_iterations < 1):
             raise ValueError("Invalid solver controls")
@@ -71,6 +71,7 @@ class GpuTensorStokes:
         opts.refine_tolerance = refine_tolerance
         opts.inner_tolerance = inner_tolerance
         opts.max_sweeps = max_sweeps
+        opts.mixed_split = mixed_split
         opts.device = device
         handle = lib.mac_stokes_create(grid.nx, grid.ny, float(grid.lx), float(grid.ly), ctypes.byref(opts))
         if not handle:
@@ -175,6 +176,18 @@ class GpuTensorStokes:
                             "sweeps": int(report.sweeps), "total_iterations": int(report.total_iterations)}
         _native.raise_for(code, message)
         return self.last_report
+
+    # ------------------------------------------------------------ measurement
+    def set_profiling(self, on: bool) -> None:
+        """Time every transform-GEMM launch with HIP events on the solver's stream."""
+        self._lib.mac_stokes_set_profiling(self._handle, 1 if on else 0)
+
+    def kernel_profile(self) -> dict[str, float]:
+        prof = _native.KernelProfile()
+        code = self._lib.mac_stokes_get_profile(self._handle, ctypes.byref(prof))
+        _native.raise_for(code, self._lib.mac_stokes_last_error(self._handle).decode())
+        return {"gemm_ms": prof.gemm_ms, "gemm_flops": prof.gemm_flops, "gemm_bytes": prof.gemm_bytes,
+                "gemm_launches": int(prof.gemm_launches)}
 
     # ----------------------------------------------- single operators (device)
     def apply_divergence(self, velocity: Any) -> Any:
