@@ -65,6 +65,8 @@ typedef struct mac_solve_report {
   double seconds;         /* wall time of the call, device-synchronised            */
   int32_t sweeps;         /* outer defect-correction sweeps (1 for FP64)           */
   int32_t total_iterations; /* Schur-CG iterations summed over columns and sweeps  */
+  int32_t warm_started;   /* 1 if the attached network supplied the CG start        */
+  int32_t pad0;
 } mac_solve_report;
 
 /* Fills `opts` with the reference defaults (tensor_stokes.py:128-136). */
@@ -101,6 +103,43 @@ typedef struct mac_kernel_profile {
  * tensor_stokes.py:169-174): switches per-launch event timing on/off and resets it. */
 void mac_stokes_set_profiling(void* handle, int32_t on);
 int32_t mac_stokes_get_profile(void* handle, mac_kernel_profile* out);
+
+/* ---- learned warm start (no counterpart in the reference; CG stays the judge) ----
+ * The network maps [pool4(b), pool4(p_prev)] (normalised) to pool4(p): b is the
+ * Schur right-hand side, p_prev the previous solve's pressure of the same column.
+ * Served grids: nx, ny multiples of 32. */
+void* mac_cnn_create(int32_t width, int32_t device);
+const char* mac_cnn_create_error(void);
+void mac_cnn_destroy(void* cnn);
+const char* mac_cnn_last_error(void* cnn);
+int64_t mac_cnn_num_params(void* cnn);
+/* Parameters as FP32, per conv layer c0..c6: weights (co, ci, 3, 3) then bias (co). */
+int32_t mac_cnn_set_params(void* cnn, const float* host_params);
+int32_t mac_cnn_get_params(void* cnn, float* host_params);
+/* x: [n][2][H][W], y: [n][1][H][W] device FP32, H and W multiples of 8. */
+int32_t mac_cnn_forward(void* cnn, const float* x, float* y, int32_t n, int32_t H, int32_t W);
+/* Mean squared error against target [n][1][H][W] and its gradient (device, FP32). */
+int32_t mac_cnn_loss_grad(void* cnn, const float* x, const float* target, int32_t n, int32_t H, int32_t W,
+                          float* grad_out, float* loss_out);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on that loss. */
+int32_t mac_cnn_train_step(void* cnn, const float* x, const float* target, int32_t n, int32_t H, int32_t W,
+                           float lr, float* loss_out);
+/* Schur-CG starting pressure when the caller passes none: MAC_WARM_OFF (zero, as the
+ * reference), MAC_WARM_NETWORK (the attached network's guess; alpha is the pressure /
+ * right-hand-side scale it was trained with) or MAC_WARM_PREVIOUS (the previous solve's
+ * pressures).  Either guess g is scaled by theta = (g, b)/(g, S g) before CG starts. */
+#define MAC_WARM_OFF 0
+#define MAC_WARM_NETWORK 1
+#define MAC_WARM_PREVIOUS 2
+int32_t mac_stokes_set_warm_start(void* handle, int32_t mode, void* cnn, double alpha);
+/* Forget the previous-stage pressures (a new run). */
+int32_t mac_stokes_reset_history(void* handle);
+/* Schur right-hand sides b of the last FP64 solve, ncols columns of np (device). */
+int32_t mac_stokes_last_schur_rhs(void* handle, double* out, int64_t ld, int32_t ncols);
+/* Network inputs / targets / scales exactly as the solver prepares them (device). */
+int32_t mac_warm_prepare(int32_t nx, int32_t ny, const double* b, int64_t ldb, const double* pprev, int64_t ldp,
+                         const double* target, int64_t ldt, int32_t ncols, double alpha, float* x, float* t,
+                         double* scale);
 
 /* ---- single operators, device pointers only (parity tests and host callers) ---- */
 

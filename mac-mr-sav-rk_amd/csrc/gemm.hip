@@ -10,6 +10,7 @@
 //   k_gemm_f64  : v_mfma_f64_16x16x4f64, FP64 in / FP64 out (the plain FP64 solve)
 //   k_gemm_bf16 : v_mfma_f32_16x16x32_bf16, A converted to bf16 on the way into LDS,
 //                 FP32 accumulation, output fp64 / fp32 / bf16 (the mixed solve)
+#include <cstdint>
 #include <type_traits>
 
 #include "internal.h"
@@ -142,172 +143,257 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmBatch batch) {
 }
 
 // ------------------------------------------------------------------- BF16 GEMM
-// 64x64 tile, 256 threads = 2x2 waves of 32x32 (2x2 MFMA 16x16x32 blocks each).
-// LDS rows are k-contiguous for both operands (the bf16 MFMA takes 8 consecutive k
-// per lane), padded by 8 elements to spread banks.
+// BM x BN tile (128x128 for large batches, 64x64 otherwise), K step 32 (one MFMA k-step
+// of v_mfma_f32_16x16x32_bf16), 256 threads = 2x2 waves, each wave (BM/2) x (BN/2).
+// LDS rows are k-contiguous for both operands (the MFMA takes 8 consecutive k per
+// lane), padded to 40 elements (80 bytes) to spread banks; double-buffered, the
+// global loads of step k+1 are in flight while step k computes.
 //
-// SPLIT ("bf16x3"): every A element is split on load into hi = bf16(a) and
-// lo = bf16(a - hi), the matrix is supplied as Q_hi + Q_lo, and each k-step issues
-// A_hi.B_hi + A_hi.B_lo + A_lo.B_hi into the FP32 accumulator, which keeps about 16
-// significant bits of both operands instead of 8.
-constexpr int BF_BM = 64, BF_BN = 64;
-
-template <bool SPLIT>
-struct BfTile {
-  static constexpr int BK = SPLIT ? 32 : 64;
-  static constexpr int LDS = BK + 8;
-  static constexpr int A_PER_THREAD = BF_BM * BK / 256;  // elements of A each thread stages
+// SPLIT ("bf16x3"): each A element is split on its way into LDS into hi = bf16(a) and
+// lo = bf16(a - hi), the matrix arrives as Q_hi + Q_lo, and every k-step issues
+// A_lo.B_hi + A_hi.B_lo + A_hi.B_hi into the FP32 accumulator: about 16 significant
+// bits of both operands instead of 8.
+//
+// Tiles are visited in groups of 4 tile-rows (all tile-columns of a group before the
+// next), and consecutive tiles go to the same XCD, so each XCD's L2 holds a few A
+// row panels and the B panels they meet.
+template <int BM_, int BN_, bool SPLIT_>
+struct Cfg {
+  static constexpr int BM = BM_, BN = BN_, BK = 32, LDS = 40;
+  static constexpr int FM = BM / 32, FN = BN / 32;     // 16x16 MFMA blocks per wave
+  static constexpr int AE = BM * BK / 256;             // A elements staged per thread
+  static constexpr int BE = BN * BK / 256;             // B elements staged per thread
+  static constexpr int PARTS = SPLIT_ ? 2 : 1;
 };
 
 __device__ inline float to_f(double v) { return (float)v; }
 __device__ inline float to_f(float v) { return v; }
 __device__ inline float to_f(__bf16 v) { return (float)v; }
 
-template <int EPI, typename TA, typename TC, bool SPLIT>
+template <typename TC>
+__device__ inline void store4(TC* p, const f4& v);
+template <>
+__device__ inline void store4<float>(float* p, const f4& v) {
+  *reinterpret_cast<f4*>(p) = v;
+}
+template <>
+__device__ inline void store4<__bf16>(__bf16* p, const f4& v) {
+  typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+  bf4 o;
+  o[0] = (__bf16)v[0]; o[1] = (__bf16)v[1]; o[2] = (__bf16)v[2]; o[3] = (__bf16)v[3];
+  *reinterpret_cast<bf4*>(p) = o;
+}
+template <>
+__device__ inline void store4<double>(double* p, const f4& v) {
+  p[0] = v[0]; p[1] = v[1]; p[2] = v[2]; p[3] = v[3];
+}
+
+// AMODE: how A reaches LDS.  0 = bf16 rows (16-byte loads of 8), 1 = fp32 rows with
+// lda % 4 == 0 and 16-byte aligned bases (float4 loads), 2 = one element per load.
+template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE>
 __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
-  using T = BfTile<SPLIT>;
-  constexpr int BK = T::BK, LDS = T::LDS, NA = T::A_PER_THREAD;
-  constexpr int NB = BF_BN * BK / 256;  // bf16 elements of B per thread (16 or 8)
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][(SPLIT ? 2 : 1) * BF_BM * LDS];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][(SPLIT ? 2 : 1) * BF_BN * LDS];
+  using C = Cfg<BM, BN, SPLIT>;
+  constexpr int BK = C::BK, LDS = C::LDS, FM = C::FM, FN = C::FN, AE = C::AE, BE = C::BE;
+  constexpr int AV = AMODE == 0 ? 8 : (AMODE == 1 ? 4 : 1);   // A elements per load
+  constexpr int NA = AE / AV;                                   // A loads per thread
+  constexpr int NB = BE / 8;                                    // B loads per thread per part
+  constexpr int APR = BK / AV;                                  // A loads per tile row
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][C::PARTS * BM * LDS];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][C::PARTS * BN * LDS];
+
   const int t = xcd_tile(blockIdx.x, batch.total_tiles);
   const GemmEntry& E = batch.e[find_entry(batch, t)];
   const int lt = t - E.tile_begin;
-  const int m0 = (lt / E.tiles_n) * BF_BM, n0 = (lt % E.tiles_n) * BF_BN;
-  const TA* __restrict__ A = static_cast<const TA*>(E.A);
-  const __bf16* __restrict__ B = static_cast<const __bf16*>(E.Bt);
-  const __bf16* __restrict__ Blo = static_cast<const __bf16*>(E.Bt_lo);
+  const int GROUP = batch.group;
+  const int per_group = GROUP * E.tiles_n;
+  const int first_m = (lt / per_group) * GROUP;
+  const int gsize = min(E.tiles_m - first_m, GROUP);
+  const int tm = first_m + (lt % per_group) % gsize;
+  const int tn = (lt % per_group) / gsize;
+  const int m0 = tm * BM, n0 = tn * BN;
+
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
   const int li = lane & 15, lg = lane >> 4;
   const int M = E.M, N = E.N, K = E.K;
-  // B staging: 4 threads per row, NB contiguous bf16 each (one or two 16-byte loads).
-  const int brow_i = tid >> 2, bk = (tid & 3) * NB;
-  const bool b_row = (n0 + brow_i) < N;
-  const __bf16* brow = B + (int64_t)(n0 + brow_i) * E.ldb;
-  const __bf16* brow_lo = SPLIT ? Blo + (int64_t)(n0 + brow_i) * E.ldb : nullptr;
-  constexpr bool A_VEC = std::is_same<TA, __bf16>::value && !SPLIT;
 
-  f4 acc[2][2];
+  // Per-thread staging addresses, fixed for the whole k loop.  Rows past M (N) are
+  // clamped to the last row: their products only reach output rows (columns) that the
+  // epilogue discards, so no per-element guard is needed except on the k tail.
+  const TA* a_ptr[NA];
+  int a_col[NA];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int e = 0; e < NA; ++e) {
+    const int v = e * 256 + tid, r = v / APR, c = (v % APR) * AV;
+    a_ptr[e] = static_cast<const TA*>(E.A) + (int64_t)min(m0 + r, M - 1) * E.lda + c;
+    a_col[e] = c;
+  }
+  const __bf16* b_ptr[C::PARTS][NB];
+  int b_col[NB];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  float ra[NA];
-  __attribute__((aligned(16))) __bf16 rav[16];
-  __attribute__((aligned(16))) __bf16 rb[NB];
-  __attribute__((aligned(16))) __bf16 rbl[SPLIT ? NB : 1];
+  for (int e = 0; e < NB; ++e) {
+    const int v = e * 256 + tid, r = v >> 2, c = (v & 3) * 8;
+    const int64_t off = (int64_t)min(n0 + r, N - 1) * E.ldb + c;
+    b_ptr[0][e] = static_cast<const __bf16*>(E.Bt) + off;
+    if constexpr (SPLIT) b_ptr[1][e] = static_cast<const __bf16*>(E.Bt_lo) + off;
+    b_col[e] = c;
+  }
+
+  f4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // One k-tile of operands staged in registers between global memory and LDS.
+  struct Stage {
+    float ra[AMODE == 0 ? 1 : AE];
+    __attribute__((aligned(16))) __bf16 rab[AMODE == 0 ? AE : 8];
+    __attribute__((aligned(16))) __bf16 rb[C::PARTS][BE];
+  };
+  Stage S0, S1;
   const int nk = (K + BK - 1) / BK;
+  const bool ragged = (K % BK) != 0;
 
-  auto load_b = [&](const __bf16* row, int k, __bf16* dst) {
-    if (b_row && k + NB <= K) {
-      const uint4* p = reinterpret_cast<const uint4*>(row + k);
-#pragma unroll
-      for (int v = 0; v < NB / 8; ++v) reinterpret_cast<uint4*>(dst)[v] = p[v];
-    } else {
-#pragma unroll
-      for (int i = 0; i < NB; ++i) dst[i] = (b_row && k + i < K) ? row[k + i] : (__bf16)0.0f;
-    }
-  };
-  auto gload = [&](int kt) {
+  auto gload = [&](int kt, Stage& st) {
     const int k0 = kt * BK;
-    if constexpr (A_VEC) {
-      // bf16 rows (ld % 8 == 0, 16-byte aligned): 4 threads x 16 contiguous elements.
-      const int arow_i = tid >> 2, ak = (tid & 3) * 16;
-      const bool ok = (m0 + arow_i) < M;
-      const __bf16* row = reinterpret_cast<const __bf16*>(A) + (int64_t)(m0 + arow_i) * E.lda;
-      const int k = k0 + ak;
-      if (ok && k + 16 <= K) {
-        const uint4* p = reinterpret_cast<const uint4*>(row + k);
-        reinterpret_cast<uint4*>(rav)[0] = p[0];
-        reinterpret_cast<uint4*>(rav)[1] = p[1];
-      } else {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) rav[i] = (ok && k + i < K) ? row[k + i] : (__bf16)0.0f;
-      }
-    } else {
-      // Wide or split A: consecutive threads read consecutive k of one row (coalesced).
+    if (!(ragged && kt == nk - 1)) {
 #pragma unroll
       for (int e = 0; e < NA; ++e) {
-        const int L = e * 256 + tid, r = L / BK, c = L % BK;
-        const bool ok = (m0 + r) < M && (k0 + c) < K;
-        ra[e] = ok ? to_f(A[(int64_t)(m0 + r) * E.lda + k0 + c]) : 0.f;
+        if constexpr (AMODE == 0) {
+          *reinterpret_cast<uint4*>(&st.rab[e * 8]) = *reinterpret_cast<const uint4*>(a_ptr[e] + k0);
+        } else if constexpr (AMODE == 1) {
+          const float4 f = *reinterpret_cast<const float4*>(a_ptr[e] + k0);
+          st.ra[e * 4 + 0] = f.x; st.ra[e * 4 + 1] = f.y; st.ra[e * 4 + 2] = f.z; st.ra[e * 4 + 3] = f.w;
+        } else {
+          st.ra[e] = to_f(a_ptr[e][k0]);
+        }
       }
-    }
-    load_b(brow, k0 + bk, rb);
-    if constexpr (SPLIT) load_b(brow_lo, k0 + bk, rbl);
-  };
-  auto sstore = [&](int buf) {
-    if constexpr (A_VEC) {
-      const int arow_i = tid >> 2, ak = (tid & 3) * 16;
-      uint4* da = reinterpret_cast<uint4*>(&As[buf][arow_i * LDS + ak]);
-      da[0] = reinterpret_cast<const uint4*>(rav)[0];
-      da[1] = reinterpret_cast<const uint4*>(rav)[1];
+#pragma unroll
+      for (int part = 0; part < C::PARTS; ++part)
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+          *reinterpret_cast<uint4*>(&st.rb[part][e * 8]) = *reinterpret_cast<const uint4*>(b_ptr[part][e] + k0);
     } else {
+      // Last, partial k-tile: zero beyond K (padding may hold anything).
 #pragma unroll
-      for (int e = 0; e < NA; ++e) {
-        const int L = e * 256 + tid, r = L / BK, c = L % BK;
-        const __bf16 hi = (__bf16)ra[e];
-        As[buf][r * LDS + c] = hi;
-        if constexpr (SPLIT) As[buf][BF_BM * LDS + r * LDS + c] = (__bf16)(ra[e] - (float)hi);
-      }
-    }
-    uint4* db = reinterpret_cast<uint4*>(&Bs[buf][brow_i * LDS + bk]);
+      for (int e = 0; e < NA; ++e)
 #pragma unroll
-    for (int v = 0; v < NB / 8; ++v) db[v] = reinterpret_cast<const uint4*>(rb)[v];
-    if constexpr (SPLIT) {
-      uint4* dbl = reinterpret_cast<uint4*>(&Bs[buf][BF_BN * LDS + brow_i * LDS + bk]);
+        for (int q = 0; q < AV; ++q) {
+          const bool ok = k0 + a_col[e] + q < K;
+          if constexpr (AMODE == 0) st.rab[e * 8 + q] = ok ? a_ptr[e][k0 + q] : (__bf16)0.0f;
+          else st.ra[e * AV + q] = ok ? to_f(a_ptr[e][k0 + q]) : 0.f;
+        }
 #pragma unroll
-      for (int v = 0; v < NB / 8; ++v) dbl[v] = reinterpret_cast<const uint4*>(rbl)[v];
+      for (int part = 0; part < C::PARTS; ++part)
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            st.rb[part][e * 8 + q] = (k0 + b_col[e] + q < K) ? b_ptr[part][e][k0 + q] : (__bf16)0.0f;
     }
   };
 
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) gload(kt + 1);
+  auto sstore = [&](int buf, const Stage& st) {
+    __bf16* as = As[buf];
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 32) {
-      const __bf16* as = &As[cur][0];
-      const __bf16* bs = &Bs[cur][0];
-      const bf8 a0 = *reinterpret_cast<const bf8*>(&as[(wm + li) * LDS + kk + lg * 8]);
-      const bf8 a1 = *reinterpret_cast<const bf8*>(&as[(wm + 16 + li) * LDS + kk + lg * 8]);
-      const bf8 b0 = *reinterpret_cast<const bf8*>(&bs[(wn + li) * LDS + kk + lg * 8]);
-      const bf8 b1 = *reinterpret_cast<const bf8*>(&bs[(wn + 16 + li) * LDS + kk + lg * 8]);
-      if constexpr (SPLIT) {
-        const bf8 l0 = *reinterpret_cast<const bf8*>(&as[BF_BM * LDS + (wm + li) * LDS + kk + lg * 8]);
-        const bf8 l1 = *reinterpret_cast<const bf8*>(&as[BF_BM * LDS + (wm + 16 + li) * LDS + kk + lg * 8]);
-        const bf8 c0 = *reinterpret_cast<const bf8*>(&bs[BF_BN * LDS + (wn + li) * LDS + kk + lg * 8]);
-        const bf8 c1 = *reinterpret_cast<const bf8*>(&bs[BF_BN * LDS + (wn + 16 + li) * LDS + kk + lg * 8]);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l1, b1, acc[1][1], 0, 0, 0);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, c0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, c1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, c0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, c1, acc[1][1], 0, 0, 0);
+    for (int e = 0; e < NA; ++e) {
+      const int v = e * 256 + tid, r = v / APR, c = (v % APR) * AV;
+      if constexpr (AMODE == 0) {
+        *reinterpret_cast<uint4*>(&as[r * LDS + c]) = *reinterpret_cast<const uint4*>(&st.rab[e * 8]);
+      } else if constexpr (AMODE == 1) {
+        typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+        bf4 hi, lo;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          hi[q] = (__bf16)st.ra[e * 4 + q];
+          lo[q] = (__bf16)(st.ra[e * 4 + q] - (float)hi[q]);
+        }
+        *reinterpret_cast<bf4*>(&as[r * LDS + c]) = hi;
+        if constexpr (SPLIT) *reinterpret_cast<bf4*>(&as[BM * LDS + r * LDS + c]) = lo;
+      } else {
+        const __bf16 hi = (__bf16)st.ra[e];
+        as[r * LDS + c] = hi;
+        if constexpr (SPLIT) as[BM * LDS + r * LDS + c] = (__bf16)(st.ra[e] - (float)hi);
       }
-      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
     }
-    if (kt + 1 < nk) sstore(cur ^ 1);
+#pragma unroll
+    for (int part = 0; part < C::PARTS; ++part) {
+      __bf16* bs = Bs[buf] + part * BN * LDS;
+#pragma unroll
+      for (int e = 0; e < NB; ++e) {
+        const int v = e * 256 + tid, r = v >> 2, c = (v & 3) * 8;
+        *reinterpret_cast<uint4*>(&bs[r * LDS + c]) = *reinterpret_cast<const uint4*>(&st.rb[part][e * 8]);
+      }
+    }
+  };
+
+  auto compute = [&](int cur) {
+    const __bf16* as = As[cur];
+    const __bf16* bs = Bs[cur];
+    bf8 ah[FM], bh[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) ah[i] = *reinterpret_cast<const bf8*>(&as[(wm + i * 16 + li) * LDS + lg * 8]);
+#pragma unroll
+    for (int j = 0; j < FN; ++j) bh[j] = *reinterpret_cast<const bf8*>(&bs[(wn + j * 16 + li) * LDS + lg * 8]);
+    if constexpr (SPLIT) {
+      bf8 al[FM], bl[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        al[i] = *reinterpret_cast<const bf8*>(&as[BM * LDS + (wm + i * 16 + li) * LDS + lg * 8]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bl[j] = *reinterpret_cast<const bf8*>(&bs[BN * LDS + (wn + j * 16 + li) * LDS + lg * 8]);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+  };
+
+  // Pipeline: LDS buffers L0/L1 alternate; two register stages keep the global loads
+  // of tiles k+1 and k+2 in flight while tile k computes.
+  gload(0, S0);
+  if (nk > 1) gload(1, S1);
+  sstore(0, S0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt += 2) {
+    // tile kt is in L0; tile kt+1 (if any) is in flight in S1
+    if (kt + 2 < nk) gload(kt + 2, S0);
+    compute(0);
+    if (kt + 1 < nk) sstore(1, S1);
+    __syncthreads();
+    if (kt + 1 >= nk) break;
+    if (kt + 3 < nk) gload(kt + 3, S1);
+    compute(1);
+    if (kt + 2 < nk) sstore(0, S0);
     __syncthreads();
   }
+
   // f32-result MFMA layout: register r of lane l holds row 4*(l/16) + r, column l%16.
+  const bool vec_t = EPI == EPI_STORE && batch.trans_c && (E.ldc & 3) == 0;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FN; ++j) {
+      const int mb = m0 + wm + i * 16 + 4 * lg, n = n0 + wn + j * 16 + li;
+      if (vec_t && mb + 3 < M && n < N) {
+        // Transposed tile: the lane's 4 rows are 4 consecutive elements of row n of C^T.
+        store4<TC>(static_cast<TC*>(E.C) + (int64_t)n * E.ldc + mb, acc[i][j]);
+      } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        epilogue<EPI, TC>(batch, E, m0 + wm + i * 16 + 4 * lg + r, n0 + wn + j * 16 + li,
-                          (double)acc[i][j][r]);
+        for (int r = 0; r < 4; ++r) epilogue<EPI, TC>(batch, E, mb + r, n, (double)acc[i][j][r]);
+      }
+    }
 }
 
 int tile_batch(GemmBatch& b, int bm, int bn) {
@@ -323,22 +409,54 @@ int tile_batch(GemmBatch& b, int bm, int bn) {
   return total;
 }
 
-template <typename TA, typename TC, bool SPLIT>
-void launch_bf16(GemmBatch& b, int total, hipStream_t s) {
+template <typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE>
+void launch_bf16_mode(GemmBatch& b, int total, hipStream_t s) {
   switch (b.epi) {
-    case EPI_STORE: k_gemm_bf16<EPI_STORE, TA, TC, SPLIT><<<total, 256, 0, s>>>(b); break;
-    case EPI_SHIFT: k_gemm_bf16<EPI_SHIFT, TA, TC, SPLIT><<<total, 256, 0, s>>>(b); break;
-    case EPI_LAMBDA: k_gemm_bf16<EPI_LAMBDA, TA, TC, SPLIT><<<total, 256, 0, s>>>(b); break;
-    case EPI_ACCUM: k_gemm_bf16<EPI_ACCUM, TA, TC, SPLIT><<<total, 256, 0, s>>>(b); break;
-    default: k_gemm_bf16<EPI_COMBINE, TA, TC, SPLIT><<<total, 256, 0, s>>>(b); break;
+    case EPI_STORE: k_gemm_bf16<EPI_STORE, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
+    case EPI_SHIFT: k_gemm_bf16<EPI_SHIFT, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
+    case EPI_LAMBDA: k_gemm_bf16<EPI_LAMBDA, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
+    case EPI_ACCUM: k_gemm_bf16<EPI_ACCUM, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
+    default: k_gemm_bf16<EPI_COMBINE, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
   }
 }
 
+template <typename TA, typename TC, bool SPLIT, int BM, int BN>
+void launch_bf16_tile(GemmBatch& b, hipStream_t s) {
+  const int total = tile_batch(b, BM, BN);
+  if constexpr (std::is_same<TA, __bf16>::value) {
+    launch_bf16_mode<TA, TC, SPLIT, BM, BN, 0>(b, total, s);
+  } else if constexpr (std::is_same<TA, float>::value) {
+    bool vec = true;
+    for (int i = 0; i < b.count; ++i)
+      vec = vec && (b.e[i].lda % 4 == 0) && (reinterpret_cast<uintptr_t>(b.e[i].A) % 16 == 0);
+    if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 1>(b, total, s);
+    else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2>(b, total, s);
+  } else {
+    launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2>(b, total, s);
+  }
+}
+
+// Tile shape: 0 = automatic, else 64 (64x64), 12864 (128x64), 128 (128x128).
+int g_bf16_tile = 0;
+int g_bf16_group = 4;
+
+template <typename TA, typename TC, bool SPLIT>
+void launch_bf16(GemmBatch& b, hipStream_t s) {
+  if (b.group <= 0) b.group = g_bf16_group;
+  int tile = g_bf16_tile;
+  // Measured at 1024^2, four blocks per launch: FP64-staged A prefers 128x128 (fewer
+  // re-reads of the wide operand), FP32/bf16-staged A prefers 128x64 (two tiles per CU).
+  if (tile == 0) tile = std::is_same<TA, double>::value ? 128 : 12864;
+  if (tile == 128) launch_bf16_tile<TA, TC, SPLIT, 128, 128>(b, s);
+  else if (tile == 12864) launch_bf16_tile<TA, TC, SPLIT, 128, 64>(b, s);
+  else launch_bf16_tile<TA, TC, SPLIT, 64, 64>(b, s);
+}
+
 template <typename TA, bool SPLIT>
-void launch_bf16_c(GemmBatch& b, int c_type, int total, hipStream_t s) {
-  if (c_type == DT_F64) launch_bf16<TA, double, SPLIT>(b, total, s);
-  else if (c_type == DT_F32) launch_bf16<TA, float, SPLIT>(b, total, s);
-  else launch_bf16<TA, __bf16, SPLIT>(b, total, s);
+void launch_bf16_c(GemmBatch& b, int c_type, hipStream_t s) {
+  if (c_type == DT_F64) launch_bf16<TA, double, SPLIT>(b, s);
+  else if (c_type == DT_F32) launch_bf16<TA, float, SPLIT>(b, s);
+  else launch_bf16<TA, __bf16, SPLIT>(b, s);
 }
 
 }  // namespace
@@ -354,17 +472,21 @@ void gemm_f64(GemmBatch& b, hipStream_t s) {
   }
 }
 
+void set_bf16_tiling(int tile, int group) {
+  g_bf16_tile = tile;
+  g_bf16_group = group;
+}
+
 void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s) {
   if (b.count <= 0) return;
-  const int total = tile_batch(b, BF_BM, BF_BN);
   if (split) {
     // Split operands come from FP32/FP64 data (a bf16 input has no low part).
-    if (a_type == DT_F64) launch_bf16_c<double, true>(b, c_type, total, s);
-    else launch_bf16_c<float, true>(b, c_type, total, s);
+    if (a_type == DT_F64) launch_bf16_c<double, true>(b, c_type, s);
+    else launch_bf16_c<float, true>(b, c_type, s);
   } else {
-    if (a_type == DT_F64) launch_bf16_c<double, false>(b, c_type, total, s);
-    else if (a_type == DT_F32) launch_bf16_c<float, false>(b, c_type, total, s);
-    else launch_bf16_c<__bf16, false>(b, c_type, total, s);
+    if (a_type == DT_F64) launch_bf16_c<double, false>(b, c_type, s);
+    else if (a_type == DT_F32) launch_bf16_c<float, false>(b, c_type, s);
+    else launch_bf16_c<__bf16, false>(b, c_type, s);
   }
 }
 

@@ -40,6 +40,7 @@ struct GemmBatch {
   int total_tiles;
   int epi;
   int trans_c;
+  int group;        // tile-rows per visiting group (set by the launcher)
   double s0, s1, thresh;
 };
 
@@ -48,6 +49,8 @@ void gemm_f64(GemmBatch& b, hipStream_t s);
 // a_type: DT_F64 / DT_F32 / DT_BF16 for A and aux/C per c_type; Bt is always bf16.
 // split: three-product hi/lo form (A must be DT_F32 or DT_F64, Bt_lo set).
 void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s);
+// Tile shape (0 auto, 64, 12864, 128) and tile-row grouping of the bf16 GEMM.
+void set_bf16_tiling(int tile, int group);
 
 // ------------------------------------------------------------- stencil kernels
 // Internal velocity layout: u block [ny][ldu], v block [ny-1][ldv] (ld >= row length),
@@ -94,6 +97,9 @@ void finalize(const double* partials, int nblk, int ncols, int mode, double* out
 void sub_mean_dot(double* x, int64_t n, int64_t stride, int ncols, const double* sums, const double* y,
                   double* partials, const CgState* st, hipStream_t s);
 void fill(double* x, int64_t n, double v, hipStream_t s);
+// x = theta g, r = b - theta q with theta = gb/gq per column (x holds g, q holds S g).
+void theta_start(double* x, double* r, const double* b, const double* q, const double* gb, const double* gq,
+                 int64_t n, int64_t stride, int ncols, const CgState* st, hipStream_t s);
 // out = a - b column-wise (skips inactive columns when st is given).
 void sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t stride, int ncols,
               const CgState* st, hipStream_t s);
@@ -116,5 +122,15 @@ void defect_max(const double* vel, const double* f, const double* p, const GridD
                 double mass, double visc, double* partials3, hipStream_t s, double* residual_out = nullptr);
 // x[c] += y[c] (FP64, contiguous columns).
 void add_cols(double* x, const double* y, int64_t n, int ncols, hipStream_t s);
+
+// Learned warm start (cnn.hip): grid sizes it serves, input preparation (pool by 4,
+// normalise) and the guess (network forward, bilinear up by 4, rescale).
+bool warm_shape_ok(int nx, int ny);
+void warm_prepare(const double* b, int64_t b_stride, const double* pprev, int64_t p_stride, const double* target,
+                  int64_t t_stride, int ncols, int nx, int ny, double alpha, float* x, float* t, double* s,
+                  hipStream_t st);
+void warm_guess(void* cnn, const double* b, int64_t b_stride, const double* pprev, int64_t p_stride, int ncols,
+                int nx, int ny, double alpha, float* x, float* y, double* s, double* guess, int64_t g_stride,
+                hipStream_t st);
 
 }  // namespace macs

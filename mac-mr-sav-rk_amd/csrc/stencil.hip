@@ -164,6 +164,21 @@ __global__ void k_sub_cols(const double* a, const double* b, double* out, int64_
     out[off + i] = a[off + i] - b[off + i];
 }
 
+// Galerkin-scaled start: theta = (g, b) / (g, S g) per column (0 when (g, S g) <= 0);
+// x = theta g, r = b - theta S g.  x holds g and q holds S g on entry.
+__global__ void k_theta_start(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ b,
+                              const double* __restrict__ q, const double* gb, const double* gq, int64_t n,
+                              int64_t stride, const CgState* st) {
+  const int c = blockIdx.y;
+  if (st != nullptr && !st[c].active) return;
+  const double theta = gq[c] > 0.0 ? gb[c] / gq[c] : 0.0;
+  const int64_t off = (int64_t)c * stride;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    x[off + i] = theta * x[off + i];
+    r[off + i] = b[off + i] - theta * q[off + i];
+  }
+}
+
 __global__ void k_fill(double* x, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) x[i] = v;
 }
@@ -383,6 +398,11 @@ void sub_mean_dot(double* x, int64_t n, int64_t stride, int ncols, const double*
 void sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t stride, int ncols,
               const CgState* st, hipStream_t s) {
   k_sub_cols<<<col_grid(n, ncols), kThreads, 0, s>>>(a, b, out, n, stride, st);
+}
+
+void theta_start(double* x, double* r, const double* b, const double* q, const double* gb, const double* gq,
+                 int64_t n, int64_t stride, int ncols, const CgState* st, hipStream_t s) {
+  k_theta_start<<<col_grid(n, ncols), kThreads, 0, s>>>(x, r, b, q, gb, gq, n, stride, st);
 }
 
 void fill(double* x, int64_t n, double v, hipStream_t s) {

@@ -416,7 +416,39 @@ class Solver {
     double residual = 0, divergence = 0, vmax = 0, fmax = 0;
     std::vector<int> iterations;
     int sweeps = 1;
+    bool warm = false;
   };
+
+  // ------------------------------------------------------- learned warm start
+  void set_warm_start(int mode, void* cnn, double alpha) {
+    warm_mode_ = mode;
+    cnn_ = cnn;
+    cnn_alpha_ = alpha;
+  }
+  void reset_history() { prev_ncols_ = 0; }
+  const double* schur_rhs() const { return b_; }
+  int last_ncols() const { return ncols_cur_; }
+
+  // Fills io_g_ with the network's pressure guess from the Schur right-hand side b and
+  // the previous solve's pressures (same column count), if a network is attached and
+  // the grid is one it serves.
+  bool learned_guess(const double* b, int ncols) {
+    if (warm_mode_ == 2) {
+      // Previous solve's pressures as the guess (same column count only).
+      if (prev_ncols_ != ncols) return false;
+      copy_cols(pprev_, np_, io_g_, np_, np_, ncols, stream_);
+      return true;
+    }
+    if (warm_mode_ != 1 || cnn_ == nullptr || !warm_shape_ok(g_.nx, g_.ny)) return false;
+    const double* prev = prev_ncols_ == ncols ? pprev_ : nullptr;
+    warm_guess(cnn_, b, np_, prev, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, io_g_, np_,
+               stream_);
+    return true;
+  }
+  void remember_pressure(const double* p, int ncols) {
+    copy_cols(p, np_, pprev_, np_, np_, ncols, stream_);
+    prev_ncols_ = ncols;
+  }
 
   Outcome solve(const double* f, int64_t f_stride, int ncols, double mass, double visc, double* vel,
                 int64_t vel_stride, double* pres, int64_t pres_stride, const double* guess, int64_t guess_stride) {
@@ -443,25 +475,12 @@ class Solver {
     HIP_CHECK(hipMemsetAsync(x_, 0, sizeof(double) * np_ * ncols, stream_));
     copy_cols(b_, np_, r_, np_, np_, ncols, stream_);
     HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
+    out.warm = false;
     if (guess != nullptr) {
-      std::vector<CgState> st(ncols);
-      HIP_CHECK(hipMemcpyAsync(st.data(), cg_, sizeof(CgState) * ncols, hipMemcpyDeviceToHost, stream_));
-      HIP_CHECK(hipStreamSynchronize(stream_));
-      std::vector<int> live;
-      for (int c = 0; c < ncols; ++c) {
-        if (!st[c].active) continue;  // zero right-hand side: pressure stays zero
-        live.push_back(c);
-        HIP_CHECK(hipMemcpyAsync(x_ + (int64_t)c * np_, guess + (int64_t)c * guess_stride, sizeof(double) * np_,
-                                 hipMemcpyDeviceToDevice, stream_));
-      }
-      if (!live.empty()) {
-        use_state_ = true;
-        schur(x_, q_, live, mass, visc, nullptr);
-        use_state_ = false;
-        sub_cols(b_, q_, r_, np_, np_, ncols, cg_, stream_);
-        reduce(r_, r_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
-        finalize(partials_, nbp, ncols, RED_SUM, rr_, stream_);
-      }
+      start_from(guess, guess_stride, ncols, mass, visc);
+    } else if (learned_guess(b_, ncols)) {
+      out.warm = true;
+      start_from(io_g_, np_, ncols, mass, visc, true);
     }
     pcg(ncols, mass, visc, out.iterations, true);
     // p = x - mean(x)
@@ -469,6 +488,7 @@ class Solver {
     finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
     sub_mean_dot(x_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
     copy_cols(x_, np_, pres, pres_stride, np_, ncols, stream_);
+    remember_pressure(x_, ncols);
     // u = L^-1 (f + D^T p)
     divergence_transpose(x_, np_, nullptr, f, f_stride, DT_F64, w_, packed_, g_, ncols, nullptr, stream_);
     velocity_inverse(w_, size_, vel, vel_stride, all, mass, visc);
@@ -539,6 +559,11 @@ class Solver {
       HIP_CHECK(hipMemsetAsync(x_, 0, sizeof(double) * np_ * ncols, stream_));
       copy_cols(b_, np_, r_, np_, np_, ncols, stream_);
       HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
+      if (sweep == 0 && guess == nullptr && learned_guess(b_, ncols)) {
+        // First sweep: the network's pressure starts the inner Schur CG (p was zero).
+        out.warm = true;
+        start_from(io_g_, np_, ncols, mass, visc, true);
+      }
       pcg(ncols, mass, visc, inner, false);
       for (int c = 0; c < ncols; ++c) out.iterations[c] += inner[c];
       add_cols(pacc_, x_, np_, ncols, stream_);
@@ -557,6 +582,7 @@ class Solver {
     sub_mean_dot(pacc_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
     copy_cols(pacc_, np_, pres, pres_stride, np_, ncols, stream_);
     copy_cols(uacc_, size_, vel, vel_stride, size_, ncols, stream_);
+    remember_pressure(pacc_, ncols);
     return out;
   }
 
@@ -604,6 +630,39 @@ class Solver {
     }
     finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
     sub_mean_dot(qout, np_, np_, ncols, sums_, dot_with, partials_, st, stream_);
+  }
+
+  // x = guess, r = b - S x, rr = (r, r) on the columns whose CG runs (a column with a
+  // zero right-hand side keeps x = 0, as the reference skips its CG).
+  void start_from(const double* guess, int64_t guess_stride, int ncols, double mass, double visc,
+                  bool scale_guess = false) {
+    const int nbp = reduce_blocks(np_);
+    std::vector<CgState> st(ncols);
+    HIP_CHECK(hipMemcpyAsync(st.data(), cg_, sizeof(CgState) * ncols, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    std::vector<int> live;
+    for (int c = 0; c < ncols; ++c) {
+      if (!st[c].active) continue;
+      live.push_back(c);
+      HIP_CHECK(hipMemcpyAsync(x_ + (int64_t)c * np_, guess + (int64_t)c * guess_stride, sizeof(double) * np_,
+                               hipMemcpyDeviceToDevice, stream_));
+    }
+    if (live.empty()) return;
+    use_state_ = true;
+    schur(x_, q_, live, mass, visc, nullptr);
+    use_state_ = false;
+    if (scale_guess) {
+      // Galerkin step along the guess: never a worse start, in the S-norm, than zero.
+      reduce(x_, b_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
+      finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+      reduce(x_, q_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
+      finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);
+      theta_start(x_, r_, b_, q_, sums_, sums2_, np_, np_, ncols, cg_, stream_);
+    } else {
+      sub_cols(b_, q_, r_, np_, np_, ncols, cg_, stream_);
+    }
+    reduce(r_, r_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
+    finalize(partials_, nbp, ncols, RED_SUM, rr_, stream_);
   }
 
   // Schur PCG on all columns at once (x, r, rr, cg_ prepared by the caller).  With
@@ -786,6 +845,11 @@ class Solver {
     p_ = (double*)take(pb);
     q_ = (double*)take(pb);
     fcomp_ = (double*)take(vb);
+    pprev_ = (double*)take(pb);
+    warm_x_ = (float*)take(sizeof(float) * 2 * (np_ / 16 + 64) * ncols);
+    warm_y_ = (float*)take(sizeof(float) * (np_ / 16 + 64) * ncols);
+    warm_s_ = (double*)take(sizeof(double) * (ncols + 4));
+    prev_ncols_ = 0;
     uacc_ = (double*)take(vb);
     ru_ = (double*)take(vb);
     pacc_ = (double*)take(pb);
@@ -838,6 +902,13 @@ class Solver {
   int64_t tstride_ = 0, toff_v_ = 0;
   VelLayout lay_b_{};
   bool mixed_ = false;
+  void* cnn_ = nullptr;
+  double cnn_alpha_ = 1.0;
+  int warm_mode_ = 0;
+  double* pprev_ = nullptr;
+  int prev_ncols_ = 0;
+  float *warm_x_ = nullptr, *warm_y_ = nullptr;
+  double* warm_s_ = nullptr;
   double *partials_ = nullptr, *sums_ = nullptr, *sums2_ = nullptr, *sums3_ = nullptr, *rr_ = nullptr;
   CgState* cg_ = nullptr;
   int* flags_ = nullptr;
@@ -988,6 +1059,7 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
       report->seconds = secs;
       report->sweeps = out.sweeps;
       report->total_iterations = total;
+      report->warm_started = out.warm ? 1 : 0;
     }
     const double tol = s->opts().tolerance;
     if (!is_finite(out.residual) || !is_finite(out.divergence) || !is_finite(out.vmax) || out.residual > tol ||
@@ -1009,6 +1081,39 @@ int32_t mac_stokes_get_profile(void* h, mac_kernel_profile* out) {
   Solver* s = static_cast<Solver*>(h);
   if (!s || !out) return MAC_INVALID_ARGUMENT;
   return guarded(s, [&] { s->profile(&out->gemm_ms, &out->gemm_flops, &out->gemm_bytes, &out->gemm_launches); });
+}
+
+int32_t mac_stokes_set_warm_start(void* h, int32_t mode, void* cnn, double alpha) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s || mode < MAC_WARM_OFF || mode > MAC_WARM_PREVIOUS || (mode == MAC_WARM_NETWORK && !cnn) ||
+      !(alpha > 0) || !is_finite(alpha))
+    return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] { s->set_warm_start(mode, cnn, alpha); });
+}
+
+int32_t mac_stokes_reset_history(void* h) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] { s->reset_history(); });
+}
+
+int32_t mac_stokes_last_schur_rhs(void* h, double* out, int64_t ld, int32_t ncols) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s || !out || ld < s->np() || ncols < 1 || ncols > s->last_ncols()) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] {
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    macs::copy_cols(s->schur_rhs(), s->np(), out, ld, s->np(), ncols, s->stream());
+    HIP_CHECK(hipStreamSynchronize(s->stream()));
+  });
+}
+
+int32_t mac_warm_prepare(int32_t nx, int32_t ny, const double* b, int64_t ldb, const double* pprev, int64_t ldp,
+                         const double* target, int64_t ldt, int32_t ncols, double alpha, float* x, float* t,
+                         double* scale) {
+  if (!macs::warm_shape_ok(nx, ny) || !b || !x || !scale || ncols < 1 || !(alpha > 0) || (target && !t))
+    return MAC_INVALID_ARGUMENT;
+  macs::warm_prepare(b, ldb, pprev, ldp, target, ldt, ncols, nx, ny, alpha, x, t, scale, nullptr);
+  return hipDeviceSynchronize() == hipSuccess ? MAC_OK : MAC_HIP_ERROR;
 }
 
 int32_t mac_apply_divergence(void* h, const double* vel, int64_t ldv, int32_t ncols, double* out, int64_t ldo) {

@@ -18,6 +18,10 @@ MAC_NOT_CONVERGED = 2
 MAC_GATE_FAILED = 3
 MAC_HIP_ERROR = 4
 
+MAC_WARM_OFF = 0
+MAC_WARM_NETWORK = 1
+MAC_WARM_PREVIOUS = 2
+
 MAC_PRECISION_FP64 = 0
 MAC_PRECISION_MIXED = 1
 
@@ -31,6 +35,20 @@ EXPORTED = (
     "mac_stokes_solve",
     "mac_stokes_set_profiling",
     "mac_stokes_get_profile",
+    "mac_cnn_create",
+    "mac_cnn_create_error",
+    "mac_cnn_destroy",
+    "mac_cnn_last_error",
+    "mac_cnn_num_params",
+    "mac_cnn_set_params",
+    "mac_cnn_get_params",
+    "mac_cnn_forward",
+    "mac_cnn_loss_grad",
+    "mac_cnn_train_step",
+    "mac_stokes_set_warm_start",
+    "mac_stokes_reset_history",
+    "mac_stokes_last_schur_rhs",
+    "mac_warm_prepare",
     "mac_apply_divergence",
     "mac_apply_divergence_transpose",
     "mac_velocity_inverse",
@@ -61,6 +79,8 @@ class SolveReport(ctypes.Structure):
         ("seconds", ctypes.c_double),
         ("sweeps", ctypes.c_int32),
         ("total_iterations", ctypes.c_int32),
+        ("warm_started", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
     ]
 
 
@@ -108,6 +128,35 @@ def load() -> ctypes.CDLL:
     lib.mac_stokes_set_profiling.restype = None
     lib.mac_stokes_get_profile.argtypes = [_P, ctypes.POINTER(KernelProfile)]
     lib.mac_stokes_get_profile.restype = _I32
+    F = ctypes.POINTER(ctypes.c_float)
+    lib.mac_cnn_create.argtypes = [_I32, _I32]
+    lib.mac_cnn_create.restype = _P
+    lib.mac_cnn_create_error.argtypes = []
+    lib.mac_cnn_create_error.restype = ctypes.c_char_p
+    lib.mac_cnn_destroy.argtypes = [_P]
+    lib.mac_cnn_destroy.restype = None
+    lib.mac_cnn_last_error.argtypes = [_P]
+    lib.mac_cnn_last_error.restype = ctypes.c_char_p
+    lib.mac_cnn_num_params.argtypes = [_P]
+    lib.mac_cnn_num_params.restype = _I64
+    lib.mac_cnn_set_params.argtypes = [_P, _P]
+    lib.mac_cnn_set_params.restype = _I32
+    lib.mac_cnn_get_params.argtypes = [_P, _P]
+    lib.mac_cnn_get_params.restype = _I32
+    lib.mac_cnn_forward.argtypes = [_P, _P, _P, _I32, _I32, _I32]
+    lib.mac_cnn_forward.restype = _I32
+    lib.mac_cnn_loss_grad.argtypes = [_P, _P, _P, _I32, _I32, _I32, _P, F]
+    lib.mac_cnn_loss_grad.restype = _I32
+    lib.mac_cnn_train_step.argtypes = [_P, _P, _P, _I32, _I32, _I32, ctypes.c_float, F]
+    lib.mac_cnn_train_step.restype = _I32
+    lib.mac_stokes_set_warm_start.argtypes = [_P, _I32, _P, _D]
+    lib.mac_stokes_set_warm_start.restype = _I32
+    lib.mac_stokes_reset_history.argtypes = [_P]
+    lib.mac_stokes_reset_history.restype = _I32
+    lib.mac_stokes_last_schur_rhs.argtypes = [_P, _P, _I64, _I32]
+    lib.mac_stokes_last_schur_rhs.restype = _I32
+    lib.mac_warm_prepare.argtypes = [_I32, _I32, _P, _I64, _P, _I64, _P, _I64, _I32, _D, _P, _P, _P]
+    lib.mac_warm_prepare.restype = _I32
     lib.mac_apply_divergence.argtypes = [_P, _P, _I64, _I32, _P, _I64]
     lib.mac_apply_divergence.restype = _I32
     lib.mac_apply_divergence_transpose.argtypes = [_P, _P, _I64, _I32, _P, _I64]

@@ -90,7 +90,9 @@ class GpuTensorStokes:
         self.last_iterations: tuple[int, ...] = ()
         self.sweeps = 0
         self.last_sweeps = 0
+        self.warm_starts = 0
         self.last_report: dict[str, float] = {}
+        self.warm_start = None
 
     def __del__(self) -> None:
         handle = getattr(self, "_handle", None)
@@ -171,11 +173,44 @@ class GpuTensorStokes:
         self.last_iterations = tuple(int(i) for i in iters)
         self.sweeps += int(report.sweeps)
         self.last_sweeps = int(report.sweeps)
+        self.warm_starts += int(report.warm_started)
         self.last_report = {"residual": report.residual, "divergence_inf": report.divergence_inf,
                             "velocity_max": report.velocity_max, "seconds": report.seconds,
-                            "sweeps": int(report.sweeps), "total_iterations": int(report.total_iterations)}
+                            "sweeps": int(report.sweeps), "total_iterations": int(report.total_iterations),
+                            "warm_started": bool(report.warm_started)}
         _native.raise_for(code, message)
         return self.last_report
+
+    # ---------------------------------------------------- learned warm start
+    def set_warm_start(self, net: Any | None) -> None:
+        """Schur-CG starting pressure when no `pressure_guess` is passed: a trained
+        `WarmStartNet`, the string "previous" (the previous solve's pressures), or None
+        (zero, as the reference).  The guess g is scaled by (g, b)/(g, S g) and starts the
+        Schur CG (FP64) or the first sweep's inner CG (mixed); the CG still runs to the
+        same tolerance, so results are unchanged to tolerance."""
+        self.warm_start = net
+        if net is None:
+            mode, handle, alpha = _native.MAC_WARM_OFF, None, 1.0
+        elif isinstance(net, str):
+            if net != "previous":
+                raise ValueError('warm start must be a WarmStartNet, "previous" or None')
+            mode, handle, alpha = _native.MAC_WARM_PREVIOUS, None, 1.0
+        else:
+            mode, handle, alpha = _native.MAC_WARM_NETWORK, net.handle, float(net.alpha)
+        code = self._lib.mac_stokes_set_warm_start(self._handle, mode, handle, alpha)
+        _native.raise_for(code, self._lib.mac_stokes_last_error(self._handle).decode())
+
+    def reset_history(self) -> None:
+        """Forget the previous solve's pressures (they feed the warm start)."""
+        self._lib.mac_stokes_reset_history(self._handle)
+
+    def last_schur_rhs(self, ncols: int) -> Any:
+        """The last FP64 solve's Schur right-hand sides b, (np, ncols) device tensor."""
+        import torch
+        out = torch.empty((ncols, self.grid.np), dtype=torch.float64, device=f"cuda:{self.device}")
+        code = self._lib.mac_stokes_last_schur_rhs(self._handle, out.data_ptr(), self.grid.np, ncols)
+        _native.raise_for(code, self._lib.mac_stokes_last_error(self._handle).decode())
+        return out.t()
 
     # ------------------------------------------------------------ measurement
     def set_profiling(self, on: bool) -> None:
