@@ -105,8 +105,9 @@ void mac_stokes_set_profiling(void* handle, int32_t on);
 int32_t mac_stokes_get_profile(void* handle, mac_kernel_profile* out);
 
 /* ---- learned warm start (no counterpart in the reference; CG stays the judge) ----
- * The network maps [pool4(b), pool4(p_prev)] (normalised) to pool4(p): b is the
- * Schur right-hand side, p_prev the previous solve's pressure of the same column.
+ * The network maps [pool4(b), pool4(p_prev)] (normalised) to pool4(p - p_prev): b is the
+ * Schur right-hand side, p_prev the previous solve's pressure of the same column; the
+ * guess is p_prev + the bilinearly upsampled, rescaled output.
  * Served grids: nx, ny multiples of 32. */
 void* mac_cnn_create(int32_t width, int32_t device);
 const char* mac_cnn_create_error(void);
@@ -136,7 +137,8 @@ int32_t mac_stokes_set_warm_start(void* handle, int32_t mode, void* cnn, double 
 int32_t mac_stokes_reset_history(void* handle);
 /* Schur right-hand sides b of the last FP64 solve, ncols columns of np (device). */
 int32_t mac_stokes_last_schur_rhs(void* handle, double* out, int64_t ld, int32_t ncols);
-/* Network inputs / targets / scales exactly as the solver prepares them (device). */
+/* Network inputs / targets / scales exactly as the solver prepares them (device);
+ * `target` is the quantity to learn, p - p_prev. */
 int32_t mac_warm_prepare(int32_t nx, int32_t ny, const double* b, int64_t ldb, const double* pprev, int64_t ldp,
                          const double* target, int64_t ldt, int32_t ncols, double alpha, float* x, float* t,
                          double* scale);

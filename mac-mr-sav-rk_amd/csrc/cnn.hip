@@ -238,7 +238,9 @@ __global__ void k_to_bf16(const float* __restrict__ w, __bf16* __restrict__ wb, 
 // ------------------------------------------------------ warm-start resampling
 // Network inputs for column c (grid nx x ny, network grid w = nx/4, h = ny/4):
 //   x[c][0] = pool4(b_c) / s_c,  x[c][1] = pool4(pprev_c) / (alpha s_c),  s_c = rms(pool4(b_c)),
-// target t[c] = pool4(p_c) / (alpha s_c); the guess is alpha s_c * bilinear_up4(y_c).
+// target t[c] = pool4(p_c - pprev_c) / (alpha s_c) (the caller passes p - pprev): the network
+// predicts the change from the previous pressure, and the guess is
+// pprev_c + alpha s_c * bilinear_up4(y_c) (pprev_c = 0 when there is none).
 __global__ void k_pool4(const double* __restrict__ src, int64_t stride, int nx, int ny, float* __restrict__ dst,
                         int64_t dst_stride) {
   const int w = nx / 4, h = ny / 4;
@@ -297,7 +299,8 @@ __global__ void k_normalize(float* __restrict__ x, float* __restrict__ t, int64_
 
 // guess[c] (nx x ny) = alpha s_c * bilinear interpolation of y[c] (nx/4 x ny/4), cell centres.
 __global__ void k_up4(const float* __restrict__ y, int nx, int ny, const double* __restrict__ s, double alpha,
-                      double* __restrict__ out, int64_t out_stride) {
+                      const double* __restrict__ base, int64_t base_stride, double* __restrict__ out,
+                      int64_t out_stride) {
   const int w = nx / 4, h = ny / 4;
   const int c = blockIdx.y;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,7 +317,8 @@ __global__ void k_up4(const float* __restrict__ y, int nx, int ny, const double*
   const float* yc = y + (int64_t)c * w * h;
   const double v = (1 - fy) * ((1 - fx) * yc[j0 * w + k0] + fx * yc[j0 * w + k1]) +
                    fy * ((1 - fx) * yc[j1 * w + k0] + fx * yc[j1 * w + k1]);
-  out[(int64_t)c * out_stride + i] = alpha * s[c] * v;
+  const double b0 = base ? base[(int64_t)c * base_stride + i] : 0.0;
+  out[(int64_t)c * out_stride + i] = b0 + alpha * s[c] * v;
 }
 
 inline int nblocks(int64_t n) { return (int)((n + 255) / 256); }
@@ -580,7 +584,7 @@ void warm_guess(void* cnn, const double* b, int64_t b_stride, const double* ppre
   net->forward(x, y, ncols, ny / 4, nx / 4);
   if (hipStreamSynchronize(net->stream()) != hipSuccess) throw std::runtime_error("warm-start forward failed");
   const dim3 grid((unsigned)(((int64_t)nx * ny + 255) / 256), (unsigned)ncols);
-  k_up4<<<grid, 256, 0, st>>>(y, nx, ny, s, alpha, guess, g_stride);
+  k_up4<<<grid, 256, 0, st>>>(y, nx, ny, s, alpha, pprev, p_stride, guess, g_stride);
 }
 
 }  // namespace macs

@@ -3,8 +3,9 @@
 `WarmStartNet` wraps the device network of libmacstokes.so (forward, backward and Adam
 written as gfx950 kernels).  It maps the Schur right-hand side b of a stage column and
 that column's pressure at the previous stage, both average-pooled by 4 and normalised
-by rms(pool4 b), to pool4(p) / (alpha rms(pool4 b)); the solver upsamples the output
-bilinearly and starts the CG from it.  The CG still runs to the same tolerance and is
+by rms(pool4 b), to the change pool4(p - p_prev) / (alpha rms(pool4 b)); the solver
+adds the bilinearly upsampled output to p_prev, scales the sum by the Galerkin step
+(g, b)/(g, S g) and starts the CG from it.  The CG still runs to the same tolerance and is
 the judge: a poor guess costs iterations, never accuracy.
 
 Training pairs come from the plain FP64 solver (`collect_pairs`): for every stage
@@ -159,11 +160,11 @@ def collect_pairs(model, scheme, initial, steps: int, dt: float) -> Pairs:
 
 
 def fit_alpha(grid, pairs: Pairs) -> float:
-    """alpha = median over columns of rms(pool4 p) / rms(pool4 b)."""
+    """alpha = median over columns of rms(pool4 (p - p_prev)) / rms(pool4 b)."""
     ratios = []
-    for b, p in zip(pairs.b, pairs.p):
+    for b, pp, p in zip(pairs.b, pairs.pprev, pairs.p):
         xb, _, sb = prepare(grid, b, None, None, 1.0)
-        xp, _, sp = prepare(grid, p, None, None, 1.0)
+        xp, _, sp = prepare(grid, p - pp, None, None, 1.0)
         ratios.extend((sp / sb).cpu().numpy().tolist())
     return float(np.median(ratios))
 
@@ -175,7 +176,7 @@ def train(net: WarmStartNet, grid, pairs: Pairs, epochs: int = 40, lr: float = 1
     import torch
     xs, ts = [], []
     for b, pp, p in zip(pairs.b, pairs.pprev, pairs.p):
-        x, t, _ = prepare(grid, b, pp, p, net.alpha)
+        x, t, _ = prepare(grid, b, pp, p - pp, net.alpha)
         xs.append(x)
         ts.append(t)
     X, T = torch.cat(xs), torch.cat(ts)

@@ -156,11 +156,15 @@ def test_1024_stage_solve_properties_on_device_tensors():
     both = s.solve(_dev(rhs[:, :1] + rhs[:, 1:]), mass=1.0, viscosity=5e-8)
     summed = (res.velocity[:, 0] + res.velocity[:, 1]).cpu().numpy()
     assert _rel(both.velocity[:, 0].cpu().numpy(), summed) < 1e-9
-    # Gradient load: zero velocity, pressure returned.
-    p = np.random.default_rng(4).normal(size=g.np)
+    # Gradient load: zero velocity, pressure returned.  The exact velocity is 0, so the
+    # computed one is pure rounding: ~eps*sqrt(n)*|rhs| from the transforms, which D
+    # multiplies by ~2/h.  With |rhs| ~ 10 here that floor is ~1e-9 in max|D u|, so this
+    # sub-case gates divergence at 1e-8 and checks velocity and pressure directly.
+    yy, xx = np.meshgrid((np.arange(g.ny) + 0.5) / g.ny, (np.arange(g.nx) + 0.5) / g.nx, indexing="ij")
+    p = (np.cos(np.pi * xx) * np.cos(2 * np.pi * yy) + 0.3 * np.cos(3 * np.pi * xx) * np.cos(np.pi * yy)).ravel()
     p -= p.mean()
     grad = mo.packed_divergence_transpose(-p, g)
-    res = s.solve(_dev(grad), mass=1.0, viscosity=5e-8)
+    res = _backend(g, tolerance=1e-8, krylov_tolerance=1e-14).solve(_dev(grad), mass=1.0, viscosity=5e-8)
     assert float(res.velocity.abs().max()) < 1e-9
     assert _rel(res.pressure.cpu().numpy().ravel(), p) < 1e-10
     del MACGrid
