@@ -442,11 +442,17 @@ int g_bf16_group = 4;
 
 template <typename TA, typename TC, bool SPLIT>
 void launch_bf16(GemmBatch& b, hipStream_t s) {
-  if (b.group <= 0) b.group = g_bf16_group;
-  int tile = g_bf16_tile;
-  // Measured at 1024^2, four blocks per launch: FP64-staged A prefers 128x128 (fewer
-  // re-reads of the wide operand), FP32/bf16-staged A prefers 128x64 (two tiles per CU).
-  if (tile == 0) tile = std::is_same<TA, double>::value ? 128 : 12864;
+  int tile = g_bf16_tile, group = g_bf16_group;
+  // Measured at 1024^2 (two or four 1024^3 products per launch): FP64-staged A prefers
+  // 128x128 once that gives a tile per CU (fewer re-reads of the wide operand),
+  // FP32/bf16-staged A prefers 128x64 at 1.5 tiles per CU or more; smaller launches run
+  // 64x64 tiles in groups of 8 tile-rows.
+  if (tile == 0) {
+    if (std::is_same<TA, double>::value) tile = tile_batch(b, 128, 128) >= 256 ? 128 : 64;
+    else tile = tile_batch(b, 128, 64) >= 384 ? 12864 : 64;
+    group = tile == 64 ? 8 : 4;
+  }
+  if (b.group <= 0) b.group = group;
   if (tile == 128) launch_bf16_tile<TA, TC, SPLIT, 128, 128>(b, s);
   else if (tile == 12864) launch_bf16_tile<TA, TC, SPLIT, 128, 64>(b, s);
   else launch_bf16_tile<TA, TC, SPLIT, 64, 64>(b, s);

@@ -76,26 +76,34 @@ struct CgState {
 constexpr int kThreads = 256;
 // Blocks per column used by every reduction (fixed, so summation order is fixed).
 int reduce_blocks(int64_t n);
+// Partial-result blocks of defect_max per quantity (three quantities follow each other).
+int defect_blocks(int64_t size, int ncols);
 
 enum Reduce : int { RED_NONE = 0, RED_SUM = 1, RED_MAXABS = 2 };
+
+// Finalize target of a reduction: when `out` is set, the launcher also reduces the
+// partials it produced into out[c] (finalize(), fixed order).
+struct Fin {
+  double* out = nullptr;
+};
 
 // q[c] = add[c] + sign * D vel[c] (add optional); optional per-block partial sums /
 // max|.| of q.
 void divergence(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
                 int64_t out_stride, double sign, int ncols, int reduce, double* partials,
-                const CgState* st, hipStream_t s, const double* add = nullptr);
+                const CgState* st, hipStream_t s, const double* add = nullptr, Fin fin = {});
 // out[c] = add[c] + D^T (p[c] - mean[c]) (add / mean optional), out in layout L.
 void divergence_transpose(const double* p, int64_t p_stride, const double* mean_sums, const double* add,
                           int64_t add_stride, int out_type, void* out, const VelLayout& L,
                           const GridDims& g, int ncols, const CgState* st, hipStream_t s);
 // partials of sum(x) (mode RED_SUM, y == null), sum(x*y) (RED_SUM, y != null), max|x|.
 void reduce(const double* x, const double* y, int64_t n, int64_t stride, int ncols, int mode,
-            double* partials, const CgState* st, hipStream_t s);
+            double* partials, const CgState* st, hipStream_t s, Fin fin = {});
 // out[c] = sum / max over the partials of column c (fixed order).
 void finalize(const double* partials, int nblk, int ncols, int mode, double* out, hipStream_t s);
 // x[c] -= sums[c]/n; optional partial sums of y*x afterwards.
 void sub_mean_dot(double* x, int64_t n, int64_t stride, int ncols, const double* sums, const double* y,
-                  double* partials, const CgState* st, hipStream_t s);
+                  double* partials, const CgState* st, hipStream_t s, Fin fin = {});
 void fill(double* x, int64_t n, double v, hipStream_t s);
 // x = theta g, r = b - theta q with theta = gb/gq per column (x holds g, q holds S g).
 void theta_start(double* x, double* r, const double* b, const double* q, const double* gb, const double* gq,
@@ -114,13 +122,22 @@ void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s);
 void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,
                  hipStream_t s);
 void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_t n, int64_t stride,
-                  int ncols, const CgState* st, double* partials, hipStream_t s);
+                  int ncols, const CgState* st, double* partials, hipStream_t s, Fin fin = {});
 
 // Final-solve checks over all columns: partial max|defect|, max|f|, max|vel| where
 // defect = mass*vel + visc*K vel - D^T p - f (vel, f packed; p mean-zero).
+// With fin.out set, fin.out[0..2] receive the three maxima directly.
 void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols,
-                double mass, double visc, double* partials3, hipStream_t s, double* residual_out = nullptr);
+                double mass, double visc, double* partials3, hipStream_t s, double* residual_out = nullptr,
+                Fin fin = {});
 // x[c] += y[c] (FP64, contiguous columns).
+// du[c] += a_c w[c] (FP32) on the columns whose CG runs: a_c = alpha of the CG state
+// (gb == gq == nullptr), gb[c]/gq[c] (0 when gq[c] <= 0; the Galerkin start), or 1 (gb
+// == nullptr, gq != nullptr; a plain start).
+void accum_velocity(float* du, const float* w, int64_t n, int64_t stride, int ncols, const CgState* st,
+                    const double* gb, const double* gq, hipStream_t s);
+// x += y (FP64 += FP32), n elements.
+void add_f32(double* x, const float* y, int64_t n, hipStream_t s);
 void add_cols(double* x, const double* y, int64_t n, int ncols, hipStream_t s);
 
 // Learned warm start (cnn.hip): grid sizes it serves, input preparation (pool by 4,

@@ -19,6 +19,22 @@ __device__ inline void st_d(__bf16* p, int64_t i, double v) { p[i] = (__bf16)(fl
 // |x| with NaN mapped to +inf, so max-norm checks catch non-finite values.
 __device__ inline double absn(double x) { return x != x ? (double)INFINITY : fabs(x); }
 
+// n / d for 0 <= n < 2^31 and 1 <= d < 2^15, from one FP64 product with the reciprocal
+// (error far below 1, so the truncated quotient is off by at most one) and a correction
+// step: a few VALU operations instead of an integer division sequence.
+struct SmallDiv {
+  int d;
+  double inv;
+  __device__ explicit SmallDiv(int d_) : d(d_), inv(1.0 / (double)d_) {}
+  __device__ inline int quot(int n) const {
+    int q = (int)((double)n * inv);
+    const int r = n - q * d;
+    if (r < 0) --q;
+    else if (r >= d) ++q;
+    return q;
+  }
+};
+
 // 256-thread block reduction in a fixed order; result valid in thread 0.
 template <int MODE>
 __device__ inline double block_reduce(double v, double* sh) {
@@ -48,11 +64,11 @@ __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, V
   const TIN* v = u + L.voff;
   double* o = out + (int64_t)c * out_stride;
   const int nx = g.nx, ny = g.ny;
-  const int64_t np = (int64_t)nx * ny;
+  const int np = nx * ny;
+  const SmallDiv dnx(nx);
   double accum = 0.0;
-  for (int64_t cell = (int64_t)blockIdx.x * kThreads + threadIdx.x; cell < np;
-       cell += (int64_t)gridDim.x * kThreads) {
-    const int j = (int)(cell / nx), i = (int)(cell - (int64_t)j * nx);
+  for (int cell = blockIdx.x * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
+    const int j = dnx.quot(cell), i = cell - j * nx;
     double value = 0.0;
     if (i < nx - 1) value += ld_d(u, (int64_t)j * L.ldu + i) / g.hx;
     if (i > 0) value -= ld_d(u, (int64_t)j * L.ldu + i - 1) / g.hx;
@@ -87,13 +103,13 @@ __global__ __launch_bounds__(kThreads) void k_dt(const double* __restrict__ p, i
   double value;
   int64_t dst;
   if (f < nu) {
-    const int j = (int)(f / (nx - 1)), i = (int)(f - (int64_t)j * (nx - 1));
+    const int j = SmallDiv(nx - 1).quot((int)f), i = (int)f - j * (nx - 1);
     const int64_t left = (int64_t)j * nx + i;
     value = mean_sums ? ((pc[left] - mean) - (pc[left + 1] - mean)) / g.hx : (pc[left] - pc[left + 1]) / g.hx;
     dst = (int64_t)j * L.ldu + i;
   } else {
-    const int64_t e = f - nu;
-    const int j = (int)(e / nx), i = (int)(e - (int64_t)j * nx);
+    const int e = (int)(f - nu);
+    const int j = SmallDiv(nx).quot(e), i = e - j * nx;
     const int64_t lower = (int64_t)j * nx + i;
     value = mean_sums ? ((pc[lower] - mean) - (pc[lower + nx] - mean)) / g.hy : (pc[lower] - pc[lower + nx]) / g.hy;
     dst = L.voff + (int64_t)j * L.ldv + i;
@@ -278,16 +294,18 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
   const double hx2 = g.hx * g.hx, hy2 = g.hy * g.hy;
   const double ox = -1.0 / hx2, oy = -1.0 / hy2;
   double md = 0.0, mf = 0.0, mv = 0.0;
-  const int64_t total = size * ncols;
-  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total; t += (int64_t)gridDim.x * kThreads) {
-    const int c = (int)(t / size);
-    const int64_t fidx = t - (int64_t)c * size;
-    const double* vc = vel + (int64_t)c * size;
-    const double* pc = p + (int64_t)c * np;
+  // Faces of one column are < 2^31 (the grid caps nx, ny at 16384): 32-bit index math,
+  // columns walked by the grid's y dimension.
+  const int c = blockIdx.y;
+  const unsigned usize = (unsigned)size, unu = (unsigned)nu;
+  const double* vc = vel + (int64_t)c * size;
+  const double* pc = p + (int64_t)c * np;
+  const SmallDiv dw(nx - 1), dnx(nx);
+  for (unsigned fidx = blockIdx.x * kThreads + threadIdx.x; fidx < usize; fidx += gridDim.x * kThreads) {
     double kv, lift;
-    if (fidx < nu) {
-      const int w = nx - 1;
-      const int j = (int)(fidx / w), i = (int)(fidx - (int64_t)j * w);
+    if (fidx < unu) {
+      const unsigned w = nx - 1;
+      const int j = dw.quot((int)fidx), i = (int)(fidx - (unsigned)j * w);
       const double* u = vc;
       const double diag = 2.0 / hx2 + ((j == 0 || j == ny - 1) ? 3.0 : 2.0) / hy2;
       kv = 0.0;
@@ -299,8 +317,8 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
       const int64_t left = (int64_t)j * nx + i;
       lift = (pc[left] - pc[left + 1]) / g.hx;
     } else {
-      const int64_t e = fidx - nu;
-      const int j = (int)(e / nx), i = (int)(e - (int64_t)j * nx);
+      const unsigned e = fidx - unu;
+      const int j = dnx.quot((int)e), i = (int)(e - (unsigned)j * nx);
       const double* v = vc + nu;
       const double diag = ((i == 0 || i == nx - 1) ? 3.0 : 2.0) / hx2 + 2.0 / hy2;
       kv = 0.0;
@@ -325,16 +343,20 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
   mf = block_reduce<RED_MAXABS>(mf, sh);
   __syncthreads();
   mv = block_reduce<RED_MAXABS>(mv, sh);
+  const int nb = gridDim.x * gridDim.y;
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = md;
-    partials[gridDim.x + blockIdx.x] = mf;
-    partials[2 * gridDim.x + blockIdx.x] = mv;
+    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    partials[b] = md;
+    partials[nb + b] = mf;
+    partials[2 * nb + b] = mv;
   }
 }
 
 inline dim3 col_grid(int64_t n, int ncols) { return dim3((unsigned)reduce_blocks(n), (unsigned)ncols); }
 
 }  // namespace
+
+int defect_blocks(int64_t size, int ncols) { return reduce_blocks(size) * ncols; }
 
 int reduce_blocks(int64_t n) {
   int64_t b = (n + (int64_t)kThreads * 8 - 1) / ((int64_t)kThreads * 8);
@@ -345,7 +367,7 @@ int reduce_blocks(int64_t n) {
 
 void divergence(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
                 int64_t out_stride, double sign, int ncols, int reduce, double* partials, const CgState* st,
-                hipStream_t s, const double* add) {
+                hipStream_t s, const double* add, Fin fin) {
   const dim3 grid = col_grid((int64_t)g.nx * g.ny, ncols);
 #define MACS_DIV(T)                                                                                          \
   switch (reduce) {                                                                                          \
@@ -360,6 +382,7 @@ void divergence(int in_type, const void* vel, const VelLayout& L, const GridDims
   else if (in_type == DT_F32) { MACS_DIV(float) }
   else { MACS_DIV(__bf16) }
 #undef MACS_DIV
+  if (fin.out != nullptr && reduce != RED_NONE) finalize(partials, grid.x, ncols, reduce, fin.out, s);
 }
 
 void divergence_transpose(const double* p, int64_t p_stride, const double* mean_sums, const double* add,
@@ -376,11 +399,12 @@ void divergence_transpose(const double* p, int64_t p_stride, const double* mean_
 }
 
 void reduce(const double* x, const double* y, int64_t n, int64_t stride, int ncols, int mode, double* partials,
-            const CgState* st, hipStream_t s) {
+            const CgState* st, hipStream_t s, Fin fin) {
   const dim3 grid = col_grid(n, ncols);
   if (mode == RED_MAXABS) k_reduce<RED_MAXABS, false><<<grid, kThreads, 0, s>>>(x, y, n, stride, partials, st);
   else if (y != nullptr) k_reduce<RED_SUM, true><<<grid, kThreads, 0, s>>>(x, y, n, stride, partials, st);
   else k_reduce<RED_SUM, false><<<grid, kThreads, 0, s>>>(x, y, n, stride, partials, st);
+  if (fin.out != nullptr) finalize(partials, grid.x, ncols, mode, fin.out, s);
 }
 
 void finalize(const double* partials, int nblk, int ncols, int mode, double* out, hipStream_t s) {
@@ -389,10 +413,14 @@ void finalize(const double* partials, int nblk, int ncols, int mode, double* out
 }
 
 void sub_mean_dot(double* x, int64_t n, int64_t stride, int ncols, const double* sums, const double* y,
-                  double* partials, const CgState* st, hipStream_t s) {
+                  double* partials, const CgState* st, hipStream_t s, Fin fin) {
   const dim3 grid = col_grid(n, ncols);
-  if (y != nullptr) k_sub_mean_dot<true><<<grid, kThreads, 0, s>>>(x, n, stride, sums, y, partials, st);
-  else k_sub_mean_dot<false><<<grid, kThreads, 0, s>>>(x, n, stride, sums, y, partials, st);
+  if (y != nullptr) {
+    k_sub_mean_dot<true><<<grid, kThreads, 0, s>>>(x, n, stride, sums, y, partials, st);
+    if (fin.out != nullptr) finalize(partials, grid.x, ncols, RED_SUM, fin.out, s);
+  } else {
+    k_sub_mean_dot<false><<<grid, kThreads, 0, s>>>(x, n, stride, sums, y, partials, st);
+  }
 }
 
 void sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t stride, int ncols,
@@ -428,20 +456,55 @@ void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncol
   k_update_p<<<col_grid(n, ncols), kThreads, 0, s>>>(p, z, n, stride, st);
 }
 void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_t n, int64_t stride, int ncols,
-                  const CgState* st, double* partials, hipStream_t s) {
-  k_update_xr<<<col_grid(n, ncols), kThreads, 0, s>>>(x, r, p, q, n, stride, st, partials);
+                  const CgState* st, double* partials, hipStream_t s, Fin fin) {
+  const dim3 grid = col_grid(n, ncols);
+  k_update_xr<<<grid, kThreads, 0, s>>>(x, r, p, q, n, stride, st, partials);
+  if (fin.out != nullptr) finalize(partials, grid.x, ncols, RED_SUM, fin.out, s);
 }
 
 void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols, double mass,
-                double visc, double* partials3, hipStream_t s, double* residual_out) {
+                double visc, double* partials3, hipStream_t s, double* residual_out, Fin fin) {
   const int64_t size = (int64_t)(g.nx - 1) * g.ny + (int64_t)g.nx * (g.ny - 1);
-  k_defect<<<reduce_blocks(size * ncols), kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3,
+  k_defect<<<col_grid(size, ncols), kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3,
                                                              residual_out);
+  if (fin.out != nullptr) {
+    const int nb = defect_blocks(size, ncols);
+    for (int q = 0; q < 3; ++q) finalize(partials3 + q * nb, nb, 1, RED_MAXABS, fin.out + q, s);
+  }
 }
 
 __global__ void k_add_cols(double* __restrict__ x, const double* __restrict__ y, int64_t total) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kThreads)
     x[i] = x[i] + y[i];
+}
+
+__global__ __launch_bounds__(kThreads) void k_accum_velocity(float* __restrict__ du, const float* __restrict__ w,
+                                                             int64_t n, int64_t stride, const CgState* st,
+                                                             const double* gb, const double* gq) {
+  const int c = blockIdx.y;
+  if (!st[c].active) return;
+  double a;
+  if (gb != nullptr) a = gq[c] > 0.0 ? gb[c] / gq[c] : 0.0;
+  else if (gq != nullptr) a = 1.0;
+  else a = st[c].alpha;
+  const float af = (float)a;
+  const int64_t off = (int64_t)c * stride;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    du[off + i] = fmaf(af, w[off + i], du[off + i]);
+}
+
+void accum_velocity(float* du, const float* w, int64_t n, int64_t stride, int ncols, const CgState* st,
+                    const double* gb, const double* gq, hipStream_t s) {
+  k_accum_velocity<<<col_grid(n, ncols), kThreads, 0, s>>>(du, w, n, stride, st, gb, gq);
+}
+
+__global__ void k_add_f32(double* __restrict__ x, const float* __restrict__ y, int64_t total) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kThreads)
+    x[i] = x[i] + (double)y[i];
+}
+
+void add_f32(double* x, const float* y, int64_t n, hipStream_t s) {
+  k_add_f32<<<reduce_blocks(n), kThreads, 0, s>>>(x, y, n);
 }
 
 void add_cols(double* x, const double* y, int64_t n, int ncols, hipStream_t s) {

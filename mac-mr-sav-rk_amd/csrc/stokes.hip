@@ -463,13 +463,10 @@ class Solver {
     out.iterations.assign(ncols, 0);
     // W = L^-1 f ; b = -D W ; b -= mean(b)
     velocity_inverse(f, f_stride, w_, size_, all, mass, visc);
-    const int nbp = reduce_blocks(np_);
-    divergence(DT_F64, w_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_);
-    finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
-    sub_mean_dot(b_, np_, np_, ncols, sums_, b_, partials_, nullptr, stream_);
-    finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);  // ||b||^2
-    reduce(b_, nullptr, np_, np_, ncols, RED_MAXABS, partials_, nullptr, stream_);
-    finalize(partials_, nbp, ncols, RED_MAXABS, sums3_, stream_);
+    divergence(DT_F64, w_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, nullptr,
+               fin(sums_));
+    sub_mean_dot(b_, np_, np_, ncols, sums_, b_, partials_, nullptr, stream_, fin(sums2_));  // ||b||^2
+    reduce(b_, nullptr, np_, np_, ncols, RED_MAXABS, partials_, nullptr, stream_, fin(sums3_));
     cg_init(cg_, sums2_, sums3_, opts_.krylov_tolerance, ncols, stream_);
     // x = 0, r = b  (or x = guess, r = b - S guess on columns whose CG runs)
     HIP_CHECK(hipMemsetAsync(x_, 0, sizeof(double) * np_ * ncols, stream_));
@@ -484,8 +481,7 @@ class Solver {
     }
     pcg(ncols, mass, visc, out.iterations, true);
     // p = x - mean(x)
-    reduce(x_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_);
-    finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+    reduce(x_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_, fin(sums_));
     sub_mean_dot(x_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
     copy_cols(x_, np_, pres, pres_stride, np_, ncols, stream_);
     remember_pressure(x_, ncols);
@@ -501,7 +497,8 @@ class Solver {
   //   repeat: r_u = f - (mass u + visc K u) + D^T p, r_c = -D u   (FP64)
   //           stop when max|r_u|/(1+max|f|) <= refine_tol and max|D u| <= refine_tol
   //           W = L~^-1 r_u; b = r_c - D W; dp = PCG(S~, b, inner_tol); p += dp
-  //           u += L~^-1 (r_u + D^T dp)
+  //           u += L~^-1 (r_u + D^T dp) = W + sum_k alpha_k w_k, where w_k = L~^-1 D^T p_k
+  //           is the intermediate of the k-th Schur product (no extra transforms)
   // (L~, S~: the velocity inverse and Schur operator with bf16 transforms.)
   Outcome solve_mixed(const double* f, int64_t f_stride, int ncols, double mass, double visc, double* vel,
                       int64_t vel_stride, double* pres, int64_t pres_stride, const double* guess,
@@ -521,18 +518,13 @@ class Solver {
     HIP_CHECK(hipMemsetAsync(uacc_, 0, sizeof(double) * size_ * ncols, stream_));
     if (guess) copy_cols(guess, guess_stride, pacc_, np_, np_, ncols, stream_);
     else HIP_CHECK(hipMemsetAsync(pacc_, 0, sizeof(double) * np_ * ncols, stream_));
-    const int nbp = reduce_blocks(np_);
-    const int nb = reduce_blocks(size_ * ncols);
+    const int nb = defect_blocks(size_, ncols);
     std::vector<int> inner(ncols);
     for (int sweep = 0;; ++sweep) {
       // FP64 residuals of the full system at the current iterate.
-      defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_);
-      finalize(partials_, nb, 1, RED_MAXABS, sums_, stream_);
-      finalize(partials_ + nb, nb, 1, RED_MAXABS, sums_ + 1, stream_);
-      finalize(partials_ + 2 * nb, nb, 1, RED_MAXABS, sums_ + 2, stream_);
+      defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_, fin(sums_));
       divergence(DT_F64, uacc_, packed_, g_, rc_, np_, -1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr,
-                 stream_);
-      finalize(partials_ + 3 * nb, nbp, ncols, RED_MAXABS, sums2_, stream_);
+                 stream_, nullptr, fin(sums2_));
       double h[3];
       std::vector<double> dv(ncols);
       HIP_CHECK(hipMemcpyAsync(h, sums_, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
@@ -548,37 +540,32 @@ class Solver {
         break;
       // Inner solve for the correction (dp, du).
       mixed_ = true;
-      velocity_inverse_bf16(ru_, DT_F64, packed_, wf_, DT_F32, size_, all, mass, visc);
-      divergence(DT_F32, wf_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_);
-      finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
-      sub_mean_dot(b_, np_, np_, ncols, sums_, b_, partials_, nullptr, stream_);
-      finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);
-      reduce(b_, nullptr, np_, np_, ncols, RED_MAXABS, partials_, nullptr, stream_);
-      finalize(partials_, nbp, ncols, RED_MAXABS, sums3_, stream_);
+      velocity_inverse_bf16(ru_, DT_F64, packed_, du_, DT_F32, size_, all, mass, visc);
+      divergence(DT_F32, du_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_,
+                 fin(sums_));
+      sub_mean_dot(b_, np_, np_, ncols, sums_, b_, partials_, nullptr, stream_, fin(sums2_));
+      reduce(b_, nullptr, np_, np_, ncols, RED_MAXABS, partials_, nullptr, stream_, fin(sums3_));
       cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_);
       HIP_CHECK(hipMemsetAsync(x_, 0, sizeof(double) * np_ * ncols, stream_));
       copy_cols(b_, np_, r_, np_, np_, ncols, stream_);
       HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
+      accumulate_ = true;
       if (sweep == 0 && guess == nullptr && learned_guess(b_, ncols)) {
         // First sweep: the network's pressure starts the inner Schur CG (p was zero).
         out.warm = true;
         start_from(io_g_, np_, ncols, mass, visc, true);
       }
       pcg(ncols, mass, visc, inner, false);
+      accumulate_ = false;
       for (int c = 0; c < ncols; ++c) out.iterations[c] += inner[c];
       add_cols(pacc_, x_, np_, ncols, stream_);
-      {
-        const int dt_type = split_velocity() ? DT_F32 : DT_BF16;
-        void* dt_buf = split_velocity() ? static_cast<void*>(tfb_) : static_cast<void*>(tb_);
-        divergence_transpose(x_, np_, nullptr, ru_, size_, dt_type, dt_buf, lay_b_, g_, ncols, nullptr, stream_);
-        velocity_inverse_bf16(dt_buf, dt_type, lay_b_, uacc_, DT_F64, size_, all, mass, visc);
-      }
+      // du = L~^-1 (r_u + D^T dp) = W + sum_k alpha_k L~^-1 D^T p_k, already summed in du_.
+      add_f32(uacc_, du_, size_ * ncols, stream_);
       mixed_ = false;
       out.sweeps = sweep + 1;
     }
     // p mean-zero; hand out u and p.
-    reduce(pacc_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_);
-    finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
+    reduce(pacc_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_, fin(sums_));
     sub_mean_dot(pacc_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
     copy_cols(pacc_, np_, pres, pres_stride, np_, ncols, stream_);
     copy_cols(uacc_, size_, vel, vel_stride, size_, ncols, stream_);
@@ -613,30 +600,30 @@ class Solver {
 
   // q = S p = D L^-1 D^T p, mean removed; optional partial (y, q) dots after the projection.
   void schur(const double* pin, double* qout, const std::vector<int>& cols, double mass, double visc,
-             const double* dot_with) {
+             const double* dot_with, double* dot_out = nullptr) {
     const int ncols = ncols_cur_;
     const CgState* st = use_state_ ? cg_ : nullptr;
-    const int nbp = reduce_blocks(np_);
     if (mixed_) {
       const int dt_type = split_velocity() ? DT_F32 : DT_BF16;
       void* dt_buf = split_velocity() ? static_cast<void*>(tfb_) : static_cast<void*>(tb_);
       divergence_transpose(pin, np_, nullptr, nullptr, 0, dt_type, dt_buf, lay_b_, g_, ncols, st, stream_);
       velocity_inverse_bf16(dt_buf, dt_type, lay_b_, wf_, DT_F32, size_, cols, mass, visc);
-      divergence(DT_F32, wf_, packed_, g_, qout, np_, 1.0, ncols, RED_SUM, partials_, st, stream_);
+      divergence(DT_F32, wf_, packed_, g_, qout, np_, 1.0, ncols, RED_SUM, partials_, st, stream_, nullptr,
+                 fin(sums_));
     } else {
       divergence_transpose(pin, np_, nullptr, nullptr, 0, DT_F64, w2_, packed_, g_, ncols, st, stream_);
       velocity_inverse(w2_, size_, w_, size_, cols, mass, visc);
-      divergence(DT_F64, w_, packed_, g_, qout, np_, 1.0, ncols, RED_SUM, partials_, st, stream_);
+      divergence(DT_F64, w_, packed_, g_, qout, np_, 1.0, ncols, RED_SUM, partials_, st, stream_, nullptr,
+                 fin(sums_));
     }
-    finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
-    sub_mean_dot(qout, np_, np_, ncols, sums_, dot_with, partials_, st, stream_);
+    sub_mean_dot(qout, np_, np_, ncols, sums_, dot_with, partials_, st, stream_,
+                 dot_with != nullptr ? fin(dot_out) : Fin{});
   }
 
   // x = guess, r = b - S x, rr = (r, r) on the columns whose CG runs (a column with a
   // zero right-hand side keeps x = 0, as the reference skips its CG).
   void start_from(const double* guess, int64_t guess_stride, int ncols, double mass, double visc,
                   bool scale_guess = false) {
-    const int nbp = reduce_blocks(np_);
     std::vector<CgState> st(ncols);
     HIP_CHECK(hipMemcpyAsync(st.data(), cg_, sizeof(CgState) * ncols, hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -653,22 +640,20 @@ class Solver {
     use_state_ = false;
     if (scale_guess) {
       // Galerkin step along the guess: never a worse start, in the S-norm, than zero.
-      reduce(x_, b_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
-      finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
-      reduce(x_, q_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
-      finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);
+      reduce(x_, b_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(sums_));
+      reduce(x_, q_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(sums2_));
       theta_start(x_, r_, b_, q_, sums_, sums2_, np_, np_, ncols, cg_, stream_);
+      if (accumulate_) accum_velocity(du_, wf_, size_, size_, ncols, cg_, sums_, sums2_, stream_);
     } else {
       sub_cols(b_, q_, r_, np_, np_, ncols, cg_, stream_);
+      if (accumulate_) accum_velocity(du_, wf_, size_, size_, ncols, cg_, nullptr, sums_, stream_);
     }
-    reduce(r_, r_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
-    finalize(partials_, nbp, ncols, RED_SUM, rr_, stream_);
+    reduce(r_, r_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(rr_));
   }
 
   // Schur PCG on all columns at once (x, r, rr, cg_ prepared by the caller).  With
   // `fail`, a column not converged after max_iterations raises MAC_NOT_CONVERGED.
   void pcg(int ncols, double mass, double visc, std::vector<int>& iters, bool fail) {
-    const int nbp = reduce_blocks(np_);
     ncols_cur_ = ncols;
     use_state_ = true;
     std::vector<int> active;
@@ -683,18 +668,15 @@ class Solver {
       // z = mass (D D^T)^+ r + visc r, mean removed; rho = (r, z)
       if (mixed_) pressure_inverse_bf16(r_, z_, active, mass, visc);
       else pressure_inverse(r_, np_, z_, np_, active, true, mass, visc);
-      reduce(z_, nullptr, np_, np_, ncols, RED_SUM, partials_, cg_, stream_);
-      finalize(partials_, nbp, ncols, RED_SUM, sums_, stream_);
-      sub_mean_dot(z_, np_, np_, ncols, sums_, r_, partials_, cg_, stream_);
-      finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);
+      reduce(z_, nullptr, np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(sums_));
+      sub_mean_dot(z_, np_, np_, ncols, sums_, r_, partials_, cg_, stream_, fin(sums2_));
       cg_rho(cg_, sums2_, ncols, stream_);
       cg_update_p(p_, z_, np_, np_, ncols, cg_, stream_);
       // q = S p; alpha = rho / (p, q)
-      schur(p_, q_, active, mass, visc, p_);
-      finalize(partials_, nbp, ncols, RED_SUM, sums2_, stream_);
+      schur(p_, q_, active, mass, visc, p_, sums2_);
       cg_alpha(cg_, sums2_, ncols, stream_);
-      cg_update_xr(x_, r_, p_, q_, np_, np_, ncols, cg_, partials_, stream_);
-      finalize(partials_, nbp, ncols, RED_SUM, rr_, stream_);
+      cg_update_xr(x_, r_, p_, q_, np_, np_, ncols, cg_, partials_, stream_, fin(rr_));
+      if (accumulate_) accum_velocity(du_, wf_, size_, size_, ncols, cg_, nullptr, nullptr, stream_);
     }
     use_state_ = false;
     // A column still active after max_iterations completed iterations failed (the
@@ -723,14 +705,10 @@ class Solver {
       copy_cols(vel, vel_stride, w2_, size_, size_, ncols, stream_);
       vc = w2_;
     }
-    const int nb = reduce_blocks(size_ * ncols);
-    defect_max(vc, fc, p, g_, ncols, mass, visc, partials_, stream_);
-    finalize(partials_, nb, 1, RED_MAXABS, sums_, stream_);
-    finalize(partials_ + nb, nb, 1, RED_MAXABS, sums_ + 1, stream_);
-    finalize(partials_ + 2 * nb, nb, 1, RED_MAXABS, sums_ + 2, stream_);
-    const int nbp = reduce_blocks(np_);
-    divergence(DT_F64, vc, packed_, g_, q_, np_, 1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr, stream_);
-    finalize(partials_ + 3 * nb, nbp, ncols, RED_MAXABS, sums2_, stream_);
+    const int nb = defect_blocks(size_, ncols);
+    defect_max(vc, fc, p, g_, ncols, mass, visc, partials_, stream_, nullptr, fin(sums_));
+    divergence(DT_F64, vc, packed_, g_, q_, np_, 1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr, stream_,
+               nullptr, fin(sums2_));
     double h[3];
     std::vector<double> dv(ncols);
     HIP_CHECK(hipMemcpyAsync(h, sums_, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
@@ -855,13 +833,14 @@ class Solver {
     pacc_ = (double*)take(pb);
     rc_ = (double*)take(pb);
     wf_ = (float*)take(sizeof(float) * size_ * ncols);
+    du_ = (float*)take(sizeof(float) * size_ * ncols);
     tb_ = (__bf16*)take(sizeof(__bf16) * tstride_ * ncols);
     tb1_ = (__bf16*)take(sizeof(__bf16) * tstride_ * ncols);
     tb2_ = (__bf16*)take(sizeof(__bf16) * tstride_ * ncols);
     tf1_ = (float*)take(sizeof(float) * tstride_ * ncols);
     tf2_ = (float*)take(sizeof(float) * tstride_ * ncols);
     tfb_ = (float*)take(sizeof(float) * tstride_ * ncols);
-    const int nb = std::max(reduce_blocks(size_ * ncols), reduce_blocks(np_) * ncols);
+    const int nb = std::max(defect_blocks(size_, ncols), reduce_blocks(np_) * ncols);
     partials_ = (double*)take(sizeof(double) * (4 * (size_t)nb + 16));
     sums_ = (double*)take(sizeof(double) * (ncols + 4));
     sums2_ = (double*)take(sizeof(double) * (ncols + 4));
@@ -897,6 +876,9 @@ class Solver {
   double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;
   double *fcomp_ = nullptr, *uacc_ = nullptr, *ru_ = nullptr, *pacc_ = nullptr, *rc_ = nullptr;
   float* wf_ = nullptr;
+  static Fin fin(double* out) { return Fin{out}; }
+  float* du_ = nullptr;       // mixed sweeps: velocity correction, accumulated in the CG
+  bool accumulate_ = false;   // pcg / start_from add alpha * (L~^-1 D^T p) into du_
   __bf16 *tb_ = nullptr, *tb1_ = nullptr, *tb2_ = nullptr;
   float *tf1_ = nullptr, *tf2_ = nullptr, *tfb_ = nullptr;
   int64_t tstride_ = 0, toff_v_ = 0;
@@ -967,8 +949,8 @@ void* mac_stokes_create(int32_t nx, int32_t ny, double lx, double ly, const mac_
   mac_stokes_opts o;
   mac_stokes_default_opts(&o);
   if (opts) o = *opts;
-  if (nx < 3 || ny < 3 || !is_finite(lx) || !is_finite(ly) || lx <= 0 || ly <= 0) {
-    g_create_error = "Require nx,ny >= 3 and positive finite domain lengths";
+  if (nx < 3 || ny < 3 || nx > 16384 || ny > 16384 || !is_finite(lx) || !is_finite(ly) || lx <= 0 || ly <= 0) {
+    g_create_error = "Require 3 <= nx,ny <= 16384 and positive finite domain lengths";
     return nullptr;
   }
   if (!(o.tolerance > 0) || !(o.krylov_tolerance > 0) || !is_finite(o.tolerance) || !is_finite(o.krylov_tolerance) ||
