@@ -33,6 +33,33 @@ for _p in (str(ROOT), str(ROOT / "mac-mr-sav-rk_amd")):
 
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 2500.0}  # MI355X dense: FP64 MFMA, BF16 MFMA
 NU, DT, STAGE_DIAG = 1e-3, 2e-4, 0.25            # sdirk3 D[i,i] = 1/4
+CALIBRATION_BYTES = 512 << 20                    # one plain copy of known size, before timing
+TRAFFIC_FILE = ROOT / "profiles" / "k_gemm_traffic_1024.md"
+
+
+def calibration_copy():
+    """One device copy of CALIBRATION_BYTES (read and write each), outside the timed
+    region: hardware byte counters of the transform GEMM are scaled by what the same
+    counters read for this copy."""
+    import torch
+    src = torch.ones(CALIBRATION_BYTES // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def measured_traffic(precision: str):
+    """HBM bytes per transform-GEMM launch from the committed counter summary (a line
+    'traffic_bytes_per_launch <precision> <bytes>'), or None."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == precision:
+                return float(parts[2])
+    except OSError:
+        return None
+    return None
 
 
 def stage_pairs(grid, count: int, modes: int):
@@ -87,6 +114,7 @@ def main() -> int:
 
     grid = MACGrid(args.n, args.n)
     kappa = NU * DT * STAGE_DIAG
+    calibration_copy()
     pairs_host = stage_pairs(grid, 4, args.modes)
     pairs = [torch.as_tensor(p, device="cuda:0") for p in pairs_host]
 
@@ -135,7 +163,10 @@ def main() -> int:
     else:
         main_stats, prof = plain, plain_prof
     gemm_s = prof["gemm_ms"] * 1e-3
+    # Flops of the MFMA work the algorithm issues: 2MNK per product, times 3 for the
+    # split bf16x3 products of the mixed form (A_lo B_hi + A_hi B_lo + A_hi B_hi).
     achieved = prof["gemm_flops"] / gemm_s / 1e12
+    split = 3.0 if args.precision == "mixed" else 1.0
     peak = PEAK_TFLOPS[args.precision]
     value = main_stats["stage_solves_per_s"]
 
@@ -159,11 +190,13 @@ def main() -> int:
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": value / plain["stage_solves_per_s"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": measured_traffic(args.precision),
                      "kernel": "k_gemm_f64" if args.precision == "fp64" else "k_gemm_bf16",
                      "launches": prof["gemm_launches"],
                      "avg_launch_us": 1e6 * gemm_s / max(prof["gemm_launches"], 1),
-                     "flops_per_launch": prof["gemm_flops"] / max(prof["gemm_launches"], 1)},
+                     "flops_per_launch": prof["gemm_flops"] / max(prof["gemm_launches"], 1),
+                     "transform_flops_per_launch": prof["gemm_flops"] / split / max(prof["gemm_launches"], 1),
+                     "algorithmic_bytes_per_launch": prof["gemm_bytes"] / max(prof["gemm_launches"], 1)},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(grid, pairs_host[0], kappa, args.cpu_seconds)

@@ -94,7 +94,8 @@ int32_t mac_stokes_solve(void* handle, const double* rhs, int64_t ld_rhs, int32_
 /* ---- measurement: HIP-event timing of the transform GEMM launches ---- */
 typedef struct mac_kernel_profile {
   double gemm_ms;        /* summed device time of the transform-GEMM launches       */
-  double gemm_flops;     /* their algorithmic flops, 2*M*N*K per product            */
+  double gemm_flops;     /* their MFMA work, 2*M*N*K per issued product (three per
+                            k-step in the split bf16x3 form, one otherwise)         */
   double gemm_bytes;     /* their algorithmic bytes: A, Bt and C elements once each  */
   int64_t gemm_launches; /* number of launches timed                                */
 } mac_kernel_profile;

@@ -45,16 +45,16 @@ __device__ inline void st_from_double(float* p, int64_t i, double v) { p[i] = (f
 __device__ inline void st_from_double(__bf16* p, int64_t i, double v) { p[i] = (__bf16)(float)v; }
 
 template <int EPI, typename TC>
-__device__ inline void epilogue(const GemmBatch& b, const GemmEntry& E, int m, int n, double v) {
+__device__ inline void epilogue(const GemmEntry& E, int trans_c, int m, int n, double v) {
   if (m >= E.M || n >= E.N) return;
-  const int64_t idx = b.trans_c ? (int64_t)n * E.ldc + m : (int64_t)m * E.ldc + n;
+  const int64_t idx = trans_c ? (int64_t)n * E.ldc + m : (int64_t)m * E.ldc + n;
   if (EPI == EPI_SHIFT) {
-    v = v / (b.s0 + b.s1 * (E.ea[m] + E.eb[n]));
+    v = v / (E.s0 + E.s1 * (E.ea[m] + E.eb[n]));
   } else if (EPI == EPI_LAMBDA) {
     const double lam = E.ea[m] + E.eb[n];
-    v = lam > b.thresh ? v / lam : 0.0;
+    v = lam > E.thresh ? v / lam : 0.0;
   } else if (EPI == EPI_COMBINE) {
-    v = b.s0 * v + b.s1 * ld_as_double(static_cast<const TC*>(E.aux), idx);
+    v = E.s0 * v + E.s1 * ld_as_double(static_cast<const TC*>(E.aux), idx);
   } else if (EPI == EPI_ACCUM) {
     v = ld_as_double(static_cast<const TC*>(E.C), idx) + v;
   }
@@ -66,13 +66,11 @@ __device__ inline void epilogue(const GemmBatch& b, const GemmEntry& E, int m, i
 // K step 16, LDS double-buffered, global loads for step k+1 in flight during step k.
 constexpr int F64_BM = 64, F64_BN = 64, F64_BK = 16, F64_LDS = 64 + 4;
 
+// One 64x64 output tile (index lt within entry E) of C = epi(A . Bt^T).
 template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_f64(const GemmBatch batch) {
+__device__ inline void f64_tile(const GemmEntry& E, int lt, int trans_c) {
   __shared__ double As[2][F64_BK * F64_LDS];
   __shared__ double Bs[2][F64_BK * F64_LDS];
-  const int t = xcd_tile(blockIdx.x, batch.total_tiles);
-  const GemmEntry& E = batch.e[find_entry(batch, t)];
-  const int lt = t - E.tile_begin;
   const int m0 = (lt / E.tiles_n) * F64_BM, n0 = (lt % E.tiles_n) * F64_BN;
   const double* __restrict__ A = static_cast<const double*>(E.A);
   const double* __restrict__ B = static_cast<const double*>(E.Bt);
@@ -138,8 +136,31 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmBatch batch) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        epilogue<EPI, double>(batch, E, m0 + wm + i * 16 + lg + 4 * r, n0 + wn + j * 16 + li,
+        epilogue<EPI, double>(E, trans_c, m0 + wm + i * 16 + lg + 4 * r, n0 + wn + j * 16 + li,
                               acc[i][j][r]);
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_f64(const GemmBatch batch) {
+  const int t = xcd_tile(blockIdx.x, batch.total_tiles);
+  const GemmEntry& E = batch.e[find_entry(batch, t)];
+  f64_tile<EPI>(E, t - E.tile_begin, batch.trans_c);
+}
+
+// Grouped launch over a device table: each block finds its entry by binary search
+// over the ascending tile_begin array; entries of converged columns are skipped.
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_f64_table(const GemmTable tb) {
+  const int t = xcd_tile(blockIdx.x, tb.total_tiles);
+  int lo = 0, hi = tb.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tb.begin[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmEntry E = tb.e[lo];
+  if (tb.use_state && E.st != nullptr && !E.st->active) return;
+  f64_tile<EPI>(E, t - E.tile_begin, tb.trans_c);
 }
 
 // ------------------------------------------------------------------- BF16 GEMM
@@ -391,7 +412,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
         store4<TC>(static_cast<TC*>(E.C) + (int64_t)n * E.ldc + mb, acc[i][j]);
       } else {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) epilogue<EPI, TC>(batch, E, mb + r, n, (double)acc[i][j][r]);
+        for (int r = 0; r < 4; ++r) epilogue<EPI, TC>(E, batch.trans_c, mb + r, n, (double)acc[i][j][r]);
       }
     }
 }
@@ -467,8 +488,17 @@ void launch_bf16_c(GemmBatch& b, int c_type, hipStream_t s) {
 
 }  // namespace
 
+static void set_scalars(GemmBatch& b) {
+  for (int i = 0; i < b.count; ++i) {
+    b.e[i].s0 = b.s0;
+    b.e[i].s1 = b.s1;
+    b.e[i].thresh = b.thresh;
+  }
+}
+
 void gemm_f64(GemmBatch& b, hipStream_t s) {
   if (b.count <= 0) return;
+  set_scalars(b);
   const int total = tile_batch(b, F64_BM, F64_BN);
   switch (b.epi) {
     case EPI_STORE: k_gemm_f64<EPI_STORE><<<total, 256, 0, s>>>(b); break;
@@ -483,8 +513,32 @@ void set_bf16_tiling(int tile, int group) {
   g_bf16_group = group;
 }
 
+int tile_table_f64(GemmEntry* entries, int count, int* begins) {
+  int total = 0;
+  for (int i = 0; i < count; ++i) {
+    GemmEntry& e = entries[i];
+    e.tiles_m = (e.M + F64_BM - 1) / F64_BM;
+    e.tiles_n = (e.N + F64_BN - 1) / F64_BN;
+    e.tile_begin = total;
+    begins[i] = total;
+    total += e.tiles_m * e.tiles_n;
+  }
+  return total;
+}
+
+void gemm_f64_table(const GemmTable& t, hipStream_t s) {
+  if (t.count <= 0 || t.total_tiles <= 0) return;
+  switch (t.epi) {
+    case EPI_STORE: k_gemm_f64_table<EPI_STORE><<<t.total_tiles, 256, 0, s>>>(t); break;
+    case EPI_SHIFT: k_gemm_f64_table<EPI_SHIFT><<<t.total_tiles, 256, 0, s>>>(t); break;
+    case EPI_LAMBDA: k_gemm_f64_table<EPI_LAMBDA><<<t.total_tiles, 256, 0, s>>>(t); break;
+    default: k_gemm_f64_table<EPI_COMBINE><<<t.total_tiles, 256, 0, s>>>(t); break;
+  }
+}
+
 void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s) {
   if (b.count <= 0) return;
+  set_scalars(b);
   if (split) {
     // Split operands come from FP32/FP64 data (a bf16 input has no low part).
     if (a_type == DT_F64) launch_bf16_c<double, true>(b, c_type, s);

@@ -20,6 +20,8 @@ enum Epilogue : int {
 };
 enum DType : int { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2 };
 
+struct CgState;
+
 struct GemmEntry {
   const void* A;
   const void* Bt;
@@ -31,6 +33,9 @@ struct GemmEntry {
   int M, N, K;
   int lda, ldb, ldc;
   int tiles_m, tiles_n, tile_begin;
+  double s0, s1;       // EPI_SHIFT / EPI_COMBINE scalars (the launcher copies the batch's)
+  double thresh;       // EPI_LAMBDA zero-mode threshold (likewise)
+  const CgState* st;   // table launches with use_state: skip the entry if !st->active
 };
 
 constexpr int kMaxEntries = 8;
@@ -44,8 +49,21 @@ struct GemmBatch {
   double s0, s1, thresh;
 };
 
+// Device-resident entry table for grouped launches over many blocks (a batch of
+// cavities): entries carry their own scalars and tile_begin (ascending, set by
+// tile_table), `begin` mirrors tile_begin for the per-block search.
+struct GemmTable {
+  const GemmEntry* e;
+  const int* begin;
+  int count, total_tiles, epi, trans_c, use_state;
+};
+
 // Tile edge used by the launchers (entries are tiled by the launcher itself).
 void gemm_f64(GemmBatch& b, hipStream_t s);
+// FP64 grouped launch over a device table; tile_table fills tiles_m/n and tile_begin
+// of host entries for the FP64 tile shape and returns the total tile count.
+void gemm_f64_table(const GemmTable& t, hipStream_t s);
+int tile_table_f64(GemmEntry* entries, int count, int* begins);
 // a_type: DT_F64 / DT_F32 / DT_BF16 for A and aux/C per c_type; Bt is always bf16.
 // split: three-product hi/lo form (A must be DT_F32 or DT_F64, Bt_lo set).
 void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s);
@@ -80,6 +98,45 @@ int reduce_blocks(int64_t n);
 int defect_blocks(int64_t size, int ncols);
 
 enum Reduce : int { RED_NONE = 0, RED_SUM = 1, RED_MAXABS = 2 };
+
+// ---- device helpers shared by the vector kernels
+// |x| with NaN mapped to +inf, so max-norm checks catch non-finite values.
+__device__ inline double absn(double x) { return x != x ? (double)INFINITY : fabs(x); }
+
+// n / d for 0 <= n < 2^31 and 1 <= d < 2^15, from one FP64 product with the reciprocal
+// (error far below 1, so the truncated quotient is off by at most one) and a correction
+// step: a few VALU operations instead of an integer division sequence.
+struct SmallDiv {
+  int d;
+  double inv;
+  __device__ explicit SmallDiv(int d_) : d(d_), inv(1.0 / (double)d_) {}
+  __device__ inline int quot(int n) const {
+    int q = (int)((double)n * inv);
+    const int r = n - q * d;
+    if (r < 0) --q;
+    else if (r >= d) ++q;
+    return q;
+  }
+};
+
+// 256-thread block reduction in a fixed order; result valid in thread 0.
+template <int MODE>
+__device__ inline double block_reduce(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(v, off, 64);
+    v = MODE == RED_MAXABS ? fmax(v, o) : v + o;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = MODE == RED_MAXABS ? fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]))
+                           : (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  }
+  return v;
+}
+
 
 // Finalize target of a reduction: when `out` is set, the launcher also reduces the
 // partials it produced into out[c] (finalize(), fixed order).

@@ -16,43 +16,6 @@ __device__ inline double ld_d(const __bf16* p, int64_t i) { return (double)(floa
 __device__ inline void st_d(double* p, int64_t i, double v) { p[i] = v; }
 __device__ inline void st_d(float* p, int64_t i, double v) { p[i] = (float)v; }
 __device__ inline void st_d(__bf16* p, int64_t i, double v) { p[i] = (__bf16)(float)v; }
-// |x| with NaN mapped to +inf, so max-norm checks catch non-finite values.
-__device__ inline double absn(double x) { return x != x ? (double)INFINITY : fabs(x); }
-
-// n / d for 0 <= n < 2^31 and 1 <= d < 2^15, from one FP64 product with the reciprocal
-// (error far below 1, so the truncated quotient is off by at most one) and a correction
-// step: a few VALU operations instead of an integer division sequence.
-struct SmallDiv {
-  int d;
-  double inv;
-  __device__ explicit SmallDiv(int d_) : d(d_), inv(1.0 / (double)d_) {}
-  __device__ inline int quot(int n) const {
-    int q = (int)((double)n * inv);
-    const int r = n - q * d;
-    if (r < 0) --q;
-    else if (r >= d) ++q;
-    return q;
-  }
-};
-
-// 256-thread block reduction in a fixed order; result valid in thread 0.
-template <int MODE>
-__device__ inline double block_reduce(double v, double* sh) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_down(v, off, 64);
-    v = MODE == RED_MAXABS ? fmax(v, o) : v + o;
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    v = MODE == RED_MAXABS ? fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]))
-                           : (sh[0] + sh[1]) + (sh[2] + sh[3]);
-  }
-  return v;
-}
-
 template <typename TIN, int MODE>
 __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, VelLayout L, GridDims g,
                                                   double* __restrict__ out, int64_t out_stride, double sign,
@@ -208,7 +171,7 @@ __global__ void k_copy_cols(const double* src, int64_t ss, double* dst, int64_t 
 // ---- CG scalars (scipy 1.15.3 cg loop: stop when ||r|| < rtol*||b||, tested at the
 // top of each iteration; rho = (r,z); beta = rho/rho_prev; alpha = rho/(p,q)).
 __global__ void k_cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols) {
-  const int c = threadIdx.x;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
   CgState s{};
   s.bnorm = sqrt(bb[c]);
@@ -221,7 +184,7 @@ __global__ void k_cg_init(CgState* st, const double* bb, const double* bmax, dou
 }
 
 __global__ void k_cg_check(CgState* st, const double* rr, int* flags, int ncols) {
-  const int c = threadIdx.x;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
   if (st[c].active) {
     st[c].rr = rr[c];
@@ -234,14 +197,14 @@ __global__ void k_cg_check(CgState* st, const double* rr, int* flags, int ncols)
 }
 
 __global__ void k_cg_rho(CgState* st, const double* rz, int ncols) {
-  const int c = threadIdx.x;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols || !st[c].active) return;
   st[c].rho = rz[c];
   st[c].beta = st[c].iters > 0 ? rz[c] / st[c].rho_prev : 0.0;
 }
 
 __global__ void k_cg_alpha(CgState* st, const double* pq, int ncols) {
-  const int c = threadIdx.x;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols || !st[c].active) return;
   st[c].alpha = st[c].rho / pq[c];
   st[c].rho_prev = st[c].rho;
@@ -442,14 +405,14 @@ void copy_cols(const double* src, int64_t ss, double* dst, int64_t ds, int64_t n
 }
 
 void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s) {
-  k_cg_init<<<1, 64, 0, s>>>(st, bb, bmax, rtol, ncols);
+  k_cg_init<<<(ncols + 63) / 64, 64, 0, s>>>(st, bb, bmax, rtol, ncols);
 }
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s) {
-  k_cg_check<<<1, 64, 0, s>>>(st, rr, flags, ncols);
+  k_cg_check<<<(ncols + 63) / 64, 64, 0, s>>>(st, rr, flags, ncols);
 }
-void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s) { k_cg_rho<<<1, 64, 0, s>>>(st, rz, ncols); }
+void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s) { k_cg_rho<<<(ncols + 63) / 64, 64, 0, s>>>(st, rz, ncols); }
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s) {
-  k_cg_alpha<<<1, 64, 0, s>>>(st, pq, ncols);
+  k_cg_alpha<<<(ncols + 63) / 64, 64, 0, s>>>(st, pq, ncols);
 }
 void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,
                  hipStream_t s) {
