@@ -161,6 +161,30 @@ int32_t mac_velocity_inverse(void* handle, const double* in, int64_t ld_in, int3
 int32_t mac_pressure_poisson_inverse(void* handle, const double* in, int64_t ld_in, int32_t ncols,
                                      double* out, int64_t ld_out);
 
+/* ---- batched cavities ----
+ * Replaces a host loop of `TensorStokes.solve` (tensor_stokes.py:245-364) over many
+ * independent cavities, e.g. one `MACNavierStokes` per cavity stepped in lockstep: every
+ * cavity keeps its own grid, (mass, viscosity) and columns, and its results are those of
+ * the cavity solved alone.  Segments (cavity i, column c) are laid out cavity-major,
+ * column-minor; mac_batch_layout gives each segment's first element in the velocity and
+ * pressure buffers (plus the totals at index count*ncols).  FP64 only. */
+void* mac_batch_create(int32_t count, const int32_t* nx, const int32_t* ny, const double* lx, const double* ly,
+                       int32_t ncols, const mac_stokes_opts* opts);
+const char* mac_batch_create_error(void);
+void mac_batch_destroy(void* handle);
+const char* mac_batch_last_error(void* handle);
+int32_t mac_batch_layout(void* handle, int64_t* velocity_offsets, int64_t* pressure_offsets);
+/* Launch shape: replay one captured CG iteration as a hipGraph (use_graph), and queue
+ * iters_per_check iterations between host reads of the convergence flags (1..16).  The
+ * results do not depend on either. */
+int32_t mac_batch_set_launch(void* handle, int32_t use_graph, int32_t iters_per_check);
+/* mass, viscosity: per cavity.  iterations: per segment.  residual, divergence_inf: per
+ * cavity (max over its columns, the reference's definitions).  Errors as mac_stokes_solve;
+ * the message names the first failing cavity. */
+int32_t mac_stokes_solve_batched(void* handle, const double* rhs, const double* mass, const double* viscosity,
+                                 double* velocity, double* pressure, int32_t* iterations, double* residual,
+                                 double* divergence_inf, int32_t device_pointers);
+
 #ifdef __cplusplus
 }
 #endif

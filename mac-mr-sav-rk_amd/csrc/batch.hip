@@ -1,0 +1,893 @@
+// Batched MAC stage solve: many independent cavities of uneven sizes advanced together.
+//
+// Each cavity runs the reference's TensorStokes.solve (solver/mac/tensor_stokes.py:245-364)
+// on its own grid, viscosity and columns; nothing couples them.  A (cavity, column) pair is
+// a "segment".  Every transform pass is ONE grouped GEMM launch over a device table holding
+// every segment's u/v (or pressure) block, and every vector step is one launch whose grid's
+// y dimension walks the segments.  A segment whose Schur-CG has converged is skipped on the
+// device (its GEMM tiles and vector blocks return at once), so the host only reads the
+// per-segment active flags once per iteration.  One CG iteration is a fixed launch sequence,
+// captured once into a hipGraph and replayed.
+//
+// Per-segment reductions use the same block partition and order as the single-cavity
+// solver (reduce_blocks of the segment's own length), so every segment's iterates, CG
+// iteration count and results are bitwise those of the same cavity solved alone.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host_common.h"
+#include "internal.h"
+#include "macstokes.h"
+
+namespace macs {
+namespace {
+
+struct Seg {
+  int nx, ny;
+  double hx, hy;
+  int nu, size, np;   // u faces, all faces, cells
+  int nbv, nbp;       // reduction blocks over faces / cells
+  int cav;
+  int64_t voff, poff; // first element of the segment's velocity / pressure column
+};
+
+enum Len : int { LEN_FACES = 0, LEN_CELLS = 1 };
+
+// out = add + sign * D vel per segment (add optional); partial sums / max of the result.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_bdiv(const double* __restrict__ vel, double* __restrict__ out,
+                                                   const Seg* segs, double sign, double* partials, int pstride,
+                                                   const CgState* st) {
+  const int s = blockIdx.y;
+  if (st != nullptr && !st[s].active) return;
+  const Seg S = segs[s];
+  if ((int)blockIdx.x >= S.nbp) return;
+  __shared__ double sh[4];
+  const double* u = vel + S.voff;
+  const double* v = u + S.nu;
+  double* o = out + S.poff;
+  const int nx = S.nx, ny = S.ny;
+  const SmallDiv dnx(nx);
+  double accum = 0.0;
+  for (int cell = blockIdx.x * kThreads + threadIdx.x; cell < S.np; cell += S.nbp * kThreads) {
+    const int j = dnx.quot(cell), i = cell - j * nx;
+    double value = 0.0;
+    if (i < nx - 1) value += u[j * (nx - 1) + i] / S.hx;
+    if (i > 0) value -= u[j * (nx - 1) + i - 1] / S.hx;
+    if (j < ny - 1) value += v[j * nx + i] / S.hy;
+    if (j > 0) value -= v[(j - 1) * nx + i] / S.hy;
+    value = sign * value;
+    o[cell] = value;
+    if (MODE == RED_SUM) accum += value;
+    if (MODE == RED_MAXABS) accum = fmax(accum, absn(value));
+  }
+  if (MODE != RED_NONE) {
+    accum = block_reduce<MODE>(accum, sh);
+    if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
+  }
+}
+
+// out = add + D^T p per segment (packed velocity layout; add optional).
+__global__ __launch_bounds__(kThreads) void k_bdt(const double* __restrict__ p, const double* __restrict__ add,
+                                                  double* __restrict__ out, const Seg* segs, const CgState* st) {
+  const int s = blockIdx.y;
+  if (st != nullptr && !st[s].active) return;
+  const Seg S = segs[s];
+  const int f = blockIdx.x * kThreads + threadIdx.x;
+  if (f >= S.size) return;
+  const double* pc = p + S.poff;
+  const int nx = S.nx;
+  double value;
+  if (f < S.nu) {
+    const int j = SmallDiv(nx - 1).quot(f), i = f - j * (nx - 1);
+    const int left = j * nx + i;
+    value = (pc[left] - pc[left + 1]) / S.hx;
+  } else {
+    const int e = f - S.nu;
+    const int j = SmallDiv(nx).quot(e), i = e - j * nx;
+    const int lower = j * nx + i;
+    value = (pc[lower] - pc[lower + nx]) / S.hy;
+  }
+  if (add != nullptr) value = add[S.voff + f] + value;
+  out[S.voff + f] = value;
+}
+
+template <int MODE, bool DOT>
+__global__ __launch_bounds__(kThreads) void k_breduce(const double* __restrict__ x, const double* __restrict__ y,
+                                                      const Seg* segs, int len, double* partials, int pstride,
+                                                      const CgState* st) {
+  const int s = blockIdx.y;
+  if (st != nullptr && !st[s].active) return;
+  const Seg S = segs[s];
+  const int nb = len == LEN_FACES ? S.nbv : S.nbp;
+  if ((int)blockIdx.x >= nb) return;
+  __shared__ double sh[4];
+  const int64_t off = len == LEN_FACES ? S.voff : S.poff;
+  const int n = len == LEN_FACES ? S.size : S.np;
+  double accum = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += nb * kThreads) {
+    if (MODE == RED_MAXABS) accum = fmax(accum, absn(x[off + i]));
+    else accum += DOT ? x[off + i] * y[off + i] : x[off + i];
+  }
+  accum = block_reduce<MODE>(accum, sh);
+  if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
+}
+
+// out[s] = the segment's partials reduced in the order of the single-cavity finalize.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_bfinalize(const double* partials, int pstride, const Seg* segs,
+                                                        int len, double* out) {
+  const int s = blockIdx.x;
+  const Seg S = segs[s];
+  const int nb = len == LEN_FACES ? S.nbv : S.nbp;
+  __shared__ double sh[4];
+  double accum = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) {
+    const double v = partials[(int64_t)s * pstride + i];
+    accum = MODE == RED_MAXABS ? fmax(accum, v) : accum + v;
+  }
+  accum = block_reduce<MODE>(accum, sh);
+  if (threadIdx.x == 0) out[s] = accum;
+}
+
+// x -= sums[s]/np on the cells; optional partial sums of y*x afterwards.
+template <bool DOT>
+__global__ __launch_bounds__(kThreads) void k_bsub_mean_dot(double* x, const double* sums, const double* y,
+                                                            const Seg* segs, double* partials, int pstride,
+                                                            const CgState* st) {
+  const int s = blockIdx.y;
+  if (st != nullptr && !st[s].active) return;
+  const Seg S = segs[s];
+  if ((int)blockIdx.x >= S.nbp) return;
+  __shared__ double sh[4];
+  double* xc = x + S.poff;
+  const double* yc = DOT ? y + S.poff : nullptr;
+  const double mean = sums[s] / (double)S.np;
+  double accum = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads) {
+    const double v = xc[i] - mean;
+    xc[i] = v;
+    if (DOT) accum += yc[i] * v;
+  }
+  if (DOT) {
+    accum = block_reduce<RED_SUM>(accum, sh);
+    if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_bupdate_p(double* __restrict__ p, const double* __restrict__ z,
+                                                        const Seg* segs, const CgState* st) {
+  const int s = blockIdx.y;
+  if (!st[s].active) return;
+  const Seg S = segs[s];
+  if ((int)blockIdx.x >= S.nbp) return;
+  const bool first = st[s].iters == 0;
+  const double beta = st[s].beta;
+  double* pc = p + S.poff;
+  const double* zc = z + S.poff;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads)
+    pc[i] = first ? zc[i] : __dadd_rn(__dmul_rn(pc[i], beta), zc[i]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bupdate_xr(double* __restrict__ x, double* __restrict__ r,
+                                                         const double* __restrict__ p, const double* __restrict__ q,
+                                                         const Seg* segs, const CgState* st, double* partials,
+                                                         int pstride) {
+  const int s = blockIdx.y;
+  if (!st[s].active) return;
+  const Seg S = segs[s];
+  if ((int)blockIdx.x >= S.nbp) return;
+  __shared__ double sh[4];
+  const double alpha = st[s].alpha;
+  const int64_t off = S.poff;
+  double accum = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads) {
+    x[off + i] = __dadd_rn(x[off + i], __dmul_rn(alpha, p[off + i]));
+    const double rv = __dsub_rn(r[off + i], __dmul_rn(alpha, q[off + i]));
+    r[off + i] = rv;
+    accum += rv * rv;
+  }
+  accum = block_reduce<RED_SUM>(accum, sh);
+  if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
+}
+
+// Per segment: max|mass u + visc K u - D^T p - f|, max|f|, max|u| (partials at q * nseg * pstride).
+__global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__ vel, const double* __restrict__ f,
+                                                      const double* __restrict__ p, const Seg* segs,
+                                                      const double* __restrict__ mass_visc, double* partials,
+                                                      int pstride) {
+  const int s = blockIdx.y;
+  const Seg S = segs[s];
+  if ((int)blockIdx.x >= S.nbv) return;
+  __shared__ double sh[4];
+  const int nx = S.nx, ny = S.ny, nu = S.nu;
+  const double hx2 = S.hx * S.hx, hy2 = S.hy * S.hy;
+  const double ox = -1.0 / hx2, oy = -1.0 / hy2;
+  const double mass = mass_visc[2 * s], visc = mass_visc[2 * s + 1];
+  const double* vc = vel + S.voff;
+  const double* fc = f + S.voff;
+  const double* pc = p + S.poff;
+  const SmallDiv dw(nx - 1), dnx(nx);
+  double md = 0.0, mf = 0.0, mv = 0.0;
+  for (int fidx = blockIdx.x * kThreads + threadIdx.x; fidx < S.size; fidx += S.nbv * kThreads) {
+    double kv, lift;
+    if (fidx < nu) {
+      const int w = nx - 1;
+      const int j = dw.quot(fidx), i = fidx - j * w;
+      const double* u = vc;
+      const double diag = 2.0 / hx2 + ((j == 0 || j == ny - 1) ? 3.0 : 2.0) / hy2;
+      kv = 0.0;
+      if (j > 0) kv += oy * u[fidx - w];
+      if (i > 0) kv += ox * u[fidx - 1];
+      kv += diag * u[fidx];
+      if (i < w - 1) kv += ox * u[fidx + 1];
+      if (j < ny - 1) kv += oy * u[fidx + w];
+      const int left = j * nx + i;
+      lift = (pc[left] - pc[left + 1]) / S.hx;
+    } else {
+      const int e = fidx - nu;
+      const int j = dnx.quot(e), i = e - j * nx;
+      const double* v = vc + nu;
+      const double diag = ((i == 0 || i == nx - 1) ? 3.0 : 2.0) / hx2 + 2.0 / hy2;
+      kv = 0.0;
+      if (j > 0) kv += oy * v[e - nx];
+      if (i > 0) kv += ox * v[e - 1];
+      kv += diag * v[e];
+      if (i < nx - 1) kv += ox * v[e + 1];
+      if (j < ny - 2) kv += oy * v[e + nx];
+      const int lower = j * nx + i;
+      lift = (pc[lower] - pc[lower + nx]) / S.hy;
+    }
+    const double fv = fc[fidx];
+    const double vv = vc[fidx];
+    const double d = mass * vv + visc * kv - lift - fv;
+    md = fmax(md, absn(d));
+    mf = fmax(mf, absn(fv));
+    mv = fmax(mv, absn(vv));
+  }
+  md = block_reduce<RED_MAXABS>(md, sh);
+  __syncthreads();
+  mf = block_reduce<RED_MAXABS>(mf, sh);
+  __syncthreads();
+  mv = block_reduce<RED_MAXABS>(mv, sh);
+  if (threadIdx.x == 0) {
+    const int64_t set = (int64_t)gridDim.y * pstride, b = (int64_t)s * pstride + blockIdx.x;
+    partials[b] = md;
+    partials[set + b] = mf;
+    partials[2 * set + b] = mv;
+  }
+}
+
+// dst = src over n elements (kernel form, so it can sit inside a captured graph).
+__global__ void k_bcopy(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    dst[i] = src[i];
+}
+__global__ void k_bfill(double* __restrict__ dst, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    dst[i] = v;
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096); }
+
+struct DevMatrix64 {
+  int n = 0, kind = -1;
+  double* q = nullptr;
+  double* qt = nullptr;
+};
+
+struct Cavity {
+  int nx, ny;
+  double lx, ly, hx, hy;
+  int64_t size, np;
+  double p_thresh;
+  const DevMatrix64 *ux, *uy, *vx, *vy, *px, *py;
+  double *lux, *luy, *lvx, *lvy, *lpx, *lpy;
+};
+
+class BatchSolver {
+ public:
+  BatchSolver(int count, const int* nx, const int* ny, const double* lx, const double* ly, int ncols,
+              const mac_stokes_opts& o)
+      : opts_(o), ncols_(ncols) {
+    HIP_CHECK(hipSetDevice(o.device));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    cav_.resize(count);
+    for (int i = 0; i < count; ++i) {
+      Cavity& C = cav_[i];
+      C.nx = nx[i];
+      C.ny = ny[i];
+      C.lx = lx[i];
+      C.ly = ly[i];
+      C.hx = lx[i] / nx[i];
+      C.hy = ly[i] / ny[i];
+      C.size = (int64_t)(C.nx - 1) * C.ny + (int64_t)C.nx * (C.ny - 1);
+      C.np = (int64_t)C.nx * C.ny;
+      C.ux = matrix(C.nx - 1, KIND_DST1);
+      C.uy = matrix(C.ny, KIND_DST2);
+      C.vx = matrix(C.nx, KIND_DST2);
+      C.vy = matrix(C.ny - 1, KIND_DST1);
+      C.px = matrix(C.nx, KIND_DCT2);
+      C.py = matrix(C.ny, KIND_DCT2);
+      C.lux = upload(eigenvalues_1d(C.nx - 1, C.hx, KIND_DST1));
+      C.luy = upload(eigenvalues_1d(C.ny, C.hy, KIND_DST2));
+      C.lvx = upload(eigenvalues_1d(C.nx, C.hx, KIND_DST2));
+      C.lvy = upload(eigenvalues_1d(C.ny - 1, C.hy, KIND_DST1));
+      const std::vector<double> lpx = eigenvalues_1d(C.nx, C.hx, KIND_DCT2);
+      const std::vector<double> lpy = eigenvalues_1d(C.ny, C.hy, KIND_DCT2);
+      C.lpx = upload(lpx);
+      C.lpy = upload(lpy);
+      double lmax = 0.0;
+      for (double a : lpy)
+        for (double b : lpx) lmax = std::max(lmax, a + b);
+      C.p_thresh = 100.0 * 2.220446049250313e-16 * lmax;
+    }
+    // Segments: cavity-major, column-minor; each segment's columns contiguous.
+    nseg_ = count * ncols;
+    segs_host_.resize(nseg_);
+    int64_t voff = 0, poff = 0;
+    pstride_ = 1;
+    maxfaces_ = 1;
+    for (int i = 0; i < count; ++i)
+      for (int c = 0; c < ncols; ++c) {
+        Seg& S = segs_host_[i * ncols + c];
+        const Cavity& C = cav_[i];
+        S.nx = C.nx;
+        S.ny = C.ny;
+        S.hx = C.hx;
+        S.hy = C.hy;
+        S.nu = (C.nx - 1) * C.ny;
+        S.size = (int)C.size;
+        S.np = (int)C.np;
+        S.nbv = reduce_blocks(C.size);
+        S.nbp = reduce_blocks(C.np);
+        S.cav = i;
+        S.voff = voff;
+        S.poff = poff;
+        voff += C.size;
+        poff += C.np;
+        pstride_ = std::max(pstride_, std::max(S.nbv, S.nbp));
+        nbv_max_ = std::max(nbv_max_, S.nbv);
+        nbp_max_ = std::max(nbp_max_, S.nbp);
+        maxfaces_ = std::max<int64_t>(maxfaces_, C.size);
+      }
+    V_ = voff;
+    P_ = poff;
+    segs_ = static_cast<Seg*>(upload_raw(segs_host_.data(), sizeof(Seg) * nseg_));
+    auto take = [&](size_t bytes) { return dev_alloc(bytes); };
+    const size_t vb = sizeof(double) * V_, pb = sizeof(double) * P_;
+    io_f_ = (double*)take(vb);
+    io_v_ = (double*)take(vb);
+    io_p_ = (double*)take(pb);
+    vin_ = (double*)take(vb);
+    vout_ = (double*)take(vb);
+    s1_ = (double*)take(vb);
+    s2_ = (double*)take(vb);
+    b_ = (double*)take(pb);
+    x_ = (double*)take(pb);
+    r_ = (double*)take(pb);
+    z_ = (double*)take(pb);
+    p_ = (double*)take(pb);
+    q_ = (double*)take(pb);
+    partials_ = (double*)take(sizeof(double) * 3 * (size_t)nseg_ * pstride_);
+    sums_ = (double*)take(sizeof(double) * nseg_);
+    sums2_ = (double*)take(sizeof(double) * nseg_);
+    sums3_ = (double*)take(sizeof(double) * nseg_);
+    rr_ = (double*)take(sizeof(double) * nseg_);
+    chk_ = (double*)take(sizeof(double) * 4 * nseg_);
+    mv_ = (double*)take(sizeof(double) * 2 * nseg_);
+    cg_ = (CgState*)take(sizeof(CgState) * nseg_);
+    flags_ = (int*)take(sizeof(int) * nseg_);
+    HIP_CHECK(hipHostMalloc((void**)&host_flags_, sizeof(int) * nseg_, hipHostMallocDefault));
+    build_tables();
+  }
+
+  ~BatchSolver() {
+    (void)hipSetDevice(opts_.device);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    for (void* p : allocs_) (void)hipFree(p);
+    if (host_flags_) (void)hipHostFree(host_flags_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  int count() const { return (int)cav_.size(); }
+  int ncols() const { return ncols_; }
+  int64_t total_faces() const { return V_; }
+  int64_t total_cells() const { return P_; }
+  const Seg& seg(int s) const { return segs_host_[s]; }
+  const Cavity& cavity(int i) const { return cav_[i]; }
+  hipStream_t stream() const { return stream_; }
+  const mac_stokes_opts& opts() const { return opts_; }
+  double* io_f() { return io_f_; }
+  double* io_v() { return io_v_; }
+  double* io_p() { return io_p_; }
+  std::string error;
+  int use_graph = 1;        // replay one captured CG iteration instead of launching it
+  int iters_per_check = 1;  // CG iterations queued between host reads of the flags
+
+  struct Outcome {
+    std::vector<int> iterations;              // per segment
+    std::vector<double> residual, divergence, vmax;  // per cavity
+    std::vector<int> failed_cg;               // cavities whose CG did not converge
+  };
+
+  // f, vel, pres: device buffers in the batch layout.  mass, visc: per cavity.
+  // Per-segment max|f| (non-finite values reported as inf).
+  std::vector<double> rhs_max(const double* f) {
+    k_breduce<RED_MAXABS, false><<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(f, nullptr, segs_, LEN_FACES,
+                                                                                partials_, pstride_, nullptr);
+    bfinalize(RED_MAXABS, LEN_FACES, sums_);
+    std::vector<double> m(nseg_);
+    HIP_CHECK(hipMemcpyAsync(m.data(), sums_, sizeof(double) * nseg_, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    return m;
+  }
+
+  Outcome solve(const double* f, const double* mass, const double* visc, double* vel, double* pres) {
+    const int nc = count();
+    set_shifts(mass, visc);
+    // W = L^-1 f ; b = -D W, mean removed; ||b||^2 and max|b| per segment.
+    copy(f, vin_, V_);
+    velocity_inverse(false);
+    bdiv(vout_, b_, -1.0, RED_SUM, nullptr);
+    bfinalize(RED_SUM, LEN_CELLS, sums_);
+    k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, sums_, b_, segs_, partials_, pstride_,
+                                                                         nullptr);
+    bfinalize(RED_SUM, LEN_CELLS, sums2_);
+    k_breduce<RED_MAXABS, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, nullptr, segs_, LEN_CELLS,
+                                                                                partials_, pstride_, nullptr);
+    bfinalize(RED_MAXABS, LEN_CELLS, sums3_);
+    cg_init(cg_, sums2_, sums3_, opts_.krylov_tolerance, nseg_, stream_);
+    k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(x_, P_, 0.0);
+    copy(b_, r_, P_);
+    copy(sums2_, rr_, nseg_);
+    // Schur PCG, every segment at once; converged segments drop out on the device.  As in
+    // scipy's loop, the convergence test runs only at the top of an iteration, so none
+    // follows the last of max_iterations.
+    int done_iters = 0;
+    while (done_iters < opts_.max_iterations) {
+      cg_check(cg_, rr_, flags_, nseg_, stream_);
+      HIP_CHECK(hipMemcpyAsync(host_flags_, flags_, sizeof(int) * nseg_, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      int active = 0;
+      for (int s = 0; s < nseg_; ++s) active += host_flags_[s] != 0;
+      if (active == 0) break;
+      // Queue up to iters_per_check iterations (never past max_iterations); the state
+      // test at the top of each queued iteration runs on the device.
+      const int burst = std::min(iters_per_check, opts_.max_iterations - done_iters);
+      for (int k = 0; k < burst; ++k) {
+        if (k > 0) cg_check(cg_, rr_, flags_, nseg_, stream_);
+        iteration();
+      }
+      done_iters += burst;
+    }
+    std::vector<CgState> st(nseg_);
+    HIP_CHECK(hipMemcpyAsync(st.data(), cg_, sizeof(CgState) * nseg_, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    Outcome out;
+    out.iterations.resize(nseg_);
+    for (int s = 0; s < nseg_; ++s) {
+      out.iterations[s] = st[s].iters;
+      const int cav = segs_host_[s].cav;
+      if (!st[s].converged && (out.failed_cg.empty() || out.failed_cg.back() != cav)) out.failed_cg.push_back(cav);
+    }
+    if (!out.failed_cg.empty()) return out;
+    // p = x - mean(x); u = L^-1 (f + D^T p).
+    k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, nullptr, segs_, LEN_CELLS,
+                                                                             partials_, pstride_, nullptr);
+    bfinalize(RED_SUM, LEN_CELLS, sums_);
+    k_bsub_mean_dot<false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, sums_, nullptr, segs_, nullptr,
+                                                                          pstride_, nullptr);
+    k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(x_, f, vin_, segs_, nullptr);
+    velocity_inverse(false);
+    copy(vout_, vel, V_);
+    copy(x_, pres, P_);
+    // Checks: residual and divergence per cavity (max over its columns).
+    k_bdefect<<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(vel, f, pres, segs_, mv_, partials_, pstride_);
+    for (int q = 0; q < 3; ++q)
+      k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_ + (int64_t)q * nseg_ * pstride_, pstride_,
+                                                              segs_, LEN_FACES, chk_ + q * nseg_);
+    bdiv(vel, q_, 1.0, RED_MAXABS, nullptr);
+    bfinalize(RED_MAXABS, LEN_CELLS, chk_ + 3 * nseg_);
+    std::vector<double> h(4 * (size_t)nseg_);
+    HIP_CHECK(hipMemcpyAsync(h.data(), chk_, sizeof(double) * 4 * nseg_, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    out.residual.assign(nc, 0.0);
+    out.divergence.assign(nc, 0.0);
+    out.vmax.assign(nc, 0.0);
+    std::vector<double> dmax(nc, 0.0), fmax_(nc, 0.0);
+    for (int s = 0; s < nseg_; ++s) {
+      const int c = segs_host_[s].cav;
+      dmax[c] = std::max(dmax[c], h[s]);
+      fmax_[c] = std::max(fmax_[c], h[nseg_ + s]);
+      out.vmax[c] = std::max(out.vmax[c], h[2 * nseg_ + s]);
+      out.divergence[c] = std::max(out.divergence[c], h[3 * nseg_ + s]);
+    }
+    for (int c = 0; c < nc; ++c) out.residual[c] = dmax[c] / (1.0 + fmax_[c]);
+    return out;
+  }
+
+ private:
+  // ------------------------------------------------------------ one CG iteration
+  void iteration() {
+    if (use_graph) {
+      if (!graph_exec_) {
+        HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        iteration_body();
+        HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
+        HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+      }
+      HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+    } else {
+      iteration_body();
+    }
+  }
+
+  void iteration_body() {
+    // z = mass (D D^T)^+ r + visc r (per cavity), mean removed; rho = (r, z)
+    pressure_precondition();
+    k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(z_, nullptr, segs_, LEN_CELLS,
+                                                                             partials_, pstride_, cg_);
+    bfinalize(RED_SUM, LEN_CELLS, sums_);
+    k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(z_, sums_, r_, segs_, partials_, pstride_,
+                                                                         cg_);
+    bfinalize(RED_SUM, LEN_CELLS, sums2_);
+    cg_rho(cg_, sums2_, nseg_, stream_);
+    k_bupdate_p<<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(p_, z_, segs_, cg_);
+    // q = S p = D L^-1 D^T p, mean removed; alpha = rho / (p, q)
+    k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(p_, nullptr, vin_, segs_, cg_);
+    velocity_inverse(true);
+    bdiv(vout_, q_, 1.0, RED_SUM, cg_);
+    bfinalize(RED_SUM, LEN_CELLS, sums_);
+    k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(q_, sums_, p_, segs_, partials_, pstride_,
+                                                                         cg_);
+    bfinalize(RED_SUM, LEN_CELLS, sums2_);
+    cg_alpha(cg_, sums2_, nseg_, stream_);
+    k_bupdate_xr<<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, r_, p_, q_, segs_, cg_, partials_, pstride_);
+    bfinalize(RED_SUM, LEN_CELLS, rr_);
+  }
+
+  // ------------------------------------------------------------ transforms
+  // Velocity: vout = L^-1 vin for every segment (per-cavity shifts in the tables).
+  void velocity_inverse(bool use_state) {
+    for (int k = 0; k < 4; ++k) {
+      GemmTable t = vel_tab_[k];
+      t.use_state = use_state ? 1 : 0;
+      gemm_f64_table(t, stream_);
+    }
+  }
+  // Pressure: z = mass (D D^T)^+ r + visc r.
+  void pressure_precondition() {
+    for (int k = 0; k < 4; ++k) gemm_f64_table(pre_tab_[k], stream_);
+  }
+
+  static void pass(std::vector<GemmEntry>& v, const double* A, int lda, const DevMatrix64* Q, double* C, int ldc,
+                   int M, int N, int K, const double* ea, const double* eb, const CgState* st) {
+    GemmEntry e{};
+    e.A = A;
+    e.Bt = Q ? Q->q : nullptr;
+    e.C = C;
+    e.ea = ea;
+    e.eb = eb;
+    e.M = M;
+    e.N = N;
+    e.K = K;
+    e.lda = lda;
+    e.ldb = Q->n;
+    e.ldc = ldc;
+    e.st = st;
+    v.push_back(e);
+  }
+
+  // The four passes of every segment's transforms, as in the single-cavity solver:
+  // x-forward (transposed store), y-forward (+ divide), y-inverse (transposed store),
+  // x-inverse.  Q^T passes take the transposed matrix as Bt.
+  void build_tables() {
+    std::vector<GemmEntry> va, vbb, vc, vd, pa, pbb, pc, pd;
+    for (int s = 0; s < nseg_; ++s) {
+      const Seg& S = segs_host_[s];
+      const Cavity& C = cav_[S.cav];
+      const CgState* st = cg_ + s;
+      struct Blk { int ny, nx; const DevMatrix64 *my, *mx; double *ly, *lx; int64_t off; };
+      const Blk blocks[2] = {{C.ny, C.nx - 1, C.uy, C.ux, C.luy, C.lux, 0},
+                             {C.ny - 1, C.nx, C.vy, C.vx, C.lvy, C.lvx, (int64_t)S.nu}};
+      for (const Blk& B : blocks) {
+        const int64_t o = S.voff + B.off;
+        pass(va, vin_ + o, B.nx, B.mx, s1_ + o, B.ny, B.ny, B.nx, B.nx, nullptr, nullptr, st);
+        pass(vbb, s1_ + o, B.ny, B.my, s2_ + o, B.ny, B.nx, B.ny, B.ny, B.lx, B.ly, st);
+        pass(vc, s2_ + o, B.ny, B.my, s1_ + o, B.nx, B.nx, B.ny, B.ny, nullptr, nullptr, st);
+        vc.back().Bt = B.my->qt;
+        pass(vd, s1_ + o, B.nx, B.mx, vout_ + o, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr, st);
+        vd.back().Bt = B.mx->qt;
+      }
+      const int64_t o = S.poff;
+      // Pressure scratch reuses the segment's velocity scratch: np <= size for nx, ny >= 3.
+      double* t1 = s1_ + S.voff;
+      double* t2 = s2_ + S.voff;
+      pass(pa, r_ + o, C.nx, C.px, t1, C.ny, C.ny, C.nx, C.nx, nullptr, nullptr, st);
+      pass(pbb, t1, C.ny, C.py, t2, C.ny, C.nx, C.ny, C.ny, C.lpx, C.lpy, st);
+      pbb.back().thresh = C.p_thresh;
+      pass(pc, t2, C.ny, C.py, t1, C.nx, C.nx, C.ny, C.ny, nullptr, nullptr, st);
+      pc.back().Bt = C.py->qt;
+      pass(pd, t1, C.nx, C.px, z_ + o, C.nx, C.ny, C.nx, C.nx, nullptr, nullptr, st);
+      pd.back().Bt = C.px->qt;
+      pd.back().aux = r_ + o;
+    }
+    vel_host_[0] = va; vel_host_[1] = vbb; vel_host_[2] = vc; vel_host_[3] = vd;
+    pre_host_[0] = pa; pre_host_[1] = pbb; pre_host_[2] = pc; pre_host_[3] = pd;
+    const int vepi[4] = {EPI_STORE, EPI_SHIFT, EPI_STORE, EPI_STORE};
+    const int pepi[4] = {EPI_STORE, EPI_LAMBDA, EPI_STORE, EPI_COMBINE};
+    const int trans[4] = {1, 0, 1, 0};
+    for (int k = 0; k < 4; ++k) {
+      vel_tab_[k] = make_table(vel_host_[k], vel_dev_[k], vel_beg_[k], vepi[k], trans[k]);
+      pre_tab_[k] = make_table(pre_host_[k], pre_dev_[k], pre_beg_[k], pepi[k], trans[k]);
+    }
+  }
+
+  GemmTable make_table(std::vector<GemmEntry>& host, GemmEntry*& dev, int*& beg, int epi, int trans) {
+    std::vector<int> begins(host.size());
+    GemmTable t{};
+    t.total_tiles = tile_table_f64(host.data(), (int)host.size(), begins.data());
+    dev = static_cast<GemmEntry*>(upload_raw(host.data(), sizeof(GemmEntry) * host.size()));
+    beg = static_cast<int*>(upload_raw(begins.data(), sizeof(int) * begins.size()));
+    t.e = dev;
+    t.begin = beg;
+    t.count = (int)host.size();
+    t.epi = epi;
+    t.trans_c = trans;
+    t.use_state = 1;
+    return t;
+  }
+
+  // Per-cavity (mass, viscosity) into the velocity shift pass, the preconditioner
+  // combination and the defect check.
+  void set_shifts(const double* mass, const double* visc) {
+    std::vector<double> mv(2 * (size_t)nseg_);
+    for (int s = 0; s < nseg_; ++s) {
+      mv[2 * s] = mass[segs_host_[s].cav];
+      mv[2 * s + 1] = visc[segs_host_[s].cav];
+    }
+    bool changed = shifts_.size() != mv.size() || !std::equal(mv.begin(), mv.end(), shifts_.begin());
+    if (!changed) return;
+    shifts_ = mv;
+    for (size_t i = 0; i < vel_host_[1].size(); ++i) {
+      const int s = (int)(i / 2);
+      vel_host_[1][i].s0 = mv[2 * s];
+      vel_host_[1][i].s1 = mv[2 * s + 1];
+    }
+    for (size_t s = 0; s < pre_host_[3].size(); ++s) {
+      pre_host_[3][s].s0 = mv[2 * s];
+      pre_host_[3][s].s1 = mv[2 * s + 1];
+    }
+    HIP_CHECK(hipMemcpyAsync(vel_dev_[1], vel_host_[1].data(), sizeof(GemmEntry) * vel_host_[1].size(),
+                             hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipMemcpyAsync(pre_dev_[3], pre_host_[3].data(), sizeof(GemmEntry) * pre_host_[3].size(),
+                             hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipMemcpyAsync(mv_, mv.data(), sizeof(double) * mv.size(), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+
+  // ------------------------------------------------------------ helpers
+  dim3 seg_grid(int len) const { return dim3((unsigned)(len == LEN_FACES ? nbv_max_ : nbp_max_), (unsigned)nseg_); }
+  dim3 dt_grid() const { return dim3((unsigned)((maxfaces_ + kThreads - 1) / kThreads), (unsigned)nseg_); }
+  void bdiv(const double* vel, double* out, double sign, int mode, const CgState* st) {
+    const dim3 g = seg_grid(LEN_CELLS);
+    if (mode == RED_SUM) k_bdiv<RED_SUM><<<g, kThreads, 0, stream_>>>(vel, out, segs_, sign, partials_, pstride_, st);
+    else k_bdiv<RED_MAXABS><<<g, kThreads, 0, stream_>>>(vel, out, segs_, sign, partials_, pstride_, st);
+  }
+  void bfinalize(int mode, int len, double* out) {
+    if (mode == RED_SUM) k_bfinalize<RED_SUM><<<nseg_, kThreads, 0, stream_>>>(partials_, pstride_, segs_, len, out);
+    else k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_, pstride_, segs_, len, out);
+  }
+  void copy(const double* src, double* dst, int64_t n) {
+    k_bcopy<<<blocks_for(n), kThreads, 0, stream_>>>(src, dst, n);
+  }
+
+  const DevMatrix64* matrix(int n, int kind) {
+    for (auto& m : mats_)
+      if (m->n == n && m->kind == kind) return m.get();
+    auto m = std::make_unique<DevMatrix64>();
+    m->n = n;
+    m->kind = kind;
+    std::vector<double> q = transform_matrix(n, kind);
+    std::vector<double> qt((size_t)n * n);
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) qt[(size_t)b * n + a] = q[(size_t)a * n + b];
+    m->q = upload(q);
+    m->qt = upload(qt);
+    mats_.push_back(std::move(m));
+    return mats_.back().get();
+  }
+  void* dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+    allocs_.push_back(p);
+    return p;
+  }
+  void* upload_raw(const void* h, size_t bytes) {
+    void* d = dev_alloc(bytes);
+    HIP_CHECK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+    return d;
+  }
+  double* upload(const std::vector<double>& v) {
+    return static_cast<double*>(upload_raw(v.data(), v.size() * sizeof(double)));
+  }
+
+  mac_stokes_opts opts_;
+  int ncols_ = 0, nseg_ = 0, pstride_ = 1, nbv_max_ = 1, nbp_max_ = 1;
+  int64_t V_ = 0, P_ = 0, maxfaces_ = 1;
+  hipStream_t stream_ = nullptr;
+  std::vector<Cavity> cav_;
+  std::vector<Seg> segs_host_;
+  Seg* segs_ = nullptr;
+  std::vector<std::unique_ptr<DevMatrix64>> mats_;
+  std::vector<void*> allocs_;
+  double *io_f_ = nullptr, *io_v_ = nullptr, *io_p_ = nullptr;
+  double *vin_ = nullptr, *vout_ = nullptr, *s1_ = nullptr, *s2_ = nullptr;
+  double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;
+  double *partials_ = nullptr, *sums_ = nullptr, *sums2_ = nullptr, *sums3_ = nullptr, *rr_ = nullptr;
+  double *chk_ = nullptr, *mv_ = nullptr;
+  CgState* cg_ = nullptr;
+  int* flags_ = nullptr;
+  int* host_flags_ = nullptr;
+  std::vector<GemmEntry> vel_host_[4], pre_host_[4];
+  GemmEntry *vel_dev_[4] = {}, *pre_dev_[4] = {};
+  int *vel_beg_[4] = {}, *pre_beg_[4] = {};
+  GemmTable vel_tab_[4] = {}, pre_tab_[4] = {};
+  std::vector<double> shifts_;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+};
+
+std::string g_batch_create_error;
+
+int batch_fail(BatchSolver* s, int code, const std::string& msg) {
+  s->error = msg;
+  return code;
+}
+
+template <typename F>
+int batch_guarded(BatchSolver* s, F&& body) {
+  try {
+    s->error.clear();
+    body();
+    return MAC_OK;
+  } catch (const ArgFailure& e) {
+    return batch_fail(s, MAC_INVALID_ARGUMENT, e.what());
+  } catch (const SolveFailure& e) {
+    return batch_fail(s, e.code, e.what());
+  } catch (const std::exception& e) {
+    return batch_fail(s, MAC_HIP_ERROR, e.what());
+  }
+}
+
+}  // namespace
+}  // namespace macs
+
+using macs::BatchSolver;
+
+extern "C" {
+
+void* mac_batch_create(int32_t count, const int32_t* nx, const int32_t* ny, const double* lx, const double* ly,
+                       int32_t ncols, const mac_stokes_opts* opts) {
+  macs::g_batch_create_error.clear();
+  mac_stokes_opts o;
+  mac_stokes_default_opts(&o);
+  if (opts) o = *opts;
+  if (count < 1 || count > 4096 || ncols < 1 || ncols > 8 || !nx || !ny || !lx || !ly) {
+    macs::g_batch_create_error = "Require 1 <= count <= 4096 cavities and 1 <= ncols <= 8";
+    return nullptr;
+  }
+  for (int i = 0; i < count; ++i)
+    if (nx[i] < 3 || ny[i] < 3 || nx[i] > 4096 || ny[i] > 4096 || !std::isfinite(lx[i]) || !std::isfinite(ly[i]) ||
+        lx[i] <= 0 || ly[i] <= 0) {
+      macs::g_batch_create_error = "Require 3 <= nx,ny <= 4096 and positive finite domain lengths (cavity " +
+                                   std::to_string(i) + ")";
+      return nullptr;
+    }
+  if (!(o.tolerance > 0) || !(o.krylov_tolerance > 0) || !std::isfinite(o.tolerance) ||
+      !std::isfinite(o.krylov_tolerance) || o.max_iterations < 1 || o.precision != MAC_PRECISION_FP64) {
+    macs::g_batch_create_error = "Invalid solver controls";
+    return nullptr;
+  }
+  try {
+    return new BatchSolver(count, nx, ny, lx, ly, ncols, o);
+  } catch (const std::exception& e) {
+    macs::g_batch_create_error = e.what();
+    return nullptr;
+  }
+}
+
+const char* mac_batch_create_error(void) { return macs::g_batch_create_error.c_str(); }
+
+void mac_batch_destroy(void* h) { delete static_cast<BatchSolver*>(h); }
+
+const char* mac_batch_last_error(void* h) {
+  BatchSolver* s = static_cast<BatchSolver*>(h);
+  return s ? s->error.c_str() : "null handle";
+}
+
+int32_t mac_batch_layout(void* h, int64_t* velocity_offsets, int64_t* pressure_offsets) {
+  BatchSolver* s = static_cast<BatchSolver*>(h);
+  if (!s || !velocity_offsets || !pressure_offsets) return MAC_INVALID_ARGUMENT;
+  const int n = s->count() * s->ncols();
+  for (int i = 0; i < n; ++i) {
+    velocity_offsets[i] = s->seg(i).voff;
+    pressure_offsets[i] = s->seg(i).poff;
+  }
+  velocity_offsets[n] = s->total_faces();
+  pressure_offsets[n] = s->total_cells();
+  return MAC_OK;
+}
+
+int32_t mac_batch_set_launch(void* h, int32_t use_graph, int32_t iters_per_check) {
+  BatchSolver* s = static_cast<BatchSolver*>(h);
+  if (!s || iters_per_check < 1 || iters_per_check > 16) return MAC_INVALID_ARGUMENT;
+  s->use_graph = use_graph ? 1 : 0;
+  s->iters_per_check = iters_per_check;
+  return MAC_OK;
+}
+
+int32_t mac_stokes_solve_batched(void* h, const double* rhs, const double* mass, const double* viscosity,
+                                 double* velocity, double* pressure, int32_t* iterations, double* residual,
+                                 double* divergence_inf, int32_t device_pointers) {
+  BatchSolver* s = static_cast<BatchSolver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return macs::batch_guarded(s, [&] {
+    const int nc = s->count();
+    if (!rhs || !mass || !viscosity || !velocity || !pressure) throw macs::ArgFailure("Invalid Stokes RHS");
+    for (int i = 0; i < nc; ++i)
+      if (!std::isfinite(mass[i]) || !std::isfinite(viscosity[i]) || std::min(mass[i], viscosity[i]) < 0 ||
+          mass[i] + viscosity[i] <= 0)
+        throw macs::ArgFailure("Require mass,viscosity >= 0 and nonzero total (cavity " + std::to_string(i) + ")");
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    hipStream_t st = s->stream();
+    const int64_t V = s->total_faces(), P = s->total_cells();
+    const double* f = rhs;
+    double* vel = velocity;
+    double* pr = pressure;
+    if (!device_pointers) {
+      HIP_CHECK(hipMemcpyAsync(s->io_f(), rhs, sizeof(double) * V, hipMemcpyHostToDevice, st));
+      f = s->io_f();
+      vel = s->io_v();
+      pr = s->io_p();
+    }
+    // Non-finite right-hand sides are rejected like the reference (ValueError).
+    for (double m : s->rhs_max(f))
+      if (!std::isfinite(m)) throw macs::ArgFailure("Invalid Stokes RHS");
+    BatchSolver::Outcome out = s->solve(f, mass, viscosity, vel, pr);
+    if (iterations)
+      for (size_t i = 0; i < out.iterations.size(); ++i) iterations[i] = out.iterations[i];
+    if (!out.failed_cg.empty())
+      throw macs::SolveFailure(MAC_NOT_CONVERGED, "Pressure Schur CG failed with info=" +
+                                                      std::to_string(s->opts().max_iterations) + " (cavity " +
+                                                      std::to_string(out.failed_cg.front()) + ")");
+    if (!device_pointers) {
+      HIP_CHECK(hipMemcpyAsync(velocity, vel, sizeof(double) * V, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(pressure, pr, sizeof(double) * P, hipMemcpyDeviceToHost, st));
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < nc; ++i) {
+      if (residual) residual[i] = out.residual[i];
+      if (divergence_inf) divergence_inf[i] = out.divergence[i];
+    }
+    const double tol = s->opts().tolerance;
+    for (int i = 0; i < nc; ++i) {
+      if (!(out.residual[i] <= tol) || !(out.divergence[i] <= tol * (1.0 + out.vmax[i]))) {
+        char msg[256];
+        std::snprintf(msg, sizeof(msg), "Stokes residual=%.3e, divergence=%.3e (cavity %d)", out.residual[i],
+                      out.divergence[i], i);
+        throw macs::SolveFailure(MAC_GATE_FAILED, msg);
+      }
+    }
+  });
+}
+
+}  // extern "C"

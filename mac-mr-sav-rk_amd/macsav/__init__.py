@@ -5,8 +5,9 @@ make_scheme, integrate) with `GpuTensorStokes` -- the gfx950 HIP backend -- as t
 stage solver.
 """
 from .mac.grid import MACGrid
+from .mac.batch_stokes import GpuBatchedStokes
 from .mac.gpu_stokes import GpuTensorStokes
 from .mac.operators import MACOperators
 from .mac.stokes import StokesBackend, StokesResult
 
-__all__ = ["MACGrid", "MACOperators", "GpuTensorStokes", "StokesBackend", "StokesResult"]
+__all__ = ["MACGrid", "MACOperators", "GpuTensorStokes", "GpuBatchedStokes", "StokesBackend", "StokesResult"]

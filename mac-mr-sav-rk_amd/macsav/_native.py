@@ -53,6 +53,13 @@ EXPORTED = (
     "mac_apply_divergence_transpose",
     "mac_velocity_inverse",
     "mac_pressure_poisson_inverse",
+    "mac_batch_create",
+    "mac_batch_create_error",
+    "mac_batch_destroy",
+    "mac_batch_last_error",
+    "mac_batch_layout",
+    "mac_batch_set_launch",
+    "mac_stokes_solve_batched",
 )
 
 
@@ -165,6 +172,20 @@ def load() -> ctypes.CDLL:
     lib.mac_velocity_inverse.restype = _I32
     lib.mac_pressure_poisson_inverse.argtypes = [_P, _P, _I64, _I32, _P, _I64]
     lib.mac_pressure_poisson_inverse.restype = _I32
+    lib.mac_batch_create.argtypes = [_I32, _P, _P, _P, _P, _I32, ctypes.POINTER(StokesOpts)]
+    lib.mac_batch_create.restype = _P
+    lib.mac_batch_create_error.argtypes = []
+    lib.mac_batch_create_error.restype = ctypes.c_char_p
+    lib.mac_batch_destroy.argtypes = [_P]
+    lib.mac_batch_destroy.restype = None
+    lib.mac_batch_last_error.argtypes = [_P]
+    lib.mac_batch_last_error.restype = ctypes.c_char_p
+    lib.mac_batch_layout.argtypes = [_P, _P, _P]
+    lib.mac_batch_layout.restype = _I32
+    lib.mac_batch_set_launch.argtypes = [_P, _I32, _I32]
+    lib.mac_batch_set_launch.restype = _I32
+    lib.mac_stokes_solve_batched.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I32]
+    lib.mac_stokes_solve_batched.restype = _I32
     _lib = lib
     return lib
 

@@ -53,3 +53,29 @@ def test_null_handle_is_an_invalid_argument():
     lib = _native.load()
     assert lib.mac_stokes_last_error(None) == b""
     assert lib.mac_apply_divergence(None, None, 0, 1, None, 0) == _native.MAC_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("count,nx,ncols,field,value", [
+    (0, 8, 2, None, None), (2, 2, 2, None, None), (2, 8, 0, None, None), (2, 8, 9, None, None),
+    (2, 8, 2, "tolerance", -1.0), (2, 8, 2, "precision", _native.MAC_PRECISION_MIXED),
+])
+def test_batch_create_rejects_invalid_cavities_and_controls_before_touching_the_gpu(count, nx, ncols, field,
+                                                                                   value):
+    lib = _native.load()
+    opts = _native.default_opts()
+    if field:
+        setattr(opts, field, value)
+    n = max(count, 1)
+    arr_i = (ctypes.c_int32 * n)(*([nx] * n))
+    arr_d = (ctypes.c_double * n)(*([1.0] * n))
+    assert not lib.mac_batch_create(count, arr_i, arr_i, arr_d, arr_d, ncols, ctypes.byref(opts))
+    msg = lib.mac_batch_create_error().decode()
+    assert msg.startswith("Require") or msg.startswith("Invalid solver controls")
+
+
+def test_batch_null_handle_is_an_invalid_argument():
+    lib = _native.load()
+    assert lib.mac_batch_last_error(None) == b"null handle"
+    assert lib.mac_stokes_solve_batched(None, None, None, None, None, None, None, None, None, 0) == \
+        _native.MAC_INVALID_ARGUMENT
+    assert lib.mac_batch_set_launch(None, 1, 1) == _native.MAC_INVALID_ARGUMENT

@@ -1,0 +1,126 @@
+"""GPU parity of the batched cavity solve (mac_stokes_solve_batched).
+
+Every cavity of an uneven batch must come out bitwise equal to the same cavity solved
+alone by the single-cavity library path (same reductions, same GEMM arithmetic), with
+the same Schur-CG iteration counts, and within the oracle bar (rel L2 <= 1e-10, equal
+iteration counts) of the CPU restatement of the reference's TensorStokes.
+"""
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from oracle import mac_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+# Uneven shapes and domains, including odd sizes and a skinny cavity.
+SHAPES = [(16, 12, 1.0, 1.0), (24, 16, 1.3, 0.8), (32, 32, 1.0, 1.0), (40, 24, 2.0, 1.0),
+          (12, 48, 0.5, 1.5), (64, 20, 1.0, 0.4), (33, 17, 1.1, 0.9)]
+
+
+def _grids():
+    from macsav import MACGrid
+    return [MACGrid(nx, ny, lx, ly) for nx, ny, lx, ly in SHAPES]
+
+
+def _rhs(g, seed):
+    # Stage-like right-hand side: a smooth divergence-free field plus its convection.
+    from macsav.fields import stream_function_velocity
+    from macsav.mac.kernels import packed_convection
+    u = stream_function_velocity(g, seed=seed, modes=4)
+    return np.column_stack([u, 0.25 * packed_convection(g, u) + 0.1 * np.random.default_rng(seed).normal(size=g.size)])
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def _shifts():
+    visc = np.array([1e-3, 3e-2, 5e-8, 0.1, 2e-4, 1e-2, 7e-3])
+    return np.ones(len(SHAPES)), visc
+
+
+def test_batch_equals_each_cavity_alone_and_the_oracle():
+    from macsav import GpuBatchedStokes, GpuTensorStokes
+    grids = _grids()
+    rhs = [_rhs(g, 10 + i) for i, g in enumerate(grids)]
+    mass, visc = _shifts()
+    batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9)
+    print(f"batch of {len(grids)} cavities, {batch.total_faces} faces, {batch.total_cells} cells", flush=True)
+    out = batch.solve(rhs, mass=mass, viscosity=visc)
+    for i, g in enumerate(grids):
+        alone = GpuTensorStokes(g, tolerance=1e-9)
+        ref = alone.solve(rhs[i], mass=mass[i], viscosity=visc[i])
+        np.testing.assert_array_equal(out[i].velocity, ref.velocity)
+        np.testing.assert_array_equal(out[i].pressure, ref.pressure)
+        assert batch.last_iterations[i] == list(alone.last_iterations)
+        assert out[i].residual == ref.residual and out[i].divergence_inf == ref.divergence_inf
+        orc = mo.TensorStokes(mo.Grid(g.nx, g.ny, g.lx, g.ly), tolerance=1e-9)
+        want = orc.solve(rhs[i], mass=mass[i], viscosity=visc[i])
+        assert _rel(out[i].velocity, want.velocity) < 1e-10
+        assert _rel(out[i].pressure, want.pressure) < 1e-10
+        assert batch.last_iterations[i] == list(orc.last_iterations)
+
+
+def test_launch_shapes_and_new_shifts_give_identical_results():
+    from macsav import GpuBatchedStokes
+    grids = _grids()
+    batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9)
+    flat = torch.as_tensor(batch.pack([_rhs(g, 30 + i) for i, g in enumerate(grids)]), device="cuda:0")
+    mass, visc = _shifts()
+    first = batch.solve_packed(flat, mass=mass, viscosity=visc)
+    base_v, base_p = first["velocity"].clone(), first["pressure"].clone()
+    for shape in [(1, 1), (1, 3), (0, 1), (0, 2)]:
+        batch.set_launch(*shape)
+        again = batch.solve_packed(flat, mass=mass, viscosity=visc)
+        assert torch.equal(again["velocity"], base_v) and torch.equal(again["pressure"], base_p), shape
+        assert again["iterations"] == first["iterations"]
+    # New per-cavity shifts reach the device tables (and the captured graph replays them).
+    batch.set_launch(1, 1)
+    other = batch.solve_packed(flat, mass=0.7 * mass, viscosity=2 * visc)
+    from macsav import GpuTensorStokes
+    g = grids[3]
+    s = 3 * batch.ncols
+    seg = [other["velocity"][batch.velocity_offsets[s + c]:batch.velocity_offsets[s + c] + g.size] for c in (0, 1)]
+    alone = GpuTensorStokes(g, tolerance=1e-9).solve(
+        torch.stack([flat[batch.velocity_offsets[s + c]:batch.velocity_offsets[s + c] + g.size] for c in (0, 1)], 1),
+        mass=0.7 * mass[3], viscosity=2 * visc[3])
+    assert torch.equal(torch.stack(seg, 1), alone.velocity)
+
+
+def test_autotune_picks_a_listed_shape_and_keeps_results():
+    from macsav import GpuBatchedStokes
+    grids = _grids()
+    batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9)
+    flat = torch.as_tensor(batch.pack([_rhs(g, 50 + i) for i, g in enumerate(grids)]), device="cuda:0")
+    mass, visc = _shifts()
+    ref = batch.solve_packed(flat, mass=mass, viscosity=visc)["velocity"].clone()
+    timings = batch.autotune(flat, mass=mass, viscosity=visc, repeats=1)
+    print("launch shape timings (s):", {k: round(v, 5) for k, v in timings.items()}, "chosen", batch.launch)
+    assert batch.launch in timings
+    assert torch.equal(batch.solve_packed(flat, mass=mass, viscosity=visc)["velocity"], ref)
+
+
+def test_batch_failures_raise_like_the_reference():
+    from macsav import GpuBatchedStokes
+    grids = _grids()[:3]
+    rhs = [_rhs(g, 70 + i) for i, g in enumerate(grids)]
+    with pytest.raises(RuntimeError, match="Pressure Schur CG failed"):
+        GpuBatchedStokes(grids, ncols=2, max_iterations=1).solve(rhs, mass=1.0, viscosity=0.1)
+    b = GpuBatchedStokes(grids, ncols=2)
+    with pytest.raises(ValueError):
+        b.solve(rhs, mass=[1.0, -1.0, 1.0], viscosity=0.1)
+    bad = [r.copy() for r in rhs]
+    bad[1][3, 0] = np.nan
+    with pytest.raises(ValueError):
+        b.solve(bad, mass=1.0, viscosity=0.1)
+    with pytest.raises(ValueError):
+        b.solve(rhs[:2], mass=1.0, viscosity=0.1)
+    with pytest.raises(ValueError):
+        GpuBatchedStokes([SimpleNamespace(nx=2, ny=8, lx=1.0, ly=1.0)], ncols=2)
+    # Still usable after the errors.
+    ok = b.solve(rhs, mass=1.0, viscosity=0.1)
+    assert all(r.residual < 1e-9 for r in ok)
