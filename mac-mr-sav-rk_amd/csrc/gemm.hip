@@ -383,22 +383,25 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
 
   // Pipeline: LDS buffers L0/L1 alternate; two register stages keep the global loads
   // of tiles k+1 and k+2 in flight while tile k computes.
+  // The loop runs over pairs of k-tiles with a single exit (an odd last tile is
+  // computed after it), so the accumulators stay in place across iterations.
   gload(0, S0);
   if (nk > 1) gload(1, S1);
   sstore(0, S0);
   __syncthreads();
-  for (int kt = 0; kt < nk; kt += 2) {
-    // tile kt is in L0; tile kt+1 (if any) is in flight in S1
+  int kt = 0;
+  for (; kt + 2 <= nk; kt += 2) {
+    // tile kt is in L0; tile kt+1 is in flight in S1
     if (kt + 2 < nk) gload(kt + 2, S0);
     compute(0);
-    if (kt + 1 < nk) sstore(1, S1);
+    sstore(1, S1);
     __syncthreads();
-    if (kt + 1 >= nk) break;
     if (kt + 3 < nk) gload(kt + 3, S1);
     compute(1);
     if (kt + 2 < nk) sstore(0, S0);
     __syncthreads();
   }
+  if (kt < nk) compute(0);
 
   // f32-result MFMA layout: register r of lane l holds row 4*(l/16) + r, column l%16.
   const bool vec_t = EPI == EPI_STORE && batch.trans_c && (E.ldc & 3) == 0;

@@ -171,9 +171,29 @@ void sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t 
 void copy_cols(const double* src, int64_t src_stride, double* dst, int64_t dst_stride, int64_t n, int ncols,
                hipStream_t s);
 
+// Host-visible result slot in coherent pinned memory.  A post kernel writes the values,
+// fences at system scope, then stores `seq`; the host spins on `seq` instead of issuing a
+// device-to-host copy and a stream synchronisation (one round trip per CG iteration).
+struct Mailbox {
+  int seq;
+  int pad[15];
+  int flags[64];
+  int iters[64];
+  int converged[64];
+  double vals[4 + 64];
+};
+// cg_check (ncols <= 64), then the columns' active / iteration / converged state into mb.
+void cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox* mb, int seq, hipStream_t s);
+// mb->vals = [a[0..na), b[0..nb)] (na + nb <= 68), then mb->seq = seq.
+void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s);
+
 // CG scalar steps (one thread per column).
 void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s);
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s);
+// tol = max(rtol * sqrt(rr), floor_rel * ||b||) on active columns: a tolerance relative
+// to the current residual (used after a warm start inside the mixed solve's first sweep;
+// the floor keeps a near-exact guess from asking more than the bf16 operator can give).
+void cg_retol(CgState* st, const double* rr, double rtol, double floor_rel, int ncols, hipStream_t s);
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s);
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s);
 void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,

@@ -196,6 +196,41 @@ __global__ void k_cg_check(CgState* st, const double* rr, int* flags, int ncols)
   flags[c] = st[c].active;
 }
 
+__global__ void k_cg_retol(CgState* st, const double* rr, double rtol, double floor_rel, int ncols) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncols || !st[c].active) return;
+  st[c].tol = fmax(rtol * sqrt(rr[c]), floor_rel * st[c].bnorm);
+}
+
+__global__ void k_cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox* mb, int seq) {
+  const int c = threadIdx.x;
+  if (c < ncols) {
+    if (st[c].active) {
+      st[c].rr = rr[c];
+      if (sqrt(rr[c]) < st[c].tol) {
+        st[c].active = 0;
+        st[c].converged = 1;
+      }
+    }
+    flags[c] = st[c].active;
+    mb->flags[c] = st[c].active;
+    mb->iters[c] = st[c].iters;
+    mb->converged[c] = st[c].converged;
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void k_post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq) {
+  const int t = threadIdx.x;
+  if (t < na) mb->vals[t] = a[t];
+  else if (t < na + nb) mb->vals[t] = b[t - na];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void k_cg_rho(CgState* st, const double* rz, int ncols) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols || !st[c].active) return;
@@ -321,10 +356,13 @@ inline dim3 col_grid(int64_t n, int ncols) { return dim3((unsigned)reduce_blocks
 
 int defect_blocks(int64_t size, int ncols) { return reduce_blocks(size) * ncols; }
 
+// Two elements per thread up to 4096 blocks per column: enough waves in flight to cover
+// HBM latency in the grid-stride loops (8 per thread at most 1024 blocks left these
+// HBM-bound kernels latency-bound at ~2 TB/s).
 int reduce_blocks(int64_t n) {
-  int64_t b = (n + (int64_t)kThreads * 8 - 1) / ((int64_t)kThreads * 8);
+  int64_t b = (n + (int64_t)kThreads * 2 - 1) / ((int64_t)kThreads * 2);
   if (b < 1) b = 1;
-  if (b > 1024) b = 1024;
+  if (b > 4096) b = 4096;
   return (int)b;
 }
 
@@ -409,6 +447,15 @@ void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int
 }
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s) {
   k_cg_check<<<(ncols + 63) / 64, 64, 0, s>>>(st, rr, flags, ncols);
+}
+void cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox* mb, int seq, hipStream_t s) {
+  k_cg_check_post<<<1, 64, 0, s>>>(st, rr, flags, ncols, mb, seq);
+}
+void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s) {
+  k_post_values<<<1, 128, 0, s>>>(a, na, b, nb, mb, seq);
+}
+void cg_retol(CgState* st, const double* rr, double rtol, double floor_rel, int ncols, hipStream_t s) {
+  k_cg_retol<<<(ncols + 63) / 64, 64, 0, s>>>(st, rr, rtol, floor_rel, ncols);
 }
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s) { k_cg_rho<<<(ncols + 63) / 64, 64, 0, s>>>(st, rz, ncols); }
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s) {

@@ -86,6 +86,9 @@ class Solver {
       lay_b_.stride = tstride_;
     }
     HIP_CHECK(hipHostMalloc((void**)&host_flags_, sizeof(int) * kMaxCols, hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void**)&mailbox_, sizeof(Mailbox), hipHostMallocCoherent | hipHostMallocMapped));
+    std::memset(mailbox_, 0, sizeof(Mailbox));
+    HIP_CHECK(hipHostGetDevicePointer((void**)&mailbox_dev_, mailbox_, 0));
   }
 
   ~Solver() {
@@ -94,6 +97,7 @@ class Solver {
     for (hipEvent_t e : all_events_) (void)hipEventDestroy(e);
     for (void* p : allocs_) (void)hipFree(p);
     if (host_flags_) (void)hipHostFree(host_flags_);
+    if (mailbox_) (void)hipHostFree(mailbox_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -461,16 +465,13 @@ class Solver {
       defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_, fin(sums_));
       divergence(DT_F64, uacc_, packed_, g_, rc_, np_, -1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr,
                  stream_, nullptr, fin(sums2_));
-      double h[3];
-      std::vector<double> dv(ncols);
-      HIP_CHECK(hipMemcpyAsync(h, sums_, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
-      HIP_CHECK(hipMemcpyAsync(dv.data(), sums2_, sizeof(double) * ncols, hipMemcpyDeviceToHost, stream_));
-      HIP_CHECK(hipStreamSynchronize(stream_));
-      out.residual = h[0] / (1.0 + h[1]);
-      out.fmax = h[1];
-      out.vmax = h[2];
+      post_values(sums_, 3, sums2_, ncols, mailbox_dev_, ++seq_, stream_);
+      const volatile Mailbox* mb = wait_post(seq_);
+      out.residual = mb->vals[0] / (1.0 + mb->vals[1]);
+      out.fmax = mb->vals[1];
+      out.vmax = mb->vals[2];
       out.divergence = 0.0;
-      for (double d : dv) out.divergence = std::max(out.divergence, d);
+      for (int c = 0; c < ncols; ++c) out.divergence = std::max(out.divergence, (double)mb->vals[3 + c]);
       const bool done = out.residual <= opts_.refine_tolerance && out.divergence <= opts_.refine_tolerance;
       if (done || sweep >= opts_.max_sweeps || !std::isfinite(out.residual) || !std::isfinite(out.divergence))
         break;
@@ -487,9 +488,12 @@ class Solver {
       HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
       accumulate_ = true;
       if (sweep == 0 && guess == nullptr && learned_guess(b_, ncols)) {
-        // First sweep: the network's pressure starts the inner Schur CG (p was zero).
+        // First sweep: the learned (or previous-stage) pressure starts the inner Schur CG
+        // (p was zero), and the inner tolerance becomes relative to the warm-started
+        // residual, so a good guess carries its accuracy into the outer iterate.
         out.warm = true;
         start_from(io_g_, np_, ncols, mass, visc, true);
+        cg_retol(cg_, rr_, opts_.inner_tolerance, 1e-6, ncols, stream_);
       }
       pcg(ncols, mass, visc, inner, false);
       accumulate_ = false;
@@ -593,14 +597,24 @@ class Solver {
     ncols_cur_ = ncols;
     use_state_ = true;
     std::vector<int> active;
+    bool settled = false;  // every column left the loop through the convergence test
     for (int it = 0; it < opts_.max_iterations; ++it) {
-      cg_check(cg_, rr_, flags_, ncols, stream_);
-      HIP_CHECK(hipMemcpyAsync(host_flags_, flags_, sizeof(int) * ncols, hipMemcpyDeviceToHost, stream_));
-      HIP_CHECK(hipStreamSynchronize(stream_));
+      cg_check_post(cg_, rr_, flags_, ncols, mailbox_dev_, ++seq_, stream_);
+      const volatile Mailbox* mb = wait_post(seq_);
       active.clear();
       for (int c = 0; c < ncols; ++c)
-        if (host_flags_[c]) active.push_back(c);
-      if (active.empty()) break;
+        if (mb->flags[c]) active.push_back(c);
+      if (active.empty()) {
+        use_state_ = false;
+        for (int c = 0; c < ncols; ++c) {
+          iters[c] = mb->iters[c];
+          if (fail && !mb->converged[c])
+            throw SolveFailure(MAC_NOT_CONVERGED,
+                               "Pressure Schur CG failed with info=" + std::to_string(opts_.max_iterations));
+        }
+        settled = true;
+        break;
+      }
       // z = mass (D D^T)^+ r + visc r, mean removed; rho = (r, z)
       if (mixed_) pressure_inverse_bf16(r_, z_, active, mass, visc);
       else pressure_inverse(r_, np_, z_, np_, active, true, mass, visc);
@@ -615,6 +629,7 @@ class Solver {
       if (accumulate_) accum_velocity(du_, wf_, size_, size_, ncols, cg_, nullptr, nullptr, stream_);
     }
     use_state_ = false;
+    if (settled) return;
     // A column still active after max_iterations completed iterations failed (the
     // convergence test is only made at the top of an iteration, as in scipy's cg).
     std::vector<CgState> st(ncols);
@@ -645,16 +660,13 @@ class Solver {
     defect_max(vc, fc, p, g_, ncols, mass, visc, partials_, stream_, nullptr, fin(sums_));
     divergence(DT_F64, vc, packed_, g_, q_, np_, 1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr, stream_,
                nullptr, fin(sums2_));
-    double h[3];
-    std::vector<double> dv(ncols);
-    HIP_CHECK(hipMemcpyAsync(h, sums_, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
-    HIP_CHECK(hipMemcpyAsync(dv.data(), sums2_, sizeof(double) * ncols, hipMemcpyDeviceToHost, stream_));
-    HIP_CHECK(hipStreamSynchronize(stream_));
-    out.residual = h[0] / (1.0 + h[1]);
-    out.fmax = h[1];
-    out.vmax = h[2];
+    post_values(sums_, 3, sums2_, ncols, mailbox_dev_, ++seq_, stream_);
+    const volatile Mailbox* mb = wait_post(seq_);
+    out.residual = mb->vals[0] / (1.0 + mb->vals[1]);
+    out.fmax = mb->vals[1];
+    out.vmax = mb->vals[2];
     out.divergence = 0.0;
-    for (double d : dv) out.divergence = std::max(out.divergence, d);
+    for (int c = 0; c < ncols; ++c) out.divergence = std::max(out.divergence, (double)mb->vals[3 + c]);
   }
 
   void build_bases() {
@@ -831,6 +843,26 @@ class Solver {
   CgState* cg_ = nullptr;
   int* flags_ = nullptr;
   int* host_flags_ = nullptr;
+  Mailbox* mailbox_ = nullptr;      // host view (coherent pinned memory)
+  Mailbox* mailbox_dev_ = nullptr;  // device view of the same memory
+  int seq_ = 0;
+
+  // Spin until the mailbox holds sequence number `seq`.  The stream is polled now and
+  // then, so a failed or finished stream ends the wait with an error instead of a hang.
+  const volatile Mailbox* wait_post(int seq) {
+    const volatile Mailbox* mb = mailbox_;
+    for (long spin = 1;; ++spin) {
+      if (__atomic_load_n(&mailbox_->seq, __ATOMIC_ACQUIRE) == seq) return mb;
+      if ((spin & 255) == 0) {
+        const hipError_t e = hipStreamQuery(stream_);
+        if (e == hipSuccess) {
+          if (__atomic_load_n(&mailbox_->seq, __ATOMIC_ACQUIRE) == seq) return mb;
+          throw HipFailure("result slot was not written by the finished stream");
+        }
+        if (e != hipErrorNotReady) throw HipFailure(std::string("stream failed: ") + hipGetErrorString(e));
+      }
+    }
+  }
 };
 
 }  // namespace macs
