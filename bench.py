@@ -1,21 +1,27 @@
 #!/usr/bin/env python3
 """Stage-solve throughput of the MAC shifted-Stokes solve at 1024^2 on one MI355X.
 
-Workload (the headline configuration): MAC 1024x1024 on the unit square, nu=1e-3,
-sdirk3_mrsav with dt=2e-4, so every stage solve has mass=1 and viscosity
-nu*dt*D[i,i] = 5e-8, and carries the two right-hand sides [stage rhs, convection]
-sharing one operator.  The stage pairs are the first-stage pairs of sdirk3_mrsav,
-[U_n, D_hat[0,0] N(U_n)], for four seeded stream-function fields with modes <= 32
-(synthetic data of the workload's shape); they are resident in HBM before timing.
+Workload (the headline configuration): MAC 1024x1024 on the unit square, nu = 1e-3,
+sdirk3_mrsav with dt = 2e-4 from the seeded stream-function field with modes <= 32, so
+every stage solve has mass 1 and viscosity nu*dt*D[i,i] = 5e-8 and carries the two
+right-hand sides [stage rhs, convection] sharing one operator.  The stage solves of a
+run are recorded (an FP64 run through the MAC model, one step of start-up first) and
+replayed in order from HBM: W untimed solves, then K timed ones.  One "step" is one
+two-RHS stage solve to residual <= 1e-10 and max|D u| <= 1e-10 (absolute, the
+reference's definitions).
 
-One "step" is one 2-RHS stage solve to the bar residual <= 1e-10 and max|D u| <= 1e-10
-(absolute, the reference's definitions).  The plain FP64 form reaches it with Schur-CG
-rtol 1e-14; the mixed form (bf16 MFMA transforms, FP64 defect correction) stops its
-outer loop there.  `value` is the default (mixed) form's stage solves per second on one
-GPU; the plain FP64 figure is reported beside it.  The dominant kernel (the transform
-GEMM) is timed live with HIP events on the solver's own stream over a second pass of
-the same K solves.  The CPU baseline times the oracle (NumPy/SciPy restatement of the
-reference path) on a bounded sample.
+Forms, all on the same recorded right-hand sides:
+  plain FP64      the reference algorithm (zero start; Schur-CG rtol 1e-14, which this
+                  size needs for the 1e-10 divergence bar)
+  FP64 + history  the same, started from the same-stage extrapolation of past steps
+  mixed           bf16x3 MFMA transforms inside FP64 defect correction, zero start
+  mixed + history the mixed form started from the extrapolated velocity and pressure
+  mixed + network the mixed form started from the trained CNN's correction of the
+                  extrapolated pressure (trained here on another seed's run)
+`value` is the fastest mixed form's stage solves/s; the others are reported beside it.
+The dominant kernel (the bf16 transform GEMM) is timed live with HIP events on the
+solver's own stream over a second pass of the same K solves.  The CPU baseline times
+the oracle (NumPy/SciPy restatement of the reference path) on a bounded sample.
 """
 from __future__ import annotations
 
@@ -33,6 +39,7 @@ for _p in (str(ROOT), str(ROOT / "mac-mr-sav-rk_amd")):
 
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 2500.0}  # MI355X dense: FP64 MFMA, BF16 MFMA
 NU, DT, STAGE_DIAG = 1e-3, 2e-4, 0.25            # sdirk3 D[i,i] = 1/4
+STAGES = 4                                       # sdirk3_mrsav stage solves per step
 CALIBRATION_BYTES = 512 << 20                    # one plain copy of known size, before timing
 TRAFFIC_FILE = ROOT / "profiles" / "k_gemm_traffic_1024.md"
 
@@ -62,15 +69,50 @@ def measured_traffic(precision: str):
     return None
 
 
-def stage_pairs(grid, count: int, modes: int):
+def record_stage_solves(grid, count: int, seed: int = 0, modes: int = 32):
+    """Right-hand sides (device tensors), mass and viscosity of `count` consecutive
+    sdirk3_mrsav stage solves of a plain FP64 run through the MAC model, after one step
+    of start-up."""
     import numpy as np
+    import torch
+    from macsav import GpuTensorStokes
     from macsav.fields import stream_function_velocity
-    from macsav.mac.kernels import packed_convection
-    pairs = []
-    for seed in range(count):
-        U = stream_function_velocity(grid, seed=seed, modes=modes)
-        pairs.append(np.column_stack([U, STAGE_DIAG * packed_convection(grid, U)]))
-    return pairs
+    from macsav.mac_ns import MACNavierStokes
+    from macsav.schemes import make_scheme
+    backend = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14)
+    model = MACNavierStokes(grid, NU, backend=backend)
+    scheme = make_scheme("sdirk3_mrsav")
+    state = model.state(0.0, stream_function_velocity(grid, seed=seed, modes=modes))
+    state = scheme.step(model, state, DT).state
+    rec = []
+    original = backend.solve
+
+    def recording(rhs, *, mass, viscosity, pressure_guess=None):
+        rec.append((torch.as_tensor(np.array(rhs), device="cuda:0"), mass, viscosity))
+        return original(rhs, mass=mass, viscosity=viscosity, pressure_guess=pressure_guess)
+
+    backend.solve = recording
+    while len(rec) < count:
+        state = scheme.step(model, state, DT).state
+    backend.solve = original
+    return rec[:count]
+
+
+def train_warm_start(grid, seed: int = 7, steps: int = 6):
+    """The CNN warm start trained on (b, base, p) pairs of another seed's FP64 run."""
+    from macsav import GpuTensorStokes
+    from macsav.fields import stream_function_velocity
+    from macsav.learned import WarmStartNet, collect_pairs, fit_alpha, train
+    from macsav.mac_ns import MACNavierStokes
+    from macsav.schemes import make_scheme
+    plain = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14)
+    model = MACNavierStokes(grid, NU, backend=plain)
+    scheme = make_scheme("sdirk3_mrsav")
+    state = scheme.step(model, model.state(0.0, stream_function_velocity(grid, seed=seed, modes=32)), DT).state
+    pairs = collect_pairs(model, scheme, state, steps=steps, dt=DT, period=STAGES)
+    net = WarmStartNet(width=32, seed=0, alpha=fit_alpha(grid, pairs))
+    losses = train(net, grid, pairs, epochs=40, lr=1e-3, batch=8)
+    return net, {"pairs": len(pairs.b), "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1]}
 
 
 def cpu_baseline(grid, pair, kappa: float, budget_s: float) -> dict:
@@ -161,16 +203,58 @@ def bench_cavities(args) -> dict:
             "cg_iterations_per_rhs": its / (2 * n * args.steps), "setup_seconds": setup}
 
 
+def replay(grid, rec, warmup: int, steps: int, precision: str, warm=None, profile: bool = False):
+    """Solve rec[:warmup] untimed (they also fill the warm-start history), then time
+    rec[warmup:warmup+steps]; optionally a second, profiled pass of the timed solves."""
+    import torch
+    from macsav import GpuTensorStokes
+    solver = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14, precision=precision,
+                             refine_tolerance=1e-10)
+    if warm is not None:
+        solver.set_warm_start(warm, period=STAGES)
+
+    def run_once(timed_results=None):
+        solver.reset_history()
+        for f, m, v in rec[:warmup]:
+            solver.solve(f, mass=m, viscosity=v)
+        torch.cuda.synchronize()
+        it0, sw0, w0 = solver.total_iterations, solver.sweeps, solver.warm_starts
+        worst = [0.0, 0.0]
+        t0 = time.perf_counter()
+        for f, m, v in rec[warmup:warmup + steps]:
+            res = solver.solve(f, mass=m, viscosity=v)
+            worst = [max(worst[0], res.residual), max(worst[1], res.divergence_inf)]
+            if timed_results is not None:
+                timed_results.append(res.velocity)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        return elapsed, solver.total_iterations - it0, solver.sweeps - sw0, solver.warm_starts - w0, worst
+
+    velocities = []
+    elapsed, its, sweeps, warm_used, worst = run_once(velocities)
+    stats = {"stage_solves_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+             "cg_iterations_per_rhs": its / (2 * steps), "sweeps_per_solve": sweeps / steps,
+             "warm_started_solves": warm_used, "worst_residual": worst[0], "worst_divergence_inf": worst[1]}
+    prof = None
+    if profile:
+        solver.set_profiling(True)
+        run_once()
+        prof = solver.kernel_profile()
+        solver.set_profiling(False)
+        stats["gemm_share_of_step"] = prof["gemm_ms"] * 1e-3 / (elapsed + 1e-30)
+    return stats, prof, velocities
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--steps", type=int, default=20, help="timed stage solves")
+    ap.add_argument("--warmup", type=int, default=12, help="untimed stage solves before them (3 steps)")
     ap.add_argument("--n", type=int, default=1024, help="grid size (n x n)")
     ap.add_argument("--modes", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-network", action="store_true")
     ap.add_argument("--workload", choices=["stage1024", "cavities"], default="stage1024")
     ap.add_argument("--cavities", type=int, default=256)
     args = ap.parse_args()
@@ -183,90 +267,70 @@ def main() -> int:
         print(json.dumps(bench_cavities(args)), flush=True)
         return 0
 
-    import numpy as np
     import torch
-    from macsav import GpuTensorStokes, MACGrid
+    from macsav import MACGrid
 
     grid = MACGrid(args.n, args.n)
     kappa = NU * DT * STAGE_DIAG
     calibration_copy()
-    pairs_host = stage_pairs(grid, 4, args.modes)
-    pairs = [torch.as_tensor(p, device="cuda:0") for p in pairs_host]
+    t0 = time.perf_counter()
+    rec = record_stage_solves(grid, args.warmup + args.steps, modes=args.modes)
+    record_s = time.perf_counter() - t0
+    W, K = args.warmup, args.steps
 
-    def run(precision: str):
-        # Both forms stop at the same bar: residual <= 1e-10 and max|D u| <= 1e-10 (absolute);
-        # the FP64 form needs Schur-CG rtol 1e-14 for that at this size.
-        solver = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14, precision=precision,
-                                 refine_tolerance=1e-10)
-        for k in range(args.warmup):
-            solver.solve(pairs[k % 4], mass=1.0, viscosity=kappa)
-        torch.cuda.synchronize()
-        iters0, sweeps0 = solver.total_iterations, solver.sweeps
-        worst_res, worst_div = 0.0, 0.0
-        t0 = time.perf_counter()
-        results = []
-        for k in range(args.steps):
-            res = solver.solve(pairs[k % 4], mass=1.0, viscosity=kappa)
-            worst_res, worst_div = max(worst_res, res.residual), max(worst_div, res.divergence_inf)
-            if k < 4:
-                results.append(res)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        iters, sweeps = solver.total_iterations - iters0, solver.sweeps - sweeps0
-        # Live kernel timing of the same K solves (HIP events on the solver's stream).
-        solver.set_profiling(True)
-        for k in range(args.steps):
-            solver.solve(pairs[k % 4], mass=1.0, viscosity=kappa)
-        prof = solver.kernel_profile()
-        solver.set_profiling(False)
-        stats = {"stage_solves_per_s": args.steps / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
-                 "cg_iterations_per_rhs": iters / (2 * args.steps),
-                 "sweeps_per_solve": sweeps / args.steps,
-                 "worst_residual": worst_res, "worst_divergence_inf": worst_div,
-                 "gemm_share_of_step": prof["gemm_ms"] * 1e-3 / (elapsed + 1e-30)}
-        return stats, prof, results
+    plain, _, plain_v = replay(grid, rec, W, K, "fp64")
+    forms = {"fp64": plain}
+    forms["fp64+history"], _, _ = replay(grid, rec, W, K, "fp64", warm="history")
+    forms["mixed"], prof_mixed, mixed_v = replay(grid, rec, W, K, "mixed", profile=True)
+    forms["mixed+history"], prof_hist, hist_v = replay(grid, rec, W, K, "mixed", warm="history", profile=True)
+    net_info = None
+    if not args.no_network:
+        net, net_info = train_warm_start(grid)
+        forms["mixed+network"], _, _ = replay(grid, rec, W, K, "mixed", warm=net)
 
-    plain, plain_prof, plain_res = run("fp64")
-    if args.precision == "mixed":
-        main_stats, prof, mixed_res = run("mixed")
-        rel_v = max(float(torch.linalg.vector_norm(a.velocity - b.velocity) / torch.linalg.vector_norm(b.velocity))
-                    for a, b in zip(mixed_res, plain_res))
-        rel_p = max(float(torch.linalg.vector_norm(a.pressure - b.pressure) / torch.linalg.vector_norm(b.pressure))
-                    for a, b in zip(mixed_res, plain_res))
-        main_stats["rel_l2_vs_fp64_velocity"] = rel_v
-        main_stats["rel_l2_vs_fp64_pressure"] = rel_p
-    else:
-        main_stats, prof = plain, plain_prof
+    def rel(a_list, b_list):
+        return max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)) for a, b in zip(a_list, b_list))
+
+    forms["mixed"]["rel_l2_vs_fp64_velocity"] = rel(mixed_v, plain_v)
+    forms["mixed+history"]["rel_l2_vs_fp64_velocity"] = rel(hist_v, plain_v)
+    best = max(("mixed", "mixed+history"), key=lambda k: forms[k]["stage_solves_per_s"])
+    main_stats = forms[best]
+    prof = prof_hist if best == "mixed+history" else prof_mixed
     gemm_s = prof["gemm_ms"] * 1e-3
     # Flops of the MFMA work the algorithm issues: 2MNK per product, times 3 for the
-    # split bf16x3 products of the mixed form (A_lo B_hi + A_hi B_lo + A_hi B_hi).
+    # split bf16x3 products (A_lo B_hi + A_hi B_lo + A_hi B_hi).
     achieved = prof["gemm_flops"] / gemm_s / 1e12
-    split = 3.0 if args.precision == "mixed" else 1.0
-    peak = PEAK_TFLOPS[args.precision]
+    split = 3.0
     value = main_stats["stage_solves_per_s"]
+    for k, f in forms.items():
+        f["speedup_vs_plain_fp64"] = f["stage_solves_per_s"] / plain["stage_solves_per_s"]
 
     line = {
         "metric": "stage solves/s at 1024^2 MAC, mixed+learned vs plain FP64 GPU, equal residual",
         "value": value,
         "unit": "stage solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": K,
+        "warmup": W,
         "ms_per_step": main_stats["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "bf16-gemm/f64-refine",
-        "data": "synthetic: first-stage sdirk3_mrsav pairs of 4 seeded stream-function fields (modes<=32)",
+        "dtype": "bf16-gemm/f64-refine",
+        "data": f"synthetic: {W}+{K} consecutive stage solves of a recorded sdirk3_mrsav run from the seeded "
+                f"stream-function field (modes<={args.modes}), after one start-up step",
         "config": {"workload": f"MAC {grid.nx}x{grid.ny} stage solve, 2 RHS, mass=1, viscosity={kappa:g} "
-                               f"(nu=1e-3, dt=2e-4, sdirk3_mrsav)", "precision": args.precision,
+                               f"(nu=1e-3, dt=2e-4, sdirk3_mrsav)", "form": best,
                    "bar": "residual<=1e-10 and divergence_inf<=1e-10 (absolute)"},
         "solve": main_stats,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": value / plain["stage_solves_per_s"],
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": measured_traffic(args.precision),
-                     "kernel": "k_gemm_f64" if args.precision == "fp64" else "k_gemm_bf16",
+        "forms": forms,
+        "warm_start_network": net_info,
+        "record_seconds": record_s,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS["mixed"], "traffic": measured_traffic("mixed"),
+                     "kernel": "k_gemm_bf16",
                      "launches": prof["gemm_launches"],
                      "avg_launch_us": 1e6 * gemm_s / max(prof["gemm_launches"], 1),
                      "flops_per_launch": prof["gemm_flops"] / max(prof["gemm_launches"], 1),
@@ -274,7 +338,8 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": prof["gemm_bytes"] / max(prof["gemm_launches"], 1)},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(grid, pairs_host[0], kappa, args.cpu_seconds)
+        f0, m0, v0 = rec[W]
+        line["cpu_baseline"] = cpu_baseline(grid, f0.cpu().numpy(), v0, args.cpu_seconds)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -126,16 +126,29 @@ int32_t mac_cnn_loss_grad(void* cnn, const float* x, const float* target, int32_
 /* One Adam step (beta 0.9/0.999, eps 1e-8) on that loss. */
 int32_t mac_cnn_train_step(void* cnn, const float* x, const float* target, int32_t n, int32_t H, int32_t W,
                            float lr, float* loss_out);
-/* Schur-CG starting pressure when the caller passes none: MAC_WARM_OFF (zero, as the
- * reference), MAC_WARM_NETWORK (the attached network's guess; alpha is the pressure /
- * right-hand-side scale it was trained with) or MAC_WARM_PREVIOUS (the previous solve's
- * pressures).  Either guess g is scaled by theta = (g, b)/(g, S g) before CG starts. */
+/* Starting pressure when the caller passes none:
+ *   MAC_WARM_OFF       zero, as the reference;
+ *   MAC_WARM_NETWORK   the attached network's correction of a base guess (the history
+ *                      extrapolation when a period is set, else the previous solve's
+ *                      pressures); alpha is the pressure / right-hand-side scale it was
+ *                      trained with;
+ *   MAC_WARM_PREVIOUS  the previous solve's pressures;
+ *   MAC_WARM_HISTORY   the same stage of the last three steps extrapolated in time,
+ *                      3 p[-P] - 3 p[-2P] + p[-3P] (2 p[-P] - p[-2P] or p[-P] earlier), P
+ *                      the stage solves per step (mac_stokes_set_warm_history); a MIXED
+ *                      handle extrapolates the velocities the same way.
+ * FP64 handles start the Schur CG from theta g, theta = (g, b)/(g, S g); MIXED handles
+ * take g as the initial outer iterate of the defect correction. */
 #define MAC_WARM_OFF 0
 #define MAC_WARM_NETWORK 1
 #define MAC_WARM_PREVIOUS 2
+#define MAC_WARM_HISTORY 3
 int32_t mac_stokes_set_warm_start(void* handle, int32_t mode, void* cnn, double alpha);
 /* Forget the previous-stage pressures (a new run). */
 int32_t mac_stokes_reset_history(void* handle);
+/* Stage solves per step for MAC_WARM_HISTORY and the network's base (0..16; 0 = off);
+ * forgets the recorded history.  sdirk3_mrsav: 4, sdirk2_mrsav: 2. */
+int32_t mac_stokes_set_warm_history(void* handle, int32_t period);
 /* Schur right-hand sides b of the last FP64 solve, ncols columns of np (device). */
 int32_t mac_stokes_last_schur_rhs(void* handle, double* out, int64_t ld, int32_t ncols);
 /* Network inputs / targets / scales exactly as the solver prepares them (device);

@@ -190,10 +190,8 @@ void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, 
 // CG scalar steps (one thread per column).
 void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s);
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s);
-// tol = max(rtol * sqrt(rr), floor_rel * ||b||) on active columns: a tolerance relative
-// to the current residual (used after a warm start inside the mixed solve's first sweep;
-// the floor keeps a near-exact guess from asking more than the bf16 operator can give).
-void cg_retol(CgState* st, const double* rr, double rtol, double floor_rel, int ncols, hipStream_t s);
+// out = 2 a - b, or 3 a - 3 b + c when c is given, over n elements.
+void extrapolate(const double* a, const double* b, const double* c, double* out, int64_t n, hipStream_t s);
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s);
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s);
 void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,

@@ -196,10 +196,12 @@ __global__ void k_cg_check(CgState* st, const double* rr, int* flags, int ncols)
   flags[c] = st[c].active;
 }
 
-__global__ void k_cg_retol(CgState* st, const double* rr, double rtol, double floor_rel, int ncols) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols || !st[c].active) return;
-  st[c].tol = fmax(rtol * sqrt(rr[c]), floor_rel * st[c].bnorm);
+// Polynomial extrapolation in time from equally spaced past values (newest first):
+// out = 2 a - b (linear), or 3 a - 3 b + c (quadratic) when c is given.
+__global__ void k_extrapolate(const double* __restrict__ a, const double* __restrict__ b,
+                              const double* __restrict__ c, double* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    out[i] = c != nullptr ? 3.0 * (a[i] - b[i]) + c[i] : 2.0 * a[i] - b[i];
 }
 
 __global__ void k_cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox* mb, int seq) {
@@ -454,8 +456,8 @@ void cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox
 void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s) {
   k_post_values<<<1, 128, 0, s>>>(a, na, b, nb, mb, seq);
 }
-void cg_retol(CgState* st, const double* rr, double rtol, double floor_rel, int ncols, hipStream_t s) {
-  k_cg_retol<<<(ncols + 63) / 64, 64, 0, s>>>(st, rr, rtol, floor_rel, ncols);
+void extrapolate(const double* a, const double* b, const double* c, double* out, int64_t n, hipStream_t s) {
+  k_extrapolate<<<reduce_blocks(n), kThreads, 0, s>>>(a, b, c, out, n);
 }
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s) { k_cg_rho<<<(ncols + 63) / 64, 64, 0, s>>>(st, rz, ncols); }
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s) {

@@ -97,6 +97,8 @@ class Solver {
     for (hipEvent_t e : all_events_) (void)hipEventDestroy(e);
     for (void* p : allocs_) (void)hipFree(p);
     if (host_flags_) (void)hipHostFree(host_flags_);
+    if (hist_) (void)hipFree(hist_);
+    if (hist_u_) (void)hipFree(hist_u_);
     if (mailbox_) (void)hipHostFree(mailbox_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -365,7 +367,18 @@ class Solver {
     cnn_ = cnn;
     cnn_alpha_ = alpha;
   }
-  void reset_history() { prev_ncols_ = 0; }
+  void reset_history() {
+    prev_ncols_ = 0;
+    hist_count_ = 0;
+  }
+  // Same-slot history: with a scheme of `period` stage solves per step, the solve `period`
+  // back is the same stage of the previous step; the guess extrapolates polynomially in
+  // time over the last three steps, 3 p[-P] - 3 p[-2P] + p[-3P] (2 p[-P] - p[-2P] after
+  // two steps, p[-P] after one).
+  void set_history(int period) {
+    history_period_ = period;
+    reset_history();
+  }
   const double* schur_rhs() const { return b_; }
   int last_ncols() const { return ncols_cur_; }
 
@@ -373,21 +386,72 @@ class Solver {
   // the previous solve's pressures (same column count), if a network is attached and
   // the grid is one it serves.
   bool learned_guess(const double* b, int ncols) {
-    if (warm_mode_ == 2) {
+    if (warm_mode_ == MAC_WARM_PREVIOUS) {
       // Previous solve's pressures as the guess (same column count only).
       if (prev_ncols_ != ncols) return false;
       copy_cols(pprev_, np_, io_g_, np_, np_, ncols, stream_);
       return true;
     }
-    if (warm_mode_ != 1 || cnn_ == nullptr || !warm_shape_ok(g_.nx, g_.ny)) return false;
-    const double* prev = prev_ncols_ == ncols ? pprev_ : nullptr;
-    warm_guess(cnn_, b, np_, prev, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, io_g_, np_,
+    if (warm_mode_ == MAC_WARM_HISTORY) return history_guess(ncols, io_g_);
+    if (warm_mode_ != MAC_WARM_NETWORK || cnn_ == nullptr || !warm_shape_ok(g_.nx, g_.ny)) return false;
+    // The network corrects a base guess: the history extrapolation when a period is set
+    // (and enough solves are recorded), else the previous solve's pressures, else zero.
+    const double* base = nullptr;
+    if (history_period_ > 0) {
+      if (history_guess(ncols, warm_base_)) base = warm_base_;
+    } else if (prev_ncols_ == ncols) {
+      base = pprev_;
+    }
+    warm_guess(cnn_, b, np_, base, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, io_g_, np_,
                stream_);
     return true;
   }
-  void remember_pressure(const double* p, int ncols) {
+  bool history_guess(int ncols, double* out) {
+    if (history_period_ <= 0 || hist_ncols_ != ncols) return false;
+    return extrapolate_history(hist_, np_ * ncols, out);
+  }
+  // out = the same-slot polynomial extrapolation from ring `ring` (elements per slot n).
+  bool extrapolate_history(const double* ring, int64_t n, double* out) {
+    const int P = history_period_;
+    const int64_t steps = hist_count_ / P >= 3 ? 3 : hist_count_ / P;
+    if (steps == 0) return false;
+    const int slots = kHistSteps * P;
+    auto slot = [&](int back) { return ring + ((hist_count_ - (int64_t)back * P) % slots) * n; };
+    if (steps == 1) copy_cols(slot(1), n, out, n, n, 1, stream_);
+    else extrapolate(slot(1), slot(2), steps == 3 ? slot(3) : nullptr, out, n, stream_);
+    return true;
+  }
+  // Records the solve's pressures (and, for the mixed solve, velocities: its outer
+  // iterate starts from both) for the warm starts.
+  void remember_pressure(const double* p, int ncols, const double* u = nullptr) {
     copy_cols(p, np_, pprev_, np_, np_, ncols, stream_);
     prev_ncols_ = ncols;
+    if (history_period_ > 0) {
+      const int slots = kHistSteps * history_period_;
+      const bool with_u = u != nullptr;
+      if (hist_ncols_ != ncols || hist_slots_ < slots || hist_has_u_ != with_u) {
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        if (hist_) (void)hipFree(hist_);
+        if (hist_u_) (void)hipFree(hist_u_);
+        hist_ = hist_u_ = nullptr;
+        HIP_CHECK(hipMalloc(&hist_, sizeof(double) * np_ * ncols * slots));
+        if (with_u) HIP_CHECK(hipMalloc(&hist_u_, sizeof(double) * size_ * ncols * slots));
+        hist_ncols_ = ncols;
+        hist_slots_ = slots;
+        hist_has_u_ = with_u;
+        hist_count_ = 0;
+      }
+      const int slot = (int)(hist_count_ % slots);
+      copy_cols(p, np_, hist_ + (int64_t)slot * np_ * ncols, np_, np_, ncols, stream_);
+      if (with_u) copy_cols(u, size_, hist_u_ + (int64_t)slot * size_ * ncols, size_, size_, ncols, stream_);
+      ++hist_count_;
+    }
+  }
+  // The history extrapolation of the velocities (mixed solve), into out; false when
+  // there is none.
+  bool history_velocity(int ncols, double* out) {
+    if (history_period_ <= 0 || !hist_has_u_ || hist_ncols_ != ncols) return false;
+    return extrapolate_history(hist_u_, size_ * ncols, out);
   }
 
   Outcome solve(const double* f, int64_t f_stride, int ncols, double mass, double visc, double* vel,
@@ -456,8 +520,38 @@ class Solver {
       fc = fcomp_;
     }
     HIP_CHECK(hipMemsetAsync(uacc_, 0, sizeof(double) * size_ * ncols, stream_));
-    if (guess) copy_cols(guess, guess_stride, pacc_, np_, np_, ncols, stream_);
-    else HIP_CHECK(hipMemsetAsync(pacc_, 0, sizeof(double) * np_ * ncols, stream_));
+    // Starting pressure: the caller's guess, else the warm start's (history, previous
+    // solve or network), else zero.  It is the outer iterate, so the first FP64 defect
+    // measures its true error and every sweep improves on it; started inside a bf16 inner
+    // solve instead, its accuracy below the bf16 operator's would be lost.
+    out.warm = false;
+    if (guess) {
+      copy_cols(guess, guess_stride, pacc_, np_, np_, ncols, stream_);
+    } else if (warm_mode_ != MAC_WARM_OFF && warm_mode_ != MAC_WARM_NETWORK && learned_guess(nullptr, ncols)) {
+      copy_cols(io_g_, np_, pacc_, np_, np_, ncols, stream_);
+      // With a history, the velocities start from their own extrapolation too: a good
+      // pressure with u = 0 leaves an O(1) velocity defect, whose bf16 correction (after
+      // D) would swamp the pressure's accuracy.
+      if (warm_mode_ == MAC_WARM_HISTORY) history_velocity(ncols, uacc_);
+      out.warm = true;
+    } else {
+      HIP_CHECK(hipMemsetAsync(pacc_, 0, sizeof(double) * np_ * ncols, stream_));
+    }
+    if (guess == nullptr && !out.warm && warm_mode_ == MAC_WARM_NETWORK) {
+      // The network reads the Schur right-hand side of the zero start, b = -D L~^-1 f
+      // (mean removed): one bf16 velocity inverse before the sweeps.
+      mixed_ = true;
+      velocity_inverse_bf16(fc, DT_F64, packed_, du_, DT_F32, size_, all, mass, visc);
+      divergence(DT_F32, du_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, nullptr,
+                 fin(sums_));
+      sub_mean_dot(b_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
+      mixed_ = false;
+      if (learned_guess(b_, ncols)) {
+        copy_cols(io_g_, np_, pacc_, np_, np_, ncols, stream_);
+        if (history_period_ > 0) history_velocity(ncols, uacc_);
+        out.warm = true;
+      }
+    }
     const int nb = defect_blocks(size_, ncols);
     std::vector<int> inner(ncols);
     for (int sweep = 0;; ++sweep) {
@@ -487,14 +581,6 @@ class Solver {
       copy_cols(b_, np_, r_, np_, np_, ncols, stream_);
       HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
       accumulate_ = true;
-      if (sweep == 0 && guess == nullptr && learned_guess(b_, ncols)) {
-        // First sweep: the learned (or previous-stage) pressure starts the inner Schur CG
-        // (p was zero), and the inner tolerance becomes relative to the warm-started
-        // residual, so a good guess carries its accuracy into the outer iterate.
-        out.warm = true;
-        start_from(io_g_, np_, ncols, mass, visc, true);
-        cg_retol(cg_, rr_, opts_.inner_tolerance, 1e-6, ncols, stream_);
-      }
       pcg(ncols, mass, visc, inner, false);
       accumulate_ = false;
       for (int c = 0; c < ncols; ++c) out.iterations[c] += inner[c];
@@ -509,7 +595,7 @@ class Solver {
     sub_mean_dot(pacc_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
     copy_cols(pacc_, np_, pres, pres_stride, np_, ncols, stream_);
     copy_cols(uacc_, size_, vel, vel_stride, size_, ncols, stream_);
-    remember_pressure(pacc_, ncols);
+    remember_pressure(pacc_, ncols, uacc_);
     return out;
   }
 
@@ -772,6 +858,7 @@ class Solver {
     q_ = (double*)take(pb);
     fcomp_ = (double*)take(vb);
     pprev_ = (double*)take(pb);
+    warm_base_ = (double*)take(pb);
     warm_x_ = (float*)take(sizeof(float) * 2 * (np_ / 16 + 64) * ncols);
     warm_y_ = (float*)take(sizeof(float) * (np_ / 16 + 64) * ncols);
     warm_s_ = (double*)take(sizeof(double) * (ncols + 4));
@@ -835,6 +922,13 @@ class Solver {
   void* cnn_ = nullptr;
   double cnn_alpha_ = 1.0;
   int warm_mode_ = 0;
+  int history_period_ = 0, hist_ncols_ = 0, hist_slots_ = 0;
+  int64_t hist_count_ = 0;
+  static constexpr int kHistSteps = 3;  // steps kept per stage slot
+  double* hist_ = nullptr;      // ring of 3*period recorded pressures (np x ncols each)
+  double* hist_u_ = nullptr;    // and velocities (size x ncols each; mixed solves)
+  bool hist_has_u_ = false;
+  double* warm_base_ = nullptr;
   double* pprev_ = nullptr;
   int prev_ncols_ = 0;
   float *warm_x_ = nullptr, *warm_y_ = nullptr;
@@ -1033,9 +1127,15 @@ int32_t mac_stokes_get_profile(void* h, mac_kernel_profile* out) {
   return guarded(s, [&] { s->profile(&out->gemm_ms, &out->gemm_flops, &out->gemm_bytes, &out->gemm_launches); });
 }
 
+int32_t mac_stokes_set_warm_history(void* h, int32_t period) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s || period < 0 || period > 16) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] { s->set_history(period); });
+}
+
 int32_t mac_stokes_set_warm_start(void* h, int32_t mode, void* cnn, double alpha) {
   Solver* s = static_cast<Solver*>(h);
-  if (!s || mode < MAC_WARM_OFF || mode > MAC_WARM_PREVIOUS || (mode == MAC_WARM_NETWORK && !cnn) ||
+  if (!s || mode < MAC_WARM_OFF || mode > MAC_WARM_HISTORY || (mode == MAC_WARM_NETWORK && !cnn) ||
       !(alpha > 0) || !is_finite(alpha))
     return MAC_INVALID_ARGUMENT;
   return guarded(s, [&] { s->set_warm_start(mode, cnn, alpha); });

@@ -21,6 +21,7 @@ MAC_HIP_ERROR = 4
 MAC_WARM_OFF = 0
 MAC_WARM_NETWORK = 1
 MAC_WARM_PREVIOUS = 2
+MAC_WARM_HISTORY = 3
 
 MAC_PRECISION_FP64 = 0
 MAC_PRECISION_MIXED = 1
@@ -47,6 +48,7 @@ EXPORTED = (
     "mac_cnn_train_step",
     "mac_stokes_set_warm_start",
     "mac_stokes_reset_history",
+    "mac_stokes_set_warm_history",
     "mac_stokes_last_schur_rhs",
     "mac_warm_prepare",
     "mac_apply_divergence",
@@ -160,6 +162,8 @@ def load() -> ctypes.CDLL:
     lib.mac_stokes_set_warm_start.restype = _I32
     lib.mac_stokes_reset_history.argtypes = [_P]
     lib.mac_stokes_reset_history.restype = _I32
+    lib.mac_stokes_set_warm_history.argtypes = [_P, _I32]
+    lib.mac_stokes_set_warm_history.restype = _I32
     lib.mac_stokes_last_schur_rhs.argtypes = [_P, _P, _I64, _I32]
     lib.mac_stokes_last_schur_rhs.restype = _I32
     lib.mac_warm_prepare.argtypes = [_I32, _I32, _P, _I64, _P, _I64, _P, _I64, _I32, _D, _P, _P, _P]

@@ -2,14 +2,15 @@
 
 `WarmStartNet` wraps the device network of libmacstokes.so (forward, backward and Adam
 written as gfx950 kernels).  It maps the Schur right-hand side b of a stage column and
-that column's pressure at the previous stage, both average-pooled by 4 and normalised
-by rms(pool4 b), to the change pool4(p - p_prev) / (alpha rms(pool4 b)); the solver
-adds the bilinearly upsampled output to p_prev, scales the sum by the Galerkin step
-(g, b)/(g, S g) and starts the CG from it.  The CG still runs to the same tolerance and is
-the judge: a poor guess costs iterations, never accuracy.
+a base guess of that column's pressure, both average-pooled by 4 and normalised by
+rms(pool4 b), to the correction pool4(p - base) / (alpha rms(pool4 b)); the solver adds
+the bilinearly upsampled output to the base.  The base is the same stage of the last two
+steps extrapolated in time when the solver has a history period, else the previous
+solve's pressure.  The solve still runs to the same tolerance and is the judge: a poor
+guess costs iterations, never accuracy.
 
 Training pairs come from the plain FP64 solver (`collect_pairs`): for every stage
-column of a run, (b, previous pressure, pressure).  No counterpart in the reference.
+column of a run, (b, base guess, pressure).  No counterpart in the reference.
 """
 from __future__ import annotations
 
@@ -120,33 +121,50 @@ def prepare(grid, b: Any, pprev: Any | None, target: Any | None, alpha: float) -
 
 @dataclass
 class Pairs:
-    """Stage columns recorded from plain FP64 solves: b, previous pressure, pressure."""
+    """Stage columns recorded from plain FP64 solves: b, base guess, pressure."""
     b: list
-    pprev: list
+    base: list
     p: list
 
 
-def collect_pairs(model, scheme, initial, steps: int, dt: float) -> Pairs:
+def base_guess(history: list, period: int | None) -> Any:
+    """The solver's base guess from the recorded pressures (oldest first) of solves with
+    the same column count: 3 p[-P] - 3 p[-2P] + p[-3P], 2 p[-P] - p[-2P] or p[-P] as
+    steps are recorded, zero before (P = period), or the previous pressure when there is
+    no period."""
+    if period:
+        steps = min(len(history) // period, 3)
+        if steps == 3:
+            return 3.0 * (history[-period] - history[-2 * period]) + history[-3 * period]
+        if steps == 2:
+            return 2.0 * history[-period] - history[-2 * period]
+        return history[-period] if steps == 1 else None
+    return history[-1] if history else None
+
+
+def collect_pairs(model, scheme, initial, steps: int, dt: float, period: int | None = None) -> Pairs:
     """Run `scheme` on `model` (whose backend must be an FP64 GpuTensorStokes) and record,
-    for every stage solve, (b, previous-stage pressure, pressure) per column as device
-    tensors of shape (np, ncols)."""
+    for every stage solve, (b, base guess, pressure) per column as device tensors of
+    shape (np, ncols); `period` as in `GpuTensorStokes.set_warm_start`."""
     import torch
     backend = model.backend
     backend.reset_history()
     pairs = Pairs([], [], [])
     original = backend.solve
-    prev = {}
+    history: dict[int, list] = {}
 
     def recording(rhs, *, mass, viscosity, pressure_guess=None):
         res = original(rhs, mass=mass, viscosity=viscosity, pressure_guess=pressure_guess)
         k = 1 if np.ndim(rhs) == 1 else np.shape(rhs)[1]
         b = backend.last_schur_rhs(k)
         p = torch.as_tensor(np.asarray(res.pressure).reshape(model.grid.np, k), device=b.device)
-        pp = prev.get(k)
+        past = history.setdefault(k, [])
+        base = base_guess(past, period)
         pairs.b.append(b.clone())
-        pairs.pprev.append(torch.zeros_like(p) if pp is None else pp)
+        pairs.base.append(torch.zeros_like(p) if base is None else base)
         pairs.p.append(p.clone())
-        prev[k] = p.clone()
+        past.append(p.clone())
+        del past[:-3 * max(period or 1, 1)]
         return res
 
     backend.solve = recording
@@ -160,9 +178,9 @@ def collect_pairs(model, scheme, initial, steps: int, dt: float) -> Pairs:
 
 
 def fit_alpha(grid, pairs: Pairs) -> float:
-    """alpha = median over columns of rms(pool4 (p - p_prev)) / rms(pool4 b)."""
+    """alpha = median over columns of rms(pool4 (p - base)) / rms(pool4 b)."""
     ratios = []
-    for b, pp, p in zip(pairs.b, pairs.pprev, pairs.p):
+    for b, pp, p in zip(pairs.b, pairs.base, pairs.p):
         xb, _, sb = prepare(grid, b, None, None, 1.0)
         xp, _, sp = prepare(grid, p - pp, None, None, 1.0)
         ratios.extend((sp / sb).cpu().numpy().tolist())
@@ -175,7 +193,7 @@ def train(net: WarmStartNet, grid, pairs: Pairs, epochs: int = 40, lr: float = 1
     per-epoch mean loss."""
     import torch
     xs, ts = [], []
-    for b, pp, p in zip(pairs.b, pairs.pprev, pairs.p):
+    for b, pp, p in zip(pairs.b, pairs.base, pairs.p):
         x, t, _ = prepare(grid, b, pp, p - pp, net.alpha)
         xs.append(x)
         ts.append(t)

@@ -182,19 +182,28 @@ class GpuTensorStokes:
         return self.last_report
 
     # ---------------------------------------------------- learned warm start
-    def set_warm_start(self, net: Any | None) -> None:
-        """Schur-CG starting pressure when no `pressure_guess` is passed: a trained
-        `WarmStartNet`, the string "previous" (the previous solve's pressures), or None
-        (zero, as the reference).  The guess g is scaled by (g, b)/(g, S g) and starts the
-        Schur CG (FP64) or the first sweep's inner CG (mixed); the CG still runs to the
-        same tolerance, so results are unchanged to tolerance."""
+    def set_warm_start(self, net: Any | None, period: int | None = None) -> None:
+        """Starting pressure when no `pressure_guess` is passed: a trained `WarmStartNet`
+        (its correction of a base guess), "previous" (the previous solve's pressures),
+        "history" (the same stage of the last two steps extrapolated linearly, with
+        `period` stage solves per step), or None (zero, as the reference).  `period` also
+        makes the network's base the history extrapolation.  FP64 starts the Schur CG
+        from the Galerkin-scaled guess; mixed takes it as the initial outer iterate.
+        Either way the solve runs to the same tolerance, so results are unchanged to
+        tolerance."""
+        if period is not None:
+            _native.raise_for(self._lib.mac_stokes_set_warm_history(self._handle, int(period)),
+                              "period must be in 0..16")
+        elif net == "history":
+            raise ValueError('warm start "history" needs period= (stage solves per step)')
         self.warm_start = net
         if net is None:
             mode, handle, alpha = _native.MAC_WARM_OFF, None, 1.0
         elif isinstance(net, str):
-            if net != "previous":
-                raise ValueError('warm start must be a WarmStartNet, "previous" or None')
-            mode, handle, alpha = _native.MAC_WARM_PREVIOUS, None, 1.0
+            if net not in ("previous", "history"):
+                raise ValueError('warm start must be a WarmStartNet, "previous", "history" or None')
+            mode = _native.MAC_WARM_PREVIOUS if net == "previous" else _native.MAC_WARM_HISTORY
+            handle, alpha = None, 1.0
         else:
             mode, handle, alpha = _native.MAC_WARM_NETWORK, net.handle, float(net.alpha)
         code = self._lib.mac_stokes_set_warm_start(self._handle, mode, handle, alpha)

@@ -94,7 +94,7 @@ def test_trained_warm_start_saves_iterations_and_keeps_results():
         if pairs is None:
             pairs = got
         else:
-            pairs.b += got.b; pairs.pprev += got.pprev; pairs.p += got.p
+            pairs.b += got.b; pairs.base += got.base; pairs.p += got.p
     alpha = fit_alpha(grid, pairs)
     net = WarmStartNet(width=32, seed=0, alpha=alpha)
     hist = train(net, grid, pairs, epochs=30, lr=1e-3, batch=8)
