@@ -151,8 +151,10 @@ int32_t mac_stokes_reset_history(void* handle);
 int32_t mac_stokes_set_warm_history(void* handle, int32_t period);
 /* Schur right-hand sides b of the last FP64 solve, ncols columns of np (device). */
 int32_t mac_stokes_last_schur_rhs(void* handle, double* out, int64_t ld, int32_t ncols);
-/* Network inputs / targets / scales exactly as the solver prepares them (device);
- * `target` is the quantity to learn, p - p_prev. */
+/* Network inputs / targets / scales exactly as the solver prepares them (device):
+ * x[c] = [pool4(b)/s_c, pool4(base)/r_c], t[c] = pool4(target)/(alpha s_c), with s_c, r_c
+ * the rms of the pooled planes; `scale` receives s (ncols) then r (ncols).  `target` is
+ * the quantity to learn, p - base. */
 int32_t mac_warm_prepare(int32_t nx, int32_t ny, const double* b, int64_t ldb, const double* pprev, int64_t ldp,
                          const double* target, int64_t ldt, int32_t ncols, double alpha, float* x, float* t,
                          double* scale);

@@ -263,7 +263,8 @@ __global__ void k_zero_f(float* __restrict__ p, int64_t stride, int64_t n) {
   if (i < n) p[(int64_t)c * stride + i] = 0.f;
 }
 
-// s[c] = rms of x[c][0] (one block per column, fixed order).
+// s[c] = rms of the plane at x + c*stride (one block per column, fixed order; 1 for a
+// zero plane).
 __global__ __launch_bounds__(256) void k_rms(const float* __restrict__ x, int64_t stride, int64_t n,
                                              double* __restrict__ s) {
   __shared__ double sh[256];
@@ -285,15 +286,16 @@ __global__ __launch_bounds__(256) void k_rms(const float* __restrict__ x, int64_
   }
 }
 
-// Scale the network planes of column c: plane 0 by 1/s, plane 1 (and the target) by 1/(alpha s).
+// Scale the network planes of column c: plane 0 (b) by 1/s, plane 1 (the base guess) by
+// its own rms 1/s2, the target by 1/(alpha s).
 __global__ void k_normalize(float* __restrict__ x, float* __restrict__ t, int64_t n, const double* __restrict__ s,
-                            double alpha) {
+                            const double* __restrict__ s2, double alpha) {
   const int c = blockIdx.y;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float* xc = x + (int64_t)c * 2 * n;
   xc[i] = (float)(xc[i] / s[c]);
-  xc[n + i] = (float)(xc[n + i] / (alpha * s[c]));
+  xc[n + i] = (float)(xc[n + i] / s2[c]);
   if (t) t[(int64_t)c * n + i] = (float)(t[(int64_t)c * n + i] / (alpha * s[c]));
 }
 
@@ -572,7 +574,8 @@ void warm_prepare(const double* b, int64_t b_stride, const double* pprev, int64_
   else k_zero_f<<<grid, 256, 0, st>>>(x + n, 2 * n, n);
   if (target) k_pool4<<<grid, 256, 0, st>>>(target, t_stride, nx, ny, t, n);
   k_rms<<<ncols, 256, 0, st>>>(x, 2 * n, n, s);
-  k_normalize<<<grid, 256, 0, st>>>(x, target ? t : nullptr, n, s, alpha);
+  k_rms<<<ncols, 256, 0, st>>>(x + n, 2 * n, n, s + ncols);
+  k_normalize<<<grid, 256, 0, st>>>(x, target ? t : nullptr, n, s, s + ncols, alpha);
 }
 
 void warm_guess(void* cnn, const double* b, int64_t b_stride, const double* pprev, int64_t p_stride, int ncols,

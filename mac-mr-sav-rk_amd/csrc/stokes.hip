@@ -394,14 +394,16 @@ class Solver {
     }
     if (warm_mode_ == MAC_WARM_HISTORY) return history_guess(ncols, io_g_);
     if (warm_mode_ != MAC_WARM_NETWORK || cnn_ == nullptr || !warm_shape_ok(g_.nx, g_.ny)) return false;
-    // The network corrects a base guess: the history extrapolation when a period is set
-    // (and enough solves are recorded), else the previous solve's pressures, else zero.
+    // The network corrects a base guess: the history extrapolation when a period is set,
+    // else the previous solve's pressures.  Without a base there is no guess (zero start),
+    // as in training.
     const double* base = nullptr;
     if (history_period_ > 0) {
       if (history_guess(ncols, warm_base_)) base = warm_base_;
     } else if (prev_ncols_ == ncols) {
       base = pprev_;
     }
+    if (base == nullptr) return false;
     warm_guess(cnn_, b, np_, base, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, io_g_, np_,
                stream_);
     return true;
@@ -861,7 +863,7 @@ class Solver {
     warm_base_ = (double*)take(pb);
     warm_x_ = (float*)take(sizeof(float) * 2 * (np_ / 16 + 64) * ncols);
     warm_y_ = (float*)take(sizeof(float) * (np_ / 16 + 64) * ncols);
-    warm_s_ = (double*)take(sizeof(double) * (ncols + 4));
+    warm_s_ = (double*)take(sizeof(double) * (2 * ncols + 4));
     prev_ncols_ = 0;
     uacc_ = (double*)take(vb);
     ru_ = (double*)take(vb);

@@ -97,9 +97,9 @@ class WarmStartNet:
 
 
 def prepare(grid, b: Any, pprev: Any | None, target: Any | None, alpha: float) -> tuple[Any, Any, Any]:
-    """Network inputs (n, 2, ny/4, nx/4), targets (n, 1, ny/4, nx/4) and scales (n,) for
-    columns of b / pprev / target, each (np, n) device tensors, exactly as the solver
-    prepares them at inference."""
+    """Network inputs (n, 2, ny/4, nx/4), targets (n, 1, ny/4, nx/4) and the b scales
+    (n,) for columns of b / base / target, each (np, n) device tensors, exactly as the
+    solver prepares them at inference (the base plane is normalised by its own rms)."""
     import torch
     lib = _native.load()
     dev = b.device
@@ -110,13 +110,13 @@ def prepare(grid, b: Any, pprev: Any | None, target: Any | None, alpha: float) -
     h, w = grid.ny // 4, grid.nx // 4
     x = torch.empty((n, 2, h, w), dtype=torch.float32, device=dev)
     t = torch.empty((n, 1, h, w), dtype=torch.float32, device=dev) if target is not None else None
-    s = torch.empty(n, dtype=torch.float64, device=dev)
+    s = torch.empty(2 * n, dtype=torch.float64, device=dev)
     torch.cuda.current_stream(dev).synchronize()
     code = lib.mac_warm_prepare(grid.nx, grid.ny, bc.data_ptr(), grid.np, None if pc is None else pc.data_ptr(),
                                 grid.np, None if tc is None else tc.data_ptr(), grid.np, n, alpha, x.data_ptr(),
                                 None if t is None else t.data_ptr(), s.data_ptr())
     _native.raise_for(code, "mac_warm_prepare failed")
-    return x, t, s
+    return x, t, s[:n]
 
 
 @dataclass
@@ -144,8 +144,8 @@ def base_guess(history: list, period: int | None) -> Any:
 
 def collect_pairs(model, scheme, initial, steps: int, dt: float, period: int | None = None) -> Pairs:
     """Run `scheme` on `model` (whose backend must be an FP64 GpuTensorStokes) and record,
-    for every stage solve, (b, base guess, pressure) per column as device tensors of
-    shape (np, ncols); `period` as in `GpuTensorStokes.set_warm_start`."""
+    for every stage solve that has a base guess, (b, base guess, pressure) per column as
+    device tensors of shape (np, ncols); `period` as in `GpuTensorStokes.set_warm_start`."""
     import torch
     backend = model.backend
     backend.reset_history()
@@ -160,9 +160,11 @@ def collect_pairs(model, scheme, initial, steps: int, dt: float, period: int | N
         p = torch.as_tensor(np.asarray(res.pressure).reshape(model.grid.np, k), device=b.device)
         past = history.setdefault(k, [])
         base = base_guess(past, period)
-        pairs.b.append(b.clone())
-        pairs.base.append(torch.zeros_like(p) if base is None else base)
-        pairs.p.append(p.clone())
+        if base is not None:
+            # Solves without a base guess are not learned: the solver then starts from zero.
+            pairs.b.append(b.clone())
+            pairs.base.append(base)
+            pairs.p.append(p.clone())
         past.append(p.clone())
         del past[:-3 * max(period or 1, 1)]
         return res

@@ -65,8 +65,8 @@ def test_prepare_matches_pooling_and_rms():
         rms = np.sqrt(np.mean(pb**2))
         assert abs(s[c].item() - rms) <= 1e-6 * rms
         np.testing.assert_allclose(x[c, 0].cpu().numpy(), pb / rms, rtol=1e-5, atol=1e-6)
-        np.testing.assert_allclose(x[c, 1].cpu().numpy(), pool(pp[:, c].cpu().numpy().reshape(g.ny, g.nx)) / (0.3 * rms),
-                                   rtol=1e-5, atol=1e-6)
+        pbase = pool(pp[:, c].cpu().numpy().reshape(g.ny, g.nx))
+        np.testing.assert_allclose(x[c, 1].cpu().numpy(), pbase / np.sqrt(np.mean(pbase**2)), rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(t[c, 0].cpu().numpy(), pool(p[:, c].cpu().numpy().reshape(g.ny, g.nx)) / (0.3 * rms),
                                    rtol=1e-5, atol=1e-6)
 
@@ -83,6 +83,10 @@ def _run(backend, seed, steps, dt=2e-4, n=128, nu=1e-3, modes=8):
 
 
 def test_trained_warm_start_saves_iterations_and_keeps_results():
+    """Held-out seed, 6 steps of sdirk3_mrsav at 128^2 (24 two-RHS stage solves): Schur-CG
+    iterations and mixed-solve sweeps from the zero start (the reference), the previous
+    solve's pressure, the same-stage history extrapolation, and the network's correction
+    of that extrapolation (trained on 3 other seeds)."""
     from macsav import GpuTensorStokes, MACGrid
     from macsav.learned import WarmStartNet, collect_pairs, fit_alpha, train
     grid = MACGrid(128, 128)
@@ -90,32 +94,37 @@ def test_trained_warm_start_saves_iterations_and_keeps_results():
     pairs = None
     for seed in range(3):
         model, scheme, state = _run(plain, seed, 0)
-        got = collect_pairs(model, scheme, state, steps=3, dt=2e-4)
+        got = collect_pairs(model, scheme, state, steps=6, dt=2e-4, period=4)
         if pairs is None:
             pairs = got
         else:
             pairs.b += got.b; pairs.base += got.base; pairs.p += got.p
     alpha = fit_alpha(grid, pairs)
     net = WarmStartNet(width=32, seed=0, alpha=alpha)
-    hist = train(net, grid, pairs, epochs=30, lr=1e-3, batch=8)
+    hist = train(net, grid, pairs, epochs=60, lr=1e-3, batch=8)
     assert hist[-1] < hist[0]
 
-    def iterations(start):
-        s = GpuTensorStokes(grid, tolerance=1e-9)
-        s.set_warm_start(start)
+    def run(start, precision="fp64", period=None):
+        s = GpuTensorStokes(grid, tolerance=1e-9, precision=precision)
+        s.set_warm_start(start, period=period)
         model, scheme, state = _run(s, seed=11, steps=0)
-        for _ in range(3):
+        for _ in range(6):
             state = scheme.step(model, state, 2e-4).state
-        return s.total_iterations, model.vector(state), s.warm_starts
+        return s.total_iterations, s.sweeps, model.vector(state), s.warm_starts
 
-    base_its, base_u, _ = iterations(None)
-    prev_its, prev_u, _ = iterations("previous")
-    warm_its, warm_u, used = iterations(net)
-    print(f"held-out seed, 24 two-RHS stage solves: Schur-CG iterations zero / previous-stage / network "
-          f"guess: {base_its} / {prev_its} / {warm_its}")
+    zero_its, _, zero_u, _ = run(None)
+    prev_its, _, prev_u, _ = run("previous")
+    hist_its, _, hist_u, _ = run("history", period=4)
+    net_its, _, net_u, used = run(net, period=4)
+    _, zero_sw, mixed_u, _ = run(None, "mixed")
+    _, hist_sw, _, _ = run("history", "mixed", period=4)
+    _, net_sw, net_mixed_u, _ = run(net, "mixed", period=4)
+    print(f"held-out seed, 24 two-RHS stage solves: FP64 Schur-CG iterations zero / previous / history / "
+          f"network: {zero_its} / {prev_its} / {hist_its} / {net_its}; mixed sweeps zero / history / network: "
+          f"{zero_sw} / {hist_sw} / {net_sw}; training loss {hist[0]:.3g} -> {hist[-1]:.3g}")
     assert used > 0
-    # The Galerkin-scaled start is never worse in the S-norm; allow one iteration of slack
-    # per solve in the 2-norm stopping test.
-    assert warm_its <= base_its + 24
-    for u in (prev_u, warm_u):
-        assert np.linalg.norm(u - base_u) <= 1e-8 * np.linalg.norm(base_u)
+    # Warm starts change the work, never the results (to the solve tolerance).
+    for u in (prev_u, hist_u, net_u, mixed_u, net_mixed_u):
+        assert np.linalg.norm(u - zero_u) <= 1e-8 * np.linalg.norm(zero_u)
+    # The same-stage history saves work in both forms.
+    assert hist_its < zero_its and hist_sw < zero_sw
