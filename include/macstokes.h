@@ -176,6 +176,27 @@ int32_t mac_velocity_inverse(void* handle, const double* in, int64_t ld_in, int3
 int32_t mac_pressure_poisson_inverse(void* handle, const double* in, int64_t ld_in, int32_t ncols,
                                      double* out, int64_t ld_out);
 
+/* ---- device MAC model operations (the schemes' algebra between stage solves) ----
+ * Device pointers; kernels run on the null stream.  mac_dot and mac_max_abs synchronise
+ * and return a host scalar; the others return once enqueued.  Status codes as above;
+ * mac_model_last_error() holds the calling thread's last message. */
+const char* mac_model_last_error(void);
+/* N(u) per packed velocity column (kernels.py:6-35, via mac_ns.py:22-25); bitwise the
+ * reference loop. */
+int32_t mac_convection(int32_t nx, int32_t ny, double lx, double ly, const double* velocity, int64_t ld,
+                       int32_t ncols, double* out, int64_t ld_out);
+/* out = sum_k coefficients[k] * fields[k], accumulated left to right (mac_ns.py:40-45);
+ * nterms <= 8, fields[] a host array of device pointers. */
+int32_t mac_combine(int64_t n, int32_t nterms, const double* coefficients, const double* const* fields,
+                    double* out);
+/* K u, G p = -D^T p and D u as the reference's CSR products (operators.py:6-29). */
+int32_t mac_apply_K(int32_t nx, int32_t ny, double lx, double ly, const double* velocity, double* out);
+int32_t mac_grid_gradient(int32_t nx, int32_t ny, double lx, double ly, const double* pressure, double* out);
+int32_t mac_grid_divergence(int32_t nx, int32_t ny, double lx, double ly, const double* velocity, double* out);
+/* sum a*b (fixed-order two-level reduction) and max|a| (NaN reported as inf). */
+int32_t mac_dot(int64_t n, const double* a, const double* b, double* result);
+int32_t mac_max_abs(int64_t n, const double* a, double* result);
+
 /* ---- batched cavities ----
  * Replaces a host loop of `TensorStokes.solve` (tensor_stokes.py:245-364) over many
  * independent cavities, e.g. one `MACNavierStokes` per cavity stepped in lockstep: every

@@ -55,6 +55,14 @@ EXPORTED = (
     "mac_apply_divergence_transpose",
     "mac_velocity_inverse",
     "mac_pressure_poisson_inverse",
+    "mac_model_last_error",
+    "mac_convection",
+    "mac_combine",
+    "mac_apply_K",
+    "mac_grid_gradient",
+    "mac_grid_divergence",
+    "mac_dot",
+    "mac_max_abs",
     "mac_batch_create",
     "mac_batch_create_error",
     "mac_batch_destroy",
@@ -176,6 +184,19 @@ def load() -> ctypes.CDLL:
     lib.mac_velocity_inverse.restype = _I32
     lib.mac_pressure_poisson_inverse.argtypes = [_P, _P, _I64, _I32, _P, _I64]
     lib.mac_pressure_poisson_inverse.restype = _I32
+    lib.mac_model_last_error.argtypes = []
+    lib.mac_model_last_error.restype = ctypes.c_char_p
+    lib.mac_convection.argtypes = [_I32, _I32, _D, _D, _P, _I64, _I32, _P, _I64]
+    lib.mac_convection.restype = _I32
+    lib.mac_combine.argtypes = [_I64, _I32, _P, _P, _P]
+    lib.mac_combine.restype = _I32
+    for name in ("mac_apply_K", "mac_grid_gradient", "mac_grid_divergence"):
+        getattr(lib, name).argtypes = [_I32, _I32, _D, _D, _P, _P]
+        getattr(lib, name).restype = _I32
+    lib.mac_dot.argtypes = [_I64, _P, _P, ctypes.POINTER(_D)]
+    lib.mac_dot.restype = _I32
+    lib.mac_max_abs.argtypes = [_I64, _P, ctypes.POINTER(_D)]
+    lib.mac_max_abs.restype = _I32
     lib.mac_batch_create.argtypes = [_I32, _P, _P, _P, _P, _I32, ctypes.POINTER(StokesOpts)]
     lib.mac_batch_create.restype = _P
     lib.mac_batch_create_error.argtypes = []

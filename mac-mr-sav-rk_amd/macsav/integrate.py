@@ -77,7 +77,9 @@ def integrate(model: Model, scheme: Scheme, initial: State, steps: list[float] |
     for k, dt in enumerate(seq):
         try:
             trial = scheme.step(model, history.state, float(dt))
-            if not np.isfinite(model.vector(trial.state)).all() or not np.isfinite(trial.state.r):
+            finite = getattr(model, "all_finite", None)
+            field_ok = finite(model.vector(trial.state)) if finite else np.isfinite(model.vector(trial.state)).all()
+            if not field_ok or not np.isfinite(trial.state.r):
                 raise FloatingPointError("Nonfinite trial state")
             diag = model.diagnostics(trial.state)
             if not all(np.isfinite(v) for v in diag.values()):

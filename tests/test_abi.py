@@ -16,7 +16,7 @@ HEADER = Path(__file__).resolve().parents[1] / "include" / "macstokes.h"
 
 def declared_symbols():
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(mac_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(mac_[A-Za-z_0-9]+)\s*\(", text)))
 
 
 def test_header_and_binding_list_the_same_entry_points():
