@@ -296,17 +296,92 @@ def bench_run(args) -> dict:
             "forms": out, "speedup_vs_host_fp64": best["steps_per_s"] / out["host+fp64"]["steps_per_s"]}
 
 
+def bench_combined(args) -> dict:
+    """Combined workload at n^2 (default 2048^2): nu = 2e-4, the top wall moving at 1 plus
+    the seeded stream-function perturbation (modes <= 64, amplitude 0.05), sdirk3_mrsav
+    under PI step control (atol = rtol = 1e-6) to t = 0.05, fields resident on the
+    device.  The plain FP64 backend and the mixed + history backend run the same
+    problem; reported: accepted and attempted steps, stage solves/s, wall time and the
+    final-field difference between them."""
+    import numpy as np
+    import torch
+    from macsav import GpuTensorStokes, MACGrid
+    from macsav.adaptivity import PIController, integrate_adaptive
+    from macsav.fields import stream_function_velocity
+    from macsav.mac_ns_device import DeviceMACNavierStokes
+    from macsav.schemes import make_scheme
+    n = args.n or 2048
+    nu, lid, modes, t_end, first = 2e-4, 1.0, 64, args.t_end, 1e-4
+    grid = MACGrid(n, n)
+    x0 = stream_function_velocity(grid, seed=0, modes=modes, amplitude=0.05)
+    controller = PIController(atol=1e-6, rtol=1e-6, max_steps=args.max_steps)
+    out, finals = {}, {}
+    for form in ("device+fp64", "device+mixed+history"):
+        precision = "mixed" if "mixed" in form else "fp64"
+
+        def backend_for():
+            b = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14, precision=precision,
+                                refine_tolerance=1e-10)
+            if "history" in form:
+                b.set_warm_start("history", period=STAGES)
+            return b
+        # One untimed step on a throwaway backend first (code paging, first-touch memory).
+        scratch = DeviceMACNavierStokes(grid, nu, backend=backend_for(), lid=lid)
+        make_scheme("sdirk3_mrsav").step(scratch, scratch.state(0.0, x0), first)
+        del scratch
+        torch.cuda.synchronize()
+        backend = backend_for()
+        model = DeviceMACNavierStokes(grid, nu, backend=backend, lid=lid)
+
+        def progress(k, t, secs, form=form):
+            if k % 10 == 0:
+                print(f"[{form}] accepted {k} t={t:.5f} {secs:.1f}s", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        res = integrate_adaptive(model, make_scheme("sdirk3_mrsav"), model.state(0.0, x0), t_end, first,
+                                 controller=controller, progress=progress, retain_history=False)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        if res.status != "complete":
+            raise RuntimeError(f"{form}: {res.status} {res.error}")
+        finals[form] = model.vector(res.final).cpu().numpy()
+        out[form] = {"accepted_steps": res.accepted_steps, "attempted_steps": res.attempted_steps,
+                     "seconds": elapsed, "stage_solves": backend.solves,
+                     "stage_solves_per_s": backend.solves / elapsed,
+                     "schur_cg_iterations": backend.total_iterations,
+                     "final_t": float(res.final.t), "final_divergence_inf": res.diagnostics[-1]["divergence_inf"],
+                     "final_kinetic": res.diagnostics[-1]["kinetic"]}
+        del model, backend
+        torch.cuda.synchronize()
+    ref = finals["device+fp64"]
+    diff = finals["device+mixed+history"] - ref
+    best, plain = out["device+mixed+history"], out["device+fp64"]
+    return {"metric": f"stage solves/s, adaptive sdirk3_mrsav at {n}^2 MAC (lid + perturbation)",
+            "value": best["stage_solves_per_s"], "unit": "stage solves/s", "n_gpus": 1,
+            "steps": best["accepted_steps"], "warmup": 1, "ms_per_step": 1e3 * best["seconds"] / best["accepted_steps"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16-gemm/f64-refine",
+            "data": f"seeded stream-function perturbation (modes<={modes}, amplitude 0.05) on a cavity with lid "
+                    f"velocity {lid:g}",
+            "config": {"workload": f"MAC {n}x{n}, nu={nu:g}, lid {lid:g}, sdirk3_mrsav with PI control "
+                                   f"atol=rtol=1e-6 to t={t_end:g}, initial step {first:g}"},
+            "forms": out, "speedup_vs_plain_fp64": best["stage_solves_per_s"] / plain["stage_solves_per_s"],
+            "wall_speedup_vs_plain_fp64": plain["seconds"] / best["seconds"],
+            "final_rel_l2_vs_fp64": float(np.linalg.norm(diff) / np.linalg.norm(ref)),
+            "final_max_abs_vs_fp64": float(np.abs(diff).max())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed stage solves")
     ap.add_argument("--warmup", type=int, default=12, help="untimed stage solves before them (3 steps)")
-    ap.add_argument("--n", type=int, default=1024, help="grid size (n x n)")
+    ap.add_argument("--n", type=int, default=None, help="grid size (n x n); 1024, or 2048 for the combined run")
     ap.add_argument("--modes", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-network", action="store_true")
-    ap.add_argument("--workload", choices=["stage1024", "cavities", "run"], default="stage1024")
+    ap.add_argument("--workload", choices=["stage1024", "cavities", "run", "combined"], default="stage1024")
+    ap.add_argument("--t-end", type=float, default=0.05, help="combined run: final time")
+    ap.add_argument("--max-steps", type=int, default=2000, help="combined run: accepted-step cap")
     ap.add_argument("--cavities", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -317,6 +392,10 @@ def main() -> int:
     if args.workload == "cavities":
         print(json.dumps(bench_cavities(args)), flush=True)
         return 0
+    if args.workload == "combined":
+        print(json.dumps(bench_combined(args)), flush=True)
+        return 0
+    args.n = args.n or 1024
     if args.workload == "run":
         print(json.dumps(bench_run(args)), flush=True)
         return 0

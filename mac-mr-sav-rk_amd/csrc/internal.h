@@ -190,8 +190,9 @@ void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, 
 // CG scalar steps (one thread per column).
 void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s);
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s);
-// out = 2 a - b, or 3 a - 3 b + c when c is given, over n elements.
-void extrapolate(const double* a, const double* b, const double* c, double* out, int64_t n, hipStream_t s);
+// out = w[0] a + w[1] b (+ w[2] c when c is given), over n elements.
+void extrapolate(const double* a, const double* b, const double* c, const double w[3], double* out, int64_t n,
+                 hipStream_t s);
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s);
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s);
 void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,

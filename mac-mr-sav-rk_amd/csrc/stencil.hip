@@ -199,9 +199,10 @@ __global__ void k_cg_check(CgState* st, const double* rr, int* flags, int ncols)
 // Polynomial extrapolation in time from equally spaced past values (newest first):
 // out = 2 a - b (linear), or 3 a - 3 b + c (quadratic) when c is given.
 __global__ void k_extrapolate(const double* __restrict__ a, const double* __restrict__ b,
-                              const double* __restrict__ c, double* __restrict__ out, int64_t n) {
+                              const double* __restrict__ c, double wa, double wb, double wc,
+                              double* __restrict__ out, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-    out[i] = c != nullptr ? 3.0 * (a[i] - b[i]) + c[i] : 2.0 * a[i] - b[i];
+    out[i] = c != nullptr ? wa * a[i] + wb * b[i] + wc * c[i] : wa * a[i] + wb * b[i];
 }
 
 __global__ void k_cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox* mb, int seq) {
@@ -456,8 +457,9 @@ void cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox
 void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s) {
   k_post_values<<<1, 128, 0, s>>>(a, na, b, nb, mb, seq);
 }
-void extrapolate(const double* a, const double* b, const double* c, double* out, int64_t n, hipStream_t s) {
-  k_extrapolate<<<reduce_blocks(n), kThreads, 0, s>>>(a, b, c, out, n);
+void extrapolate(const double* a, const double* b, const double* c, const double w[3], double* out, int64_t n,
+                 hipStream_t s) {
+  k_extrapolate<<<reduce_blocks(n), kThreads, 0, s>>>(a, b, c, w[0], w[1], w[2], out, n);
 }
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s) { k_cg_rho<<<(ncols + 63) / 64, 64, 0, s>>>(st, rz, ncols); }
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s) {

@@ -373,8 +373,11 @@ class Solver {
   }
   // Same-slot history: with a scheme of `period` stage solves per step, the solve `period`
   // back is the same stage of the previous step; the guess extrapolates polynomially in
-  // time over the last three steps, 3 p[-P] - 3 p[-2P] + p[-3P] (2 p[-P] - p[-2P] after
-  // two steps, p[-P] after one).
+  // time over the last three steps (two after two steps, the last one after one).  The
+  // step sizes come from the solves themselves: a stage's viscosity/mass is nu a_ii h,
+  // so same-slot ratios are step ratios.  Consecutive same-slot solves are taken
+  // (h_m + h_m-1) / 2 apart (the stage at mid-step); the weights are the Lagrange ones
+  // on those nodes, 3 p[-P] - 3 p[-2P] + p[-3P] for equal steps.
   void set_history(int period) {
     history_period_ = period;
     reset_history();
@@ -418,10 +421,44 @@ class Solver {
     const int64_t steps = hist_count_ / P >= 3 ? 3 : hist_count_ / P;
     if (steps == 0) return false;
     const int slots = kHistSteps * P;
-    auto slot = [&](int back) { return ring + ((hist_count_ - (int64_t)back * P) % slots) * n; };
-    if (steps == 1) copy_cols(slot(1), n, out, n, n, 1, stream_);
-    else extrapolate(slot(1), slot(2), steps == 3 ? slot(3) : nullptr, out, n, stream_);
+    auto index = [&](int back) { return (int)((hist_count_ - (int64_t)back * P) % slots); };
+    auto slot = [&](int back) { return ring + index(back) * n; };
+    if (steps == 1) {
+      copy_cols(slot(1), n, out, n, n, 1, stream_);
+      return true;
+    }
+    double w[3];
+    history_weights((int)steps, index, w);
+    extrapolate(slot(1), slot(2), steps == 3 ? slot(3) : nullptr, w, out, n, stream_);
     return true;
+  }
+  // Lagrange weights at the new solve's time for the last `steps` same-slot solves.
+  template <class Index>
+  void history_weights(int steps, Index index, double w[3]) const {
+    double h[4] = {shift_cur_, 0.0, 0.0, 0.0};
+    for (int k = 1; k <= steps; ++k) h[k] = hist_shift_[index(k)];
+    bool equal = true, known = std::isfinite(h[0]) && h[0] > 0.0;
+    for (int k = 1; k <= steps; ++k) {
+      equal = equal && h[k] == h[0];
+      known = known && std::isfinite(h[k]) && h[k] > 0.0;
+    }
+    if (equal || !known) {  // equal steps (or no step information): the fixed weights
+      w[0] = steps == 3 ? 3.0 : 2.0;
+      w[1] = steps == 3 ? -3.0 : -1.0;
+      w[2] = steps == 3 ? 1.0 : 0.0;
+      return;
+    }
+    double x[4] = {0.0, 0.0, 0.0, 0.0};  // node times relative to the new solve
+    for (int k = 1; k <= steps; ++k) x[k] = x[k - 1] - 0.5 * (h[k - 1] + h[k]);
+    for (int k = 1; k <= 3; ++k) {
+      double num = 1.0, den = 1.0;
+      for (int m = 1; m <= steps; ++m) {
+        if (m == k) continue;
+        num *= -x[m];
+        den *= x[k] - x[m];
+      }
+      w[k - 1] = k <= steps ? num / den : 0.0;
+    }
   }
   // Records the solve's pressures (and, for the mixed solve, velocities: its outer
   // iterate starts from both) for the warm starts.
@@ -444,6 +481,7 @@ class Solver {
         hist_count_ = 0;
       }
       const int slot = (int)(hist_count_ % slots);
+      hist_shift_[slot] = shift_cur_;
       copy_cols(p, np_, hist_ + (int64_t)slot * np_ * ncols, np_, np_, ncols, stream_);
       if (with_u) copy_cols(u, size_, hist_u_ + (int64_t)slot * size_ * ncols, size_, size_, ncols, stream_);
       ++hist_count_;
@@ -463,6 +501,7 @@ class Solver {
     mixed_ = false;
     ensure_capacity(ncols);
     ncols_cur_ = ncols;
+    shift_cur_ = mass > 0.0 ? visc / mass : 0.0;
     std::vector<int> all(ncols);
     for (int c = 0; c < ncols; ++c) all[c] = c;
     Outcome out;
@@ -511,6 +550,7 @@ class Solver {
                       int64_t guess_stride) {
     ensure_capacity(ncols);
     ncols_cur_ = ncols;
+    shift_cur_ = mass > 0.0 ? visc / mass : 0.0;
     std::vector<int> all(ncols);
     for (int c = 0; c < ncols; ++c) all[c] = c;
     Outcome out;
@@ -927,6 +967,8 @@ class Solver {
   int history_period_ = 0, hist_ncols_ = 0, hist_slots_ = 0;
   int64_t hist_count_ = 0;
   static constexpr int kHistSteps = 3;  // steps kept per stage slot
+  double shift_cur_ = 0.0;                     // viscosity / mass of the solve in progress
+  double hist_shift_[kHistSteps * 16] = {};    // and of each recorded slot (period <= 16)
   double* hist_ = nullptr;      // ring of 3*period recorded pressures (np x ncols each)
   double* hist_u_ = nullptr;    // and velocities (size x ncols each; mixed solves)
   bool hist_has_u_ = false;

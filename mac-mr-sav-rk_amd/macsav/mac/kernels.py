@@ -3,7 +3,10 @@
 Same discrete operator as the reference's Numba kernel (solver/mac/kernels.py:6-35):
 for every interior face, centred transport velocities on the dual cell, the
 conservative flux difference, minus half the dual-cell divergence times the local
-velocity, with odd ghosts (-u) across the tangential walls.  Written here over whole
+velocity, with odd ghosts (-u) across the tangential walls.  The transport velocity
+through a wall is its zero normal velocity, so the advective flux there vanishes for any
+ghost value: a moving lid changes nothing here and enters only through `lid_load`.
+Written here over whole
 face arrays with explicit ghost rows/columns; the operation order per face is the
 loop's, so results are bitwise those of the loop.
 """
@@ -46,3 +49,16 @@ def packed_convection(grid, velocity: Array) -> Array:
     u, v = grid.unpack(velocity)
     cu, cv = convection(u, v, grid.hx, grid.hy)
     return grid.pack(cu, cv)
+
+
+def lid_load(grid, lid: float) -> Array:
+    """The viscous boundary load per unit viscosity of a top wall moving tangentially at
+    `lid` (no counterpart in the reference's MAC model, which has no-slip walls only).
+    The wall value lid is imposed by the ghost 2 lid - u above the top row of u faces,
+    so there -Laplacian_ghost u = K u - 2 lid / hy^2: the load is 2 lid / hy^2 on those
+    faces and zero elsewhere."""
+    load = np.zeros(grid.size)
+    if lid != 0.0:
+        top = (grid.ny - 1) * (grid.nx - 1)
+        load[top:top + grid.nx - 1] = 2.0 * lid / grid.hy ** 2
+    return load

@@ -14,7 +14,7 @@ import numpy as np
 
 from .core import Stage, State
 from .mac.grid import Array, MACGrid
-from .mac.kernels import convection
+from .mac.kernels import convection, lid_load
 from .mac.operators import MACOperators
 from .mac.stokes import StokesBackend, StokesResult
 from .model import SolveResult
@@ -22,18 +22,28 @@ from .model import SolveResult
 
 class MACNavierStokes:
     def __init__(self, grid: MACGrid, nu: float, force: Callable[[float], Array] | None = None,
-                 backend: StokesBackend | None = None, cache_size: int = 4) -> None:
+                 backend: StokesBackend | None = None, cache_size: int = 4, lid: float = 0.0) -> None:
         if not np.isfinite(nu) or nu <= 0:
             raise ValueError("nu must be positive and finite")
+        if not np.isfinite(lid):
+            raise ValueError("lid must be finite")
         self.grid = grid
         self.nu = nu
+        self.lid = float(lid)
         self.ops = MACOperators(grid)
         if backend is None:
             from .mac.gpu_stokes import GpuTensorStokes
             backend = GpuTensorStokes(self.ops)
         self.backend = backend
         self.cache_size = cache_size
-        self.force = force if force is not None else (lambda t: np.zeros(grid.size))
+        body = force if force is not None else (lambda t: np.zeros(grid.size))
+        if self.lid != 0.0:
+            # A moving top wall enters as its viscous boundary load, nu * 2 lid / hy^2 on
+            # the top row of u faces, added to the body force.
+            wall = nu * lid_load(grid, self.lid)
+            self.force = lambda t: body(t) + wall
+        else:
+            self.force = body
 
     # ------------------------------------------------------------- fields
     def nonlinear(self, velocity: Array) -> Array:
@@ -42,7 +52,9 @@ class MACNavierStokes:
         return self.grid.pack(cu, cv)
 
     def nonlinear_with_lifting(self, velocity: Any) -> tuple[Any, float]:
-        # No boundary lifting on the no-slip MAC box: zero pairing.
+        # No boundary lifting pairing on the MAC box: zero pairing, as the reference's MAC
+        # model.  A moving lid adds no advective flux (the wall's normal velocity is zero);
+        # its viscous load is carried by the force.
         return self.nonlinear(velocity), 0.0
 
     def vector(self, state: State) -> Array:

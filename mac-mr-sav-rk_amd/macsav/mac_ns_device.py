@@ -30,16 +30,20 @@ from .model import SolveResult
 
 class DeviceMACNavierStokes(MACNavierStokes):
     def __init__(self, grid: MACGrid, nu: float, force: Callable[[float], Any] | None = None,
-                 backend: Any = None, device: int = 0) -> None:
+                 backend: Any = None, device: int = 0, lid: float = 0.0) -> None:
         import torch
+        from .mac.kernels import lid_load
         if backend is None:
             from .mac.gpu_stokes import GpuTensorStokes
             backend = GpuTensorStokes(grid, device=device)
-        super().__init__(grid, nu, force=None, backend=backend)
+        super().__init__(grid, nu, force=None, backend=backend, lid=lid)
         self.device = torch.device(f"cuda:{device}")
         self._lib = _native.load()
-        self._zero_force = torch.zeros(grid.size, dtype=torch.float64, device=self.device)
-        self.force = force if force is not None else (lambda t: self._zero_force)
+        wall = torch.as_tensor(nu * lid_load(grid, self.lid), device=self.device)
+        if force is None:
+            self.force = lambda t: wall
+        else:
+            self.force = lambda t: self.combine((1.0, force(t)), (1.0, wall))
 
     # ----------------------------------------------------------------- plumbing
     def _check(self, code: int) -> None:
