@@ -187,6 +187,14 @@ struct Cfg {
   static constexpr int PARTS = SPLIT_ ? 2 : 1;
 };
 
+// Eight bf16 from p as one 128-bit word, elements at or past `valid` zero (a k tail).
+__device__ inline uint4 load8_tail(const __bf16* p, int valid) {
+  unsigned h[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) h[q] = q < valid ? (unsigned)__builtin_bit_cast(unsigned short, p[q]) : 0u;
+  return make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+}
+
 __device__ inline float to_f(double v) { return (float)v; }
 __device__ inline float to_f(float v) { return v; }
 __device__ inline float to_f(__bf16 v) { return (float)v; }
@@ -267,23 +275,45 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  // One k-tile of operands staged in registers between global memory and LDS.
+  // One k-tile of operands staged in registers between global memory and LDS.  bf16
+  // operands stay packed in 128-bit words (8 elements): as arrays of __bf16 the
+  // compiler gives every element a register of its own and unpacks each load as soon as
+  // it is issued, which waits out the load and defeats the prefetch.
   struct Stage {
     float ra[AMODE == 0 ? 1 : AE];
-    __attribute__((aligned(16))) __bf16 rab[AMODE == 0 ? AE : 8];
-    __attribute__((aligned(16))) __bf16 rb[C::PARTS][BE];
+    uint4 rab[AMODE == 0 ? NA : 1];
+    uint4 rb[C::PARTS][NB];
   };
   Stage S0, S1;
   const int nk = (K + BK - 1) / BK;
   const bool ragged = (K % BK) != 0;
 
+  // Full k-tile loads (no tail handling).
+  auto gload_full = [&](int kt, Stage& st) {
+    const int k0 = kt * BK;
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      if constexpr (AMODE == 0) {
+        st.rab[e] = *reinterpret_cast<const uint4*>(a_ptr[e] + k0);
+      } else if constexpr (AMODE == 1) {
+        const float4 f = *reinterpret_cast<const float4*>(a_ptr[e] + k0);
+        st.ra[e * 4 + 0] = f.x; st.ra[e * 4 + 1] = f.y; st.ra[e * 4 + 2] = f.z; st.ra[e * 4 + 3] = f.w;
+      } else {
+        st.ra[e] = to_f(a_ptr[e][k0]);
+      }
+    }
+#pragma unroll
+    for (int part = 0; part < C::PARTS; ++part)
+#pragma unroll
+      for (int e = 0; e < NB; ++e) st.rb[part][e] = *reinterpret_cast<const uint4*>(b_ptr[part][e] + k0);
+  };
   auto gload = [&](int kt, Stage& st) {
     const int k0 = kt * BK;
     if (!(ragged && kt == nk - 1)) {
 #pragma unroll
       for (int e = 0; e < NA; ++e) {
         if constexpr (AMODE == 0) {
-          *reinterpret_cast<uint4*>(&st.rab[e * 8]) = *reinterpret_cast<const uint4*>(a_ptr[e] + k0);
+          st.rab[e] = *reinterpret_cast<const uint4*>(a_ptr[e] + k0);
         } else if constexpr (AMODE == 1) {
           const float4 f = *reinterpret_cast<const float4*>(a_ptr[e] + k0);
           st.ra[e * 4 + 0] = f.x; st.ra[e * 4 + 1] = f.y; st.ra[e * 4 + 2] = f.z; st.ra[e * 4 + 3] = f.w;
@@ -294,25 +324,23 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
 #pragma unroll
       for (int part = 0; part < C::PARTS; ++part)
 #pragma unroll
-        for (int e = 0; e < NB; ++e)
-          *reinterpret_cast<uint4*>(&st.rb[part][e * 8]) = *reinterpret_cast<const uint4*>(b_ptr[part][e] + k0);
+        for (int e = 0; e < NB; ++e) st.rb[part][e] = *reinterpret_cast<const uint4*>(b_ptr[part][e] + k0);
     } else {
       // Last, partial k-tile: zero beyond K (padding may hold anything).
 #pragma unroll
-      for (int e = 0; e < NA; ++e)
+      for (int e = 0; e < NA; ++e) {
+        if constexpr (AMODE == 0) {
+          st.rab[e] = load8_tail(a_ptr[e] + k0, K - (k0 + a_col[e]));
+        } else {
 #pragma unroll
-        for (int q = 0; q < AV; ++q) {
-          const bool ok = k0 + a_col[e] + q < K;
-          if constexpr (AMODE == 0) st.rab[e * 8 + q] = ok ? a_ptr[e][k0 + q] : (__bf16)0.0f;
-          else st.ra[e * AV + q] = ok ? to_f(a_ptr[e][k0 + q]) : 0.f;
+          for (int q = 0; q < AV; ++q)
+            st.ra[e * AV + q] = k0 + a_col[e] + q < K ? to_f(a_ptr[e][k0 + q]) : 0.f;
         }
+      }
 #pragma unroll
       for (int part = 0; part < C::PARTS; ++part)
 #pragma unroll
-        for (int e = 0; e < NB; ++e)
-#pragma unroll
-          for (int q = 0; q < 8; ++q)
-            st.rb[part][e * 8 + q] = (k0 + b_col[e] + q < K) ? b_ptr[part][e][k0 + q] : (__bf16)0.0f;
+        for (int e = 0; e < NB; ++e) st.rb[part][e] = load8_tail(b_ptr[part][e] + k0, K - (k0 + b_col[e]));
     }
   };
 
@@ -322,7 +350,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
     for (int e = 0; e < NA; ++e) {
       const int v = e * 256 + tid, r = v / APR, c = (v % APR) * AV;
       if constexpr (AMODE == 0) {
-        *reinterpret_cast<uint4*>(&as[r * LDS + c]) = *reinterpret_cast<const uint4*>(&st.rab[e * 8]);
+        *reinterpret_cast<uint4*>(&as[r * LDS + c]) = st.rab[e];
       } else if constexpr (AMODE == 1) {
         typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
         bf4 hi, lo;
@@ -345,7 +373,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
 #pragma unroll
       for (int e = 0; e < NB; ++e) {
         const int v = e * 256 + tid, r = v >> 2, c = (v & 3) * 8;
-        *reinterpret_cast<uint4*>(&bs[r * LDS + c]) = *reinterpret_cast<const uint4*>(&st.rb[part][e * 8]);
+        *reinterpret_cast<uint4*>(&bs[r * LDS + c]) = st.rb[part][e];
       }
     }
   };
@@ -383,13 +411,28 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
 
   // Pipeline: LDS buffers L0/L1 alternate; two register stages keep the global loads
   // of tiles k+1 and k+2 in flight while tile k computes.
-  // The loop runs over pairs of k-tiles with a single exit (an odd last tile is
-  // computed after it), so the accumulators stay in place across iterations.
+  // The loops run over pairs of k-tiles with a single exit (an odd last tile is
+  // computed after them), so the accumulators stay in place across iterations.  The
+  // first loop covers the pairs whose two prefetched tiles are full: its body has no
+  // branches, so each LDS store waits only for its own stage's loads (with branches in
+  // the body the wait counts merge at the joins and every store drains all loads, the
+  // prefetch included).  The second loop finishes the last pair or two.
   gload(0, S0);
   if (nk > 1) gload(1, S1);
   sstore(0, S0);
   __syncthreads();
+  const int nfull = ragged ? nk - 1 : nk;
   int kt = 0;
+  for (; kt + 3 < nfull; kt += 2) {
+    gload_full(kt + 2, S0);
+    compute(0);
+    sstore(1, S1);
+    __syncthreads();
+    gload_full(kt + 3, S1);
+    compute(1);
+    sstore(0, S0);
+    __syncthreads();
+  }
   for (; kt + 2 <= nk; kt += 2) {
     // tile kt is in L0; tile kt+1 is in flight in S1
     if (kt + 2 < nk) gload(kt + 2, S0);
@@ -467,13 +510,14 @@ int g_bf16_group = 4;
 template <typename TA, typename TC, bool SPLIT>
 void launch_bf16(GemmBatch& b, hipStream_t s) {
   int tile = g_bf16_tile, group = g_bf16_group;
-  // Measured at 1024^2 (two or four 1024^3 products per launch): FP64-staged A prefers
-  // 128x128 once that gives a tile per CU (fewer re-reads of the wide operand),
-  // FP32/bf16-staged A prefers 128x64 at 1.5 tiles per CU or more; smaller launches run
-  // 64x64 tiles in groups of 8 tile-rows.
+  // Measured at 1024^2 (two or four 1024^3 products per launch): 128x128 once that gives
+  // a tile per CU (fewer re-reads of both operands: 34.5 us against 37.0 for 128x64 at
+  // four split products), FP32/bf16-staged A at 128x64 from 1.5 tiles per CU; smaller
+  // launches run 64x64 tiles in groups of 8 tile-rows.
   if (tile == 0) {
-    if (std::is_same<TA, double>::value) tile = tile_batch(b, 128, 128) >= 256 ? 128 : 64;
-    else tile = tile_batch(b, 128, 64) >= 384 ? 12864 : 64;
+    if (tile_batch(b, 128, 128) >= 256) tile = 128;
+    else if (!std::is_same<TA, double>::value && tile_batch(b, 128, 64) >= 384) tile = 12864;
+    else tile = 64;
     group = tile == 64 ? 8 : 4;
   }
   if (b.group <= 0) b.group = group;
