@@ -167,8 +167,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64_table(const GemmTable tb) {
 // BM x BN tile (128x128 for large batches, 64x64 otherwise), K step 32 (one MFMA k-step
 // of v_mfma_f32_16x16x32_bf16), 256 threads = 2x2 waves, each wave (BM/2) x (BN/2).
 // LDS rows are k-contiguous for both operands (the MFMA takes 8 consecutive k per
-// lane), padded to 40 elements (80 bytes) to spread banks; double-buffered, the
-// global loads of step k+1 are in flight while step k computes.
+// lane), 64 bytes with XOR-permuted 16-byte chunks (`lx`); double-buffered, the global
+// loads of step k+1 are in flight while step k computes.
 //
 // SPLIT ("bf16x3"): each A element is split on its way into LDS into hi = bf16(a) and
 // lo = bf16(a - hi), the matrix arrives as Q_hi + Q_lo, and every k-step issues
@@ -178,9 +178,19 @@ __global__ __launch_bounds__(256) void k_gemm_f64_table(const GemmTable tb) {
 // Tiles are visited in groups of 4 tile-rows (all tile-columns of a group before the
 // next), and consecutive tiles go to the same XCD, so each XCD's L2 holds a few A
 // row panels and the B panels they meet.
+// Element offset of (row r, k column c) in an LDS operand part: 64-byte rows (32 bf16,
+// no padding), the four 16-byte chunks of row r permuted by chunk ^ ((r >> 1) & 3).
+// With 32 four-byte banks, the 8 lanes of a 16-byte access pass then hit distinct banks
+// both when they read one chunk of 8 consecutive rows (MFMA fragments) and when they
+// store all four chunks of 2 rows (staging).  Measured: SQ_LDS_BANK_CONFLICT from 50% of
+// the LDS-active cycles (rows padded to 40 elements) to 0.
+__device__ inline int lx(int r, int c) {
+  return r * 32 + ((((c >> 3) ^ (r >> 1)) & 3) << 3) + (c & 7);
+}
+
 template <int BM_, int BN_, bool SPLIT_>
 struct Cfg {
-  static constexpr int BM = BM_, BN = BN_, BK = 32, LDS = 40;
+  static constexpr int BM = BM_, BN = BN_, BK = 32, LDS = 32;
   static constexpr int FM = BM / 32, FN = BN / 32;     // 16x16 MFMA blocks per wave
   static constexpr int AE = BM * BK / 256;             // A elements staged per thread
   static constexpr int BE = BN * BK / 256;             // B elements staged per thread
@@ -350,7 +360,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
     for (int e = 0; e < NA; ++e) {
       const int v = e * 256 + tid, r = v / APR, c = (v % APR) * AV;
       if constexpr (AMODE == 0) {
-        *reinterpret_cast<uint4*>(&as[r * LDS + c]) = st.rab[e];
+        *reinterpret_cast<uint4*>(&as[lx(r, c)]) = st.rab[e];
       } else if constexpr (AMODE == 1) {
         typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
         bf4 hi, lo;
@@ -359,12 +369,12 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
           hi[q] = (__bf16)st.ra[e * 4 + q];
           lo[q] = (__bf16)(st.ra[e * 4 + q] - (float)hi[q]);
         }
-        *reinterpret_cast<bf4*>(&as[r * LDS + c]) = hi;
-        if constexpr (SPLIT) *reinterpret_cast<bf4*>(&as[BM * LDS + r * LDS + c]) = lo;
+        *reinterpret_cast<bf4*>(&as[lx(r, c)]) = hi;
+        if constexpr (SPLIT) *reinterpret_cast<bf4*>(&as[BM * LDS + lx(r, c)]) = lo;
       } else {
         const __bf16 hi = (__bf16)st.ra[e];
-        as[r * LDS + c] = hi;
-        if constexpr (SPLIT) as[BM * LDS + r * LDS + c] = (__bf16)(st.ra[e] - (float)hi);
+        as[lx(r, c)] = hi;
+        if constexpr (SPLIT) as[BM * LDS + lx(r, c)] = (__bf16)(st.ra[e] - (float)hi);
       }
     }
 #pragma unroll
@@ -373,7 +383,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
 #pragma unroll
       for (int e = 0; e < NB; ++e) {
         const int v = e * 256 + tid, r = v >> 2, c = (v & 3) * 8;
-        *reinterpret_cast<uint4*>(&bs[r * LDS + c]) = st.rb[part][e];
+        *reinterpret_cast<uint4*>(&bs[lx(r, c)]) = st.rb[part][e];
       }
     }
   };
@@ -383,17 +393,17 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
     const __bf16* bs = Bs[cur];
     bf8 ah[FM], bh[FN];
 #pragma unroll
-    for (int i = 0; i < FM; ++i) ah[i] = *reinterpret_cast<const bf8*>(&as[(wm + i * 16 + li) * LDS + lg * 8]);
+    for (int i = 0; i < FM; ++i) ah[i] = *reinterpret_cast<const bf8*>(&as[lx(wm + i * 16 + li, lg * 8)]);
 #pragma unroll
-    for (int j = 0; j < FN; ++j) bh[j] = *reinterpret_cast<const bf8*>(&bs[(wn + j * 16 + li) * LDS + lg * 8]);
+    for (int j = 0; j < FN; ++j) bh[j] = *reinterpret_cast<const bf8*>(&bs[lx(wn + j * 16 + li, lg * 8)]);
     if constexpr (SPLIT) {
       bf8 al[FM], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
-        al[i] = *reinterpret_cast<const bf8*>(&as[BM * LDS + (wm + i * 16 + li) * LDS + lg * 8]);
+        al[i] = *reinterpret_cast<const bf8*>(&as[BM * LDS + lx(wm + i * 16 + li, lg * 8)]);
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        bl[j] = *reinterpret_cast<const bf8*>(&bs[BN * LDS + (wn + j * 16 + li) * LDS + lg * 8]);
+        bl[j] = *reinterpret_cast<const bf8*>(&bs[BN * LDS + lx(wn + j * 16 + li, lg * 8)]);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
