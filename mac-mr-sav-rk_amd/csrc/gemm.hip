@@ -165,7 +165,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64_table(const GemmTable tb) {
 
 // ------------------------------------------------------------------- BF16 GEMM
 // BM x BN tile (128x128 for large batches, 64x64 otherwise), K step 32 (one MFMA k-step
-// of v_mfma_f32_16x16x32_bf16), 256 threads = 2x2 waves, each wave (BM/2) x (BN/2).
+// of v_mfma_f32_16x16x32_bf16), 256 threads = 2x2 waves, each wave (BM/2) x (BN/2), or
+// 512 threads = 2x4 waves of (BM/2) x (BN/4).
 // LDS rows are k-contiguous for both operands (the MFMA takes 8 consecutive k per
 // lane), 64 bytes with XOR-permuted 16-byte chunks (`lx`); double-buffered, the global
 // loads of step k+1 are in flight while step k computes.
@@ -188,12 +189,16 @@ __device__ inline int lx(int r, int c) {
   return r * 32 + ((((c >> 3) ^ (r >> 1)) & 3) << 3) + (c & 7);
 }
 
-template <int BM_, int BN_, bool SPLIT_>
+// NW waves per block: 4 as a 2x2 grid of (BM/2)x(BN/2) wave tiles, or 8 as a 2x4 grid
+// of (BM/2)x(BN/4) (two waves per SIMD on one tile: the A panel is staged once for
+// twice the waves).
+template <int BM_, int BN_, bool SPLIT_, int NW_ = 4>
 struct Cfg {
-  static constexpr int BM = BM_, BN = BN_, BK = 32, LDS = 32;
-  static constexpr int FM = BM / 32, FN = BN / 32;     // 16x16 MFMA blocks per wave
-  static constexpr int AE = BM * BK / 256;             // A elements staged per thread
-  static constexpr int BE = BN * BK / 256;             // B elements staged per thread
+  static constexpr int BM = BM_, BN = BN_, BK = 32, LDS = 32, NW = NW_, NT = 64 * NW_;
+  static constexpr int WGN = NW_ == 8 ? 4 : 2, WGM = NW_ / WGN;  // wave grid
+  static constexpr int FM = BM / WGM / 16, FN = BN / WGN / 16;   // 16x16 MFMA blocks per wave
+  static constexpr int AE = BM * BK / NT;              // A elements staged per thread
+  static constexpr int BE = BN * BK / NT;              // B elements staged per thread
   static constexpr int PARTS = SPLIT_ ? 2 : 1;
 };
 
@@ -229,10 +234,11 @@ __device__ inline void store4<double>(double* p, const f4& v) {
 
 // AMODE: how A reaches LDS.  0 = bf16 rows (16-byte loads of 8), 1 = fp32 rows with
 // lda % 4 == 0 and 16-byte aligned bases (float4 loads), 2 = one element per load.
-template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE>
-__global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
-  using C = Cfg<BM, BN, SPLIT>;
-  constexpr int BK = C::BK, LDS = C::LDS, FM = C::FM, FN = C::FN, AE = C::AE, BE = C::BE;
+template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
+__global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
+  using C = Cfg<BM, BN, SPLIT, NW>;
+  constexpr int BK = C::BK, LDS = C::LDS, FM = C::FM, FN = C::FN, AE = C::AE, BE = C::BE, NT = C::NT;
+  static_assert(AE >= 1 && BE >= 8, "tile too small for the block");
   constexpr int AV = AMODE == 0 ? 8 : (AMODE == 1 ? 4 : 1);   // A elements per load
   constexpr int NA = AE / AV;                                   // A loads per thread
   constexpr int NB = BE / 8;                                    // B loads per thread per part
@@ -253,7 +259,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+  const int wm = (wave / C::WGN) * (BM / C::WGM), wn = (wave % C::WGN) * (BN / C::WGN);
   const int li = lane & 15, lg = lane >> 4;
   const int M = E.M, N = E.N, K = E.K;
 
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
   int a_col[NA];
 #pragma unroll
   for (int e = 0; e < NA; ++e) {
-    const int v = e * 256 + tid, r = v / APR, c = (v % APR) * AV;
+    const int v = e * NT + tid, r = v / APR, c = (v % APR) * AV;
     a_ptr[e] = static_cast<const TA*>(E.A) + (int64_t)min(m0 + r, M - 1) * E.lda + c;
     a_col[e] = c;
   }
@@ -272,7 +278,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
   int b_col[NB];
 #pragma unroll
   for (int e = 0; e < NB; ++e) {
-    const int v = e * 256 + tid, r = v >> 2, c = (v & 3) * 8;
+    const int v = e * NT + tid, r = v >> 2, c = (v & 3) * 8;
     const int64_t off = (int64_t)min(n0 + r, N - 1) * E.ldb + c;
     b_ptr[0][e] = static_cast<const __bf16*>(E.Bt) + off;
     if constexpr (SPLIT) b_ptr[1][e] = static_cast<const __bf16*>(E.Bt_lo) + off;
@@ -358,7 +364,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
     __bf16* as = As[buf];
 #pragma unroll
     for (int e = 0; e < NA; ++e) {
-      const int v = e * 256 + tid, r = v / APR, c = (v % APR) * AV;
+      const int v = e * NT + tid, r = v / APR, c = (v % APR) * AV;
       if constexpr (AMODE == 0) {
         *reinterpret_cast<uint4*>(&as[lx(r, c)]) = st.rab[e];
       } else if constexpr (AMODE == 1) {
@@ -382,7 +388,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmBatch batch) {
       __bf16* bs = Bs[buf] + part * BN * LDS;
 #pragma unroll
       for (int e = 0; e < NB; ++e) {
-        const int v = e * 256 + tid, r = v >> 2, c = (v & 3) * 8;
+        const int v = e * NT + tid, r = v >> 2, c = (v & 3) * 8;
         *reinterpret_cast<uint4*>(&bs[lx(r, c)]) = st.rb[part][e];
       }
     }
@@ -486,52 +492,56 @@ int tile_batch(GemmBatch& b, int bm, int bn) {
   return total;
 }
 
-template <typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE>
+template <typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
 void launch_bf16_mode(GemmBatch& b, int total, hipStream_t s) {
+  constexpr int nt = 64 * NW;
   switch (b.epi) {
-    case EPI_STORE: k_gemm_bf16<EPI_STORE, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
-    case EPI_SHIFT: k_gemm_bf16<EPI_SHIFT, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
-    case EPI_LAMBDA: k_gemm_bf16<EPI_LAMBDA, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
-    case EPI_ACCUM: k_gemm_bf16<EPI_ACCUM, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
-    default: k_gemm_bf16<EPI_COMBINE, TA, TC, SPLIT, BM, BN, AMODE><<<total, 256, 0, s>>>(b); break;
+    case EPI_STORE: k_gemm_bf16<EPI_STORE, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
+    case EPI_SHIFT: k_gemm_bf16<EPI_SHIFT, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
+    case EPI_LAMBDA: k_gemm_bf16<EPI_LAMBDA, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
+    case EPI_ACCUM: k_gemm_bf16<EPI_ACCUM, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
+    default: k_gemm_bf16<EPI_COMBINE, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
   }
 }
 
-template <typename TA, typename TC, bool SPLIT, int BM, int BN>
+template <typename TA, typename TC, bool SPLIT, int BM, int BN, int NW = 4>
 void launch_bf16_tile(GemmBatch& b, hipStream_t s) {
   const int total = tile_batch(b, BM, BN);
   if constexpr (std::is_same<TA, __bf16>::value) {
-    launch_bf16_mode<TA, TC, SPLIT, BM, BN, 0>(b, total, s);
+    launch_bf16_mode<TA, TC, SPLIT, BM, BN, 0, NW>(b, total, s);
   } else if constexpr (std::is_same<TA, float>::value) {
     bool vec = true;
     for (int i = 0; i < b.count; ++i)
       vec = vec && (b.e[i].lda % 4 == 0) && (reinterpret_cast<uintptr_t>(b.e[i].A) % 16 == 0);
-    if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 1>(b, total, s);
-    else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2>(b, total, s);
+    if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 1, NW>(b, total, s);
+    else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW>(b, total, s);
   } else {
-    launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2>(b, total, s);
+    launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW>(b, total, s);
   }
 }
 
-// Tile shape: 0 = automatic, else 64 (64x64), 12864 (128x64), 128 (128x128).
+// Tile shape: 0 = automatic, else 64 (64x64), 12864 (128x64), 128 (128x128), 1288
+// (128x128 with 8 waves).
 int g_bf16_tile = 0;
 int g_bf16_group = 4;
 
 template <typename TA, typename TC, bool SPLIT>
 void launch_bf16(GemmBatch& b, hipStream_t s) {
   int tile = g_bf16_tile, group = g_bf16_group;
-  // Measured at 1024^2 (two or four 1024^3 products per launch): 128x128 once that gives
-  // a tile per CU (fewer re-reads of both operands: 34.5 us against 37.0 for 128x64 at
-  // four split products), FP32/bf16-staged A at 128x64 from 1.5 tiles per CU; smaller
-  // launches run 64x64 tiles in groups of 8 tile-rows.
+  // Measured at 1024^2 (two or four 1024^3 products per launch): 128x128 tiles with 8
+  // waves once that gives a tile per CU (27.9 us at four split products, against 34.0
+  // with 4 waves and 35.2 for 128x64; 42.8 against 56.1 us with FP64-staged A),
+  // FP32/bf16-staged A at 128x64 from 1.5 tiles per CU; smaller launches run 64x64
+  // tiles (4 waves) in groups of 8 tile-rows.
   if (tile == 0) {
-    if (tile_batch(b, 128, 128) >= 256) tile = 128;
+    if (tile_batch(b, 128, 128) >= 256) tile = 1288;
     else if (!std::is_same<TA, double>::value && tile_batch(b, 128, 64) >= 384) tile = 12864;
     else tile = 64;
     group = tile == 64 ? 8 : 4;
   }
   if (b.group <= 0) b.group = group;
   if (tile == 128) launch_bf16_tile<TA, TC, SPLIT, 128, 128>(b, s);
+  else if (tile == 1288) launch_bf16_tile<TA, TC, SPLIT, 128, 128, 8>(b, s);
   else if (tile == 12864) launch_bf16_tile<TA, TC, SPLIT, 128, 64>(b, s);
   else launch_bf16_tile<TA, TC, SPLIT, 64, 64>(b, s);
 }

@@ -33,6 +33,26 @@ def test_bf16_pressure_inverse_is_accurate_to_bf16_rounding(shape):
     assert _rel(got, want) < 2e-2
 
 
+@pytest.mark.parametrize("shape", [(257, 130, 1.3, 0.8), (520, 512, 1.0, 1.0), (1024, 1024, 1.0, 1.0)])
+def test_split_transforms_are_accurate_on_every_tile_path(shape):
+    """The split (bf16x3) velocity and pressure inverses against the oracle at sizes whose
+    launches take the 64x64, 128x64 and 8-wave 128x128 tiles: about 16 significant bits
+    per operand, so well inside 2e-4 relative (plain bf16 is near 1e-2)."""
+    g = mo.Grid(*shape)
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(g.size, 2))
+    want = mo.TensorStokes(g).velocity_inverse(x, 1.0, 1e-4)
+    got = _backend(g, precision="mixed").velocity_inverse(_dev(x), 1.0, 1e-4).cpu().numpy()
+    err_v = _rel(got, want)
+    y = rng.normal(size=(g.np, 2))
+    y -= y.mean(axis=0)
+    want_p = mo.TensorStokes(g).pressure_poisson_inverse(y)
+    got_p = _backend(g, precision="mixed").pressure_poisson_inverse(_dev(y)).cpu().numpy()
+    err_p = _rel(got_p, want_p)
+    print(f"{shape}: split velocity inverse {err_v:.2e}, pressure inverse {err_p:.2e}")
+    assert err_v < 2e-4 and err_p < 2e-4
+
+
 @pytest.mark.parametrize("shape,visc", [((7, 5, 2.0, 1.0), 0.07), ((64, 64, 1.0, 1.0), 2.929e-6),
                                         ((256, 192, 1.0, 1.0), 1e-4)])
 def test_mixed_solve_matches_oracle_after_refinement(shape, visc):
