@@ -854,7 +854,9 @@ int32_t mac_stokes_solve_batched(void* h, const double* rhs, const double* mass,
     double* vel = velocity;
     double* pr = pressure;
     if (!device_pointers) {
-      HIP_CHECK(hipMemcpyAsync(s->io_f(), rhs, sizeof(double) * V, hipMemcpyHostToDevice, st));
+      // Pageable host arrays: blocking copies with the stream idle.
+      HIP_CHECK(hipStreamSynchronize(st));
+      HIP_CHECK(hipMemcpy(s->io_f(), rhs, sizeof(double) * V, hipMemcpyHostToDevice));
       f = s->io_f();
       vel = s->io_v();
       pr = s->io_p();
@@ -869,11 +871,11 @@ int32_t mac_stokes_solve_batched(void* h, const double* rhs, const double* mass,
       throw macs::SolveFailure(MAC_NOT_CONVERGED, "Pressure Schur CG failed with info=" +
                                                       std::to_string(s->opts().max_iterations) + " (cavity " +
                                                       std::to_string(out.failed_cg.front()) + ")");
-    if (!device_pointers) {
-      HIP_CHECK(hipMemcpyAsync(velocity, vel, sizeof(double) * V, hipMemcpyDeviceToHost, st));
-      HIP_CHECK(hipMemcpyAsync(pressure, pr, sizeof(double) * P, hipMemcpyDeviceToHost, st));
-    }
     HIP_CHECK(hipStreamSynchronize(st));
+    if (!device_pointers) {
+      HIP_CHECK(hipMemcpy(velocity, vel, sizeof(double) * V, hipMemcpyDeviceToHost));
+      HIP_CHECK(hipMemcpy(pressure, pr, sizeof(double) * P, hipMemcpyDeviceToHost));
+    }
     for (int i = 0; i < nc; ++i) {
       if (residual) residual[i] = out.residual[i];
       if (divergence_inf) divergence_inf[i] = out.divergence[i];

@@ -1103,13 +1103,17 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
     double* pr = pressure;
     int64_t lf = ld_rhs, lv = ld_velocity, lp = ld_pressure, lg = ld_guess;
     if (!device_pointers) {
-      HIP_CHECK(hipMemcpy2DAsync(s->io_f_, size * sizeof(double), rhs, ld_rhs * sizeof(double), size * sizeof(double),
-                                 ncols, hipMemcpyHostToDevice, st));
+      // Host arrays are pageable: blocking copies, with the solver's stream idle (an
+      // asynchronous copy into or out of pageable memory followed by a stream
+      // synchronisation has been seen to leave the host array unwritten).
+      HIP_CHECK(hipStreamSynchronize(st));
+      HIP_CHECK(hipMemcpy2D(s->io_f_, size * sizeof(double), rhs, ld_rhs * sizeof(double), size * sizeof(double),
+                            ncols, hipMemcpyHostToDevice));
       f = s->io_f_;
       lf = size;
       if (guess) {
-        HIP_CHECK(hipMemcpy2DAsync(s->io_g_, np * sizeof(double), guess, ld_guess * sizeof(double),
-                                   np * sizeof(double), ncols, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpy2D(s->io_g_, np * sizeof(double), guess, ld_guess * sizeof(double), np * sizeof(double),
+                              ncols, hipMemcpyHostToDevice));
         gs = s->io_g_;
         lg = np;
       }
@@ -1127,13 +1131,13 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
     for (double v : fm)
       if (!is_finite(v)) throw macs::ArgFailure("Invalid Stokes RHS");
     Solver::Outcome out = s->solve(f, lf, ncols, mass, viscosity, vel, lv, pr, lp, gs, lg);
-    if (!device_pointers) {
-      HIP_CHECK(hipMemcpy2DAsync(velocity, ld_velocity * sizeof(double), vel, size * sizeof(double),
-                                 size * sizeof(double), ncols, hipMemcpyDeviceToHost, st));
-      HIP_CHECK(hipMemcpy2DAsync(pressure, ld_pressure * sizeof(double), pr, np * sizeof(double),
-                                 np * sizeof(double), ncols, hipMemcpyDeviceToHost, st));
-    }
     HIP_CHECK(hipStreamSynchronize(st));
+    if (!device_pointers) {
+      HIP_CHECK(hipMemcpy2D(velocity, ld_velocity * sizeof(double), vel, size * sizeof(double), size * sizeof(double),
+                            ncols, hipMemcpyDeviceToHost));
+      HIP_CHECK(hipMemcpy2D(pressure, ld_pressure * sizeof(double), pr, np * sizeof(double), np * sizeof(double),
+                            ncols, hipMemcpyDeviceToHost));
+    }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     int total = 0;
     for (int c = 0; c < ncols; ++c) {
