@@ -99,6 +99,7 @@ class Solver {
     if (host_flags_) (void)hipHostFree(host_flags_);
     if (hist_) (void)hipFree(hist_);
     if (hist_u_) (void)hipFree(hist_u_);
+    if (hist_b_) (void)hipFree(hist_b_);
     if (mailbox_) (void)hipHostFree(mailbox_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -385,9 +386,11 @@ class Solver {
   const double* schur_rhs() const { return b_; }
   int last_ncols() const { return ncols_cur_; }
 
-  // Fills io_g_ with the network's pressure guess from the Schur right-hand side b and
-  // the previous solve's pressures (same column count), if a network is attached and
-  // the grid is one it serves.
+  // Fills io_g_ with the warm start's pressure guess (previous solve, history, or the
+  // network's correction of a base guess), if there is one.  The network reads the
+  // base's residual, b - b_base (the Schur right-hand sides recorded with the base's
+  // pressures, combined like them): with one operator S p_m = b_m, so the correction
+  // it predicts, p - base, solves S (p - base) = b - b_base.
   bool learned_guess(const double* b, int ncols) {
     if (warm_mode_ == MAC_WARM_PREVIOUS) {
       // Previous solve's pressures as the guess (same column count only).
@@ -401,15 +404,29 @@ class Solver {
     // else the previous solve's pressures.  Without a base there is no guess (zero start),
     // as in training.
     const double* base = nullptr;
+    const int64_t n = np_ * ncols;
     if (history_period_ > 0) {
-      if (history_guess(ncols, warm_base_)) base = warm_base_;
-    } else if (prev_ncols_ == ncols) {
+      if (history_b_ok(ncols) && history_guess(ncols, warm_base_) && extrapolate_history(hist_b_, n, warm_db_))
+        base = warm_base_;
+    } else if (prev_ncols_ == ncols && prev_b_ok_) {
       base = pprev_;
+      copy_cols(bprev_, np_, warm_db_, np_, np_, ncols, stream_);
     }
     if (base == nullptr) return false;
-    warm_guess(cnn_, b, np_, base, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, io_g_, np_,
-               stream_);
+    const double w[3] = {1.0, -1.0, 0.0};
+    extrapolate(b, warm_db_, nullptr, w, warm_db_, n, stream_);  // b - b_base
+    warm_guess(cnn_, warm_db_, np_, base, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, io_g_,
+               np_, stream_);
     return true;
+  }
+  // Every slot the history extrapolation reads has its Schur right-hand side recorded.
+  bool history_b_ok(int ncols) const {
+    if (history_period_ <= 0 || hist_ncols_ != ncols || hist_b_ == nullptr) return false;
+    const int P = history_period_, slots = kHistSteps * P;
+    const int64_t steps = hist_count_ / P >= 3 ? 3 : hist_count_ / P;
+    for (int64_t k = 1; k <= steps; ++k)
+      if (!hist_b_ok_[(int)((hist_count_ - k * P) % slots)]) return false;
+    return steps > 0;
   }
   bool history_guess(int ncols, double* out) {
     if (history_period_ <= 0 || hist_ncols_ != ncols) return false;
@@ -461,10 +478,13 @@ class Solver {
     }
   }
   // Records the solve's pressures (and, for the mixed solve, velocities: its outer
-  // iterate starts from both) for the warm starts.
-  void remember_pressure(const double* p, int ncols, const double* u = nullptr) {
+  // iterate starts from both) for the warm starts, with the zero start's Schur
+  // right-hand side b when the solve formed it (the network's input).
+  void remember_pressure(const double* p, int ncols, const double* u = nullptr, const double* b = nullptr) {
     copy_cols(p, np_, pprev_, np_, np_, ncols, stream_);
     prev_ncols_ = ncols;
+    prev_b_ok_ = b != nullptr;
+    if (b) copy_cols(b, np_, bprev_, np_, np_, ncols, stream_);
     if (history_period_ > 0) {
       const int slots = kHistSteps * history_period_;
       const bool with_u = u != nullptr;
@@ -472,8 +492,10 @@ class Solver {
         HIP_CHECK(hipStreamSynchronize(stream_));
         if (hist_) (void)hipFree(hist_);
         if (hist_u_) (void)hipFree(hist_u_);
-        hist_ = hist_u_ = nullptr;
+        if (hist_b_) (void)hipFree(hist_b_);
+        hist_ = hist_u_ = hist_b_ = nullptr;
         HIP_CHECK(hipMalloc(&hist_, sizeof(double) * np_ * ncols * slots));
+        HIP_CHECK(hipMalloc(&hist_b_, sizeof(double) * np_ * ncols * slots));
         if (with_u) HIP_CHECK(hipMalloc(&hist_u_, sizeof(double) * size_ * ncols * slots));
         hist_ncols_ = ncols;
         hist_slots_ = slots;
@@ -482,7 +504,9 @@ class Solver {
       }
       const int slot = (int)(hist_count_ % slots);
       hist_shift_[slot] = shift_cur_;
+      hist_b_ok_[slot] = b != nullptr;
       copy_cols(p, np_, hist_ + (int64_t)slot * np_ * ncols, np_, np_, ncols, stream_);
+      if (b) copy_cols(b, np_, hist_b_ + (int64_t)slot * np_ * ncols, np_, np_, ncols, stream_);
       if (with_u) copy_cols(u, size_, hist_u_ + (int64_t)slot * size_ * ncols, size_, size_, ncols, stream_);
       ++hist_count_;
     }
@@ -529,7 +553,7 @@ class Solver {
     reduce(x_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_, fin(sums_));
     sub_mean_dot(x_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
     copy_cols(x_, np_, pres, pres_stride, np_, ncols, stream_);
-    remember_pressure(x_, ncols);
+    remember_pressure(x_, ncols, nullptr, b_);
     // u = L^-1 (f + D^T p)
     divergence_transpose(x_, np_, nullptr, f, f_stride, DT_F64, w_, packed_, g_, ncols, nullptr, stream_);
     velocity_inverse(w_, size_, vel, vel_stride, all, mass, visc);
@@ -567,6 +591,7 @@ class Solver {
     // measures its true error and every sweep improves on it; started inside a bf16 inner
     // solve instead, its accuracy below the bf16 operator's would be lost.
     out.warm = false;
+    bool bzero_ok = false;  // bzero_ holds this solve's zero-start Schur right-hand side
     if (guess) {
       copy_cols(guess, guess_stride, pacc_, np_, np_, ncols, stream_);
     } else if (warm_mode_ != MAC_WARM_OFF && warm_mode_ != MAC_WARM_NETWORK && learned_guess(nullptr, ncols)) {
@@ -588,6 +613,8 @@ class Solver {
                  fin(sums_));
       sub_mean_dot(b_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
       mixed_ = false;
+      copy_cols(b_, np_, bzero_, np_, np_, ncols, stream_);  // the sweeps reuse b_
+      bzero_ok = true;
       if (learned_guess(b_, ncols)) {
         copy_cols(io_g_, np_, pacc_, np_, np_, ncols, stream_);
         if (history_period_ > 0) history_velocity(ncols, uacc_);
@@ -637,7 +664,7 @@ class Solver {
     sub_mean_dot(pacc_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
     copy_cols(pacc_, np_, pres, pres_stride, np_, ncols, stream_);
     copy_cols(uacc_, size_, vel, vel_stride, size_, ncols, stream_);
-    remember_pressure(pacc_, ncols, uacc_);
+    remember_pressure(pacc_, ncols, uacc_, bzero_ok ? bzero_ : nullptr);
     return out;
   }
 
@@ -900,11 +927,15 @@ class Solver {
     q_ = (double*)take(pb);
     fcomp_ = (double*)take(vb);
     pprev_ = (double*)take(pb);
+    bprev_ = (double*)take(pb);
+    bzero_ = (double*)take(pb);
     warm_base_ = (double*)take(pb);
+    warm_db_ = (double*)take(pb);
     warm_x_ = (float*)take(sizeof(float) * 2 * (np_ / 16 + 64) * ncols);
     warm_y_ = (float*)take(sizeof(float) * (np_ / 16 + 64) * ncols);
     warm_s_ = (double*)take(sizeof(double) * (2 * ncols + 4));
     prev_ncols_ = 0;
+    prev_b_ok_ = false;
     uacc_ = (double*)take(vb);
     ru_ = (double*)take(vb);
     pacc_ = (double*)take(pb);
@@ -971,6 +1002,12 @@ class Solver {
   double hist_shift_[kHistSteps * 16] = {};    // and of each recorded slot (period <= 16)
   double* hist_ = nullptr;      // ring of 3*period recorded pressures (np x ncols each)
   double* hist_u_ = nullptr;    // and velocities (size x ncols each; mixed solves)
+  double* hist_b_ = nullptr;    // and zero-start Schur right-hand sides (np x ncols each)
+  bool hist_b_ok_[kHistSteps * 16] = {};       // slot's right-hand side recorded
+  double* bprev_ = nullptr;     // the previous solve's zero-start Schur right-hand side
+  bool prev_b_ok_ = false;
+  double* bzero_ = nullptr;     // mixed network solves: this solve's, kept from the sweeps
+  double* warm_db_ = nullptr;   // b_base, then b - b_base
   bool hist_has_u_ = false;
   double* warm_base_ = nullptr;
   double* pprev_ = nullptr;

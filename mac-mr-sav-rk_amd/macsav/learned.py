@@ -1,16 +1,20 @@
 """Learned warm start for the pressure Schur CG: the network, its training, its data.
 
 `WarmStartNet` wraps the device network of libmacstokes.so (forward, backward and Adam
-written as gfx950 kernels).  It maps the Schur right-hand side b of a stage column and
-a base guess of that column's pressure, both average-pooled by 4 and normalised by
-rms(pool4 b), to the correction pool4(p - base) / (alpha rms(pool4 b)); the solver adds
-the bilinearly upsampled output to the base.  The base is the same stage of the last two
-steps extrapolated in time when the solver has a history period, else the previous
-solve's pressure.  The solve still runs to the same tolerance and is the judge: a poor
-guess costs iterations, never accuracy.
+written as gfx950 kernels).  For a stage column it reads the residual of a base guess
+of the pressure, db = b - b_base (b the zero start's Schur right-hand side, b_base the
+right-hand sides recorded with the base's pressures, combined like them), and the base
+itself, both average-pooled by 4 and each normalised by its own rms.  It outputs the
+correction pool4(p - base) / (alpha rms(pool4 db)); the solver adds the bilinearly
+upsampled output to the base.  With one operator the correction solves
+S (p - base) = db, so its scale follows db's.  The base is the same stage of the last
+three steps extrapolated in time when the solver has a history period, else the
+previous solve's pressure.  The solve still runs to the same tolerance and is the
+judge: a poor guess costs iterations, never accuracy.
 
 Training pairs come from the plain FP64 solver (`collect_pairs`): for every stage
-column of a run, (b, base guess, pressure).  No counterpart in the reference.
+column of a run that has a base, (db, base guess, pressure).  No counterpart in the
+reference.
 """
 from __future__ import annotations
 
@@ -121,7 +125,8 @@ def prepare(grid, b: Any, pprev: Any | None, target: Any | None, alpha: float) -
 
 @dataclass
 class Pairs:
-    """Stage columns recorded from plain FP64 solves: b, base guess, pressure."""
+    """Stage columns recorded from plain FP64 solves: the base's residual b - b_base (in
+    `b`), base guess, pressure."""
     b: list
     base: list
     p: list
@@ -144,29 +149,33 @@ def base_guess(history: list, period: int | None) -> Any:
 
 def collect_pairs(model, scheme, initial, steps: int, dt: float, period: int | None = None) -> Pairs:
     """Run `scheme` on `model` (whose backend must be an FP64 GpuTensorStokes) and record,
-    for every stage solve that has a base guess, (b, base guess, pressure) per column as
-    device tensors of shape (np, ncols); `period` as in `GpuTensorStokes.set_warm_start`."""
+    for every stage solve that has a base guess, (b - b_base, base guess, pressure) per
+    column as device tensors of shape (np, ncols); `period` as in
+    `GpuTensorStokes.set_warm_start`."""
     import torch
     backend = model.backend
     backend.reset_history()
     pairs = Pairs([], [], [])
     original = backend.solve
     history: dict[int, list] = {}
+    rhs_history: dict[int, list] = {}
 
     def recording(rhs, *, mass, viscosity, pressure_guess=None):
         res = original(rhs, mass=mass, viscosity=viscosity, pressure_guess=pressure_guess)
         k = 1 if np.ndim(rhs) == 1 else np.shape(rhs)[1]
         b = backend.last_schur_rhs(k)
         p = torch.as_tensor(np.asarray(res.pressure).reshape(model.grid.np, k), device=b.device)
-        past = history.setdefault(k, [])
+        past, past_b = history.setdefault(k, []), rhs_history.setdefault(k, [])
         base = base_guess(past, period)
         if base is not None:
             # Solves without a base guess are not learned: the solver then starts from zero.
-            pairs.b.append(b.clone())
+            pairs.b.append(b - base_guess(past_b, period))
             pairs.base.append(base)
             pairs.p.append(p.clone())
         past.append(p.clone())
+        past_b.append(b.clone())
         del past[:-3 * max(period or 1, 1)]
+        del past_b[:-3 * max(period or 1, 1)]
         return res
 
     backend.solve = recording
@@ -180,7 +189,7 @@ def collect_pairs(model, scheme, initial, steps: int, dt: float, period: int | N
 
 
 def fit_alpha(grid, pairs: Pairs) -> float:
-    """alpha = median over columns of rms(pool4 (p - base)) / rms(pool4 b)."""
+    """alpha = median over columns of rms(pool4 (p - base)) / rms(pool4 (b - b_base))."""
     ratios = []
     for b, pp, p in zip(pairs.b, pairs.base, pairs.p):
         xb, _, sb = prepare(grid, b, None, None, 1.0)
