@@ -205,7 +205,8 @@ def bench_cavities(args) -> dict:
 
 def replay(grid, rec, warmup: int, steps: int, precision: str, warm=None, profile: bool = False):
     """Solve rec[:warmup] untimed (they also fill the warm-start history), then time
-    rec[warmup:warmup+steps]; optionally a second, profiled pass of the timed solves."""
+    rec[warmup:warmup+steps]; optionally a second pass with HIP events on the timed
+    solves' transform GEMMs."""
     import torch
     from macsav import GpuTensorStokes
     solver = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14, precision=precision,
@@ -213,11 +214,13 @@ def replay(grid, rec, warmup: int, steps: int, precision: str, warm=None, profil
     if warm is not None:
         solver.set_warm_start(warm, period=STAGES)
 
-    def run_once(timed_results=None):
+    def run_once(timed_results=None, profiled=False):
         solver.reset_history()
         for f, m, v in rec[:warmup]:
             solver.solve(f, mass=m, viscosity=v)
         torch.cuda.synchronize()
+        if profiled:  # HIP events on the timed solves only
+            solver.set_profiling(True)
         it0, sw0, w0 = solver.total_iterations, solver.sweeps, solver.warm_starts
         worst = [0.0, 0.0]
         t0 = time.perf_counter()
@@ -237,11 +240,11 @@ def replay(grid, rec, warmup: int, steps: int, precision: str, warm=None, profil
              "warm_started_solves": warm_used, "worst_residual": worst[0], "worst_divergence_inf": worst[1]}
     prof = None
     if profile:
-        solver.set_profiling(True)
-        run_once()
+        prof_elapsed = run_once(profiled=True)[0]
         prof = solver.kernel_profile()
         solver.set_profiling(False)
-        stats["gemm_share_of_step"] = prof["gemm_ms"] * 1e-3 / (elapsed + 1e-30)
+        # Share of the profiled pass's own wall time (its event pairs slow it a little).
+        stats["gemm_share_of_step"] = prof["gemm_ms"] * 1e-3 / (prof_elapsed + 1e-30)
     return stats, prof, velocities
 
 
@@ -369,6 +372,40 @@ def bench_combined(args) -> dict:
             "final_max_abs_vs_fp64": float(np.abs(diff).max())}
 
 
+def bench_sizes(args) -> dict:
+    """Two-RHS stage-solve time against grid size (n x n, n in 64..1024), plain FP64
+    against mixed (zero start), on the recorded right-hand sides of a short
+    sdirk3_mrsav run at each size (nu = 1e-3, dt = 2e-4): where the bf16 transforms start
+    to pay, which sets the precision of the batched cavity path (n <= 512)."""
+    import torch
+    from macsav import GpuTensorStokes, MACGrid
+    rows = {}
+    for n in (64, 128, 256, 384, 512, 768, 1024):
+        grid = MACGrid(n, n)
+        rec = record_stage_solves(grid, 12, modes=min(32, n // 4))
+        row = {}
+        for precision in ("fp64", "mixed"):
+            s = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14, precision=precision,
+                                refine_tolerance=1e-10)
+            for f, m, v in rec[:4]:
+                s.solve(f, mass=m, viscosity=v)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for f, m, v in rec[4:]:
+                s.solve(f, mass=m, viscosity=v)
+            torch.cuda.synchronize()
+            row[precision] = 1e3 * (time.perf_counter() - t0) / (len(rec) - 4)
+        row["mixed_speedup"] = row["fp64"] / row["mixed"]
+        rows[str(n)] = row
+        print(f"[sizes] {n}: {row}", file=sys.stderr, flush=True)
+    return {"metric": "ms per two-RHS stage solve against grid size, plain FP64 and mixed",
+            "value": rows["1024"]["mixed_speedup"], "unit": "x (mixed over FP64 at 1024^2)", "n_gpus": 1,
+            "steps": 8, "warmup": 4, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16-gemm/f64-refine", "data": "recorded sdirk3_mrsav stage right-hand sides per size",
+            "config": {"workload": "MAC n x n stage solves, n in 64..1024, zero start, residual 1e-10"},
+            "sizes": rows}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -379,7 +416,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-network", action="store_true")
-    ap.add_argument("--workload", choices=["stage1024", "cavities", "run", "combined"], default="stage1024")
+    ap.add_argument("--workload", choices=["stage1024", "cavities", "run", "combined", "sizes"], default="stage1024")
     ap.add_argument("--t-end", type=float, default=0.05, help="combined run: final time")
     ap.add_argument("--max-steps", type=int, default=2000, help="combined run: accepted-step cap")
     ap.add_argument("--cavities", type=int, default=256)
@@ -391,6 +428,9 @@ def main() -> int:
 
     if args.workload == "cavities":
         print(json.dumps(bench_cavities(args)), flush=True)
+        return 0
+    if args.workload == "sizes":
+        print(json.dumps(bench_sizes(args)), flush=True)
         return 0
     if args.workload == "combined":
         print(json.dumps(bench_combined(args)), flush=True)

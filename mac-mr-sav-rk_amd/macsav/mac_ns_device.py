@@ -135,10 +135,14 @@ class DeviceMACNavierStokes(MACNavierStokes):
 
     def solve_columns(self, columns: Sequence[Any], *, mass: float, viscosity: float) -> list[SolveResult]:
         import torch
-        stacked = torch.stack([self._dev(c).reshape(-1) for c in columns], dim=1)
+        # Columns stacked as rows of a (k, size) tensor: the backend's column-major input,
+        # so no transpose copy; the results come back the same way and each column is a
+        # contiguous row view.
+        stacked = torch.stack([self._dev(c).reshape(-1) for c in columns], dim=0).t()
         out = self.backend.solve(stacked, mass=mass, viscosity=viscosity)
-        return [StokesResult(out.velocity[:, k].contiguous(), out.pressure[:, :, k].contiguous(), out.residual,
-                             out.divergence_inf) for k in range(len(columns))]
+        velocity, pressure = out.velocity.t(), out.pressure.permute(2, 0, 1)
+        return [StokesResult(velocity[k], pressure[k], out.residual, out.divergence_inf)
+                for k in range(len(columns))]
 
     # ------------------------------------------------------------- diagnostics
     def collocated_velocity(self, velocity: Any) -> tuple[Any, Any]:
