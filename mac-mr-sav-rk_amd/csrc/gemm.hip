@@ -238,7 +238,7 @@ template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMO
 __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
   using C = Cfg<BM, BN, SPLIT, NW>;
   constexpr int BK = C::BK, LDS = C::LDS, FM = C::FM, FN = C::FN, AE = C::AE, BE = C::BE, NT = C::NT;
-  static_assert(AE >= 1 && BE >= 8, "tile too small for the block");
+  static_assert(AE >= (AMODE == 0 ? 8 : AMODE == 1 ? 4 : 1) && BE >= 8, "tile too small for the block");
   constexpr int AV = AMODE == 0 ? 8 : (AMODE == 1 ? 4 : 1);   // A elements per load
   constexpr int NA = AE / AV;                                   // A loads per thread
   constexpr int NB = BE / 8;                                    // B loads per thread per part
@@ -521,7 +521,7 @@ void launch_bf16_tile(GemmBatch& b, hipStream_t s) {
 }
 
 // Tile shape: 0 = automatic, else 64 (64x64), 12864 (128x64), 128 (128x128), 1288
-// (128x128 with 8 waves).
+// (128x128 with 8 waves), 641288 (64x128 with 8 waves; FP32/FP64-staged A only).
 int g_bf16_tile = 0;
 int g_bf16_group = 4;
 
@@ -530,11 +530,14 @@ void launch_bf16(GemmBatch& b, hipStream_t s) {
   int tile = g_bf16_tile, group = g_bf16_group;
   // Measured at 1024^2 (two or four 1024^3 products per launch): 128x128 tiles with 8
   // waves once that gives a tile per CU (27.9 us at four split products, against 34.0
-  // with 4 waves and 35.2 for 128x64; 42.8 against 56.1 us with FP64-staged A),
-  // FP32/bf16-staged A at 128x64 from 1.5 tiles per CU; smaller launches run 64x64
-  // tiles (4 waves) in groups of 8 tile-rows.
+  // with 4 waves and 35.2 for 128x64; 42.8 against 56.1 us with FP64-staged A), else
+  // 64x128 tiles with 8 waves at a tile per CU (19.0 us at two split products against
+  // 21.8 for 64x64; 29.3 against 35.6 with FP64-staged A; not for bf16-staged A, whose
+  // 16-byte loads need 8 elements per thread), FP32-staged A at 128x64 from 1.5 tiles
+  // per CU; smaller launches run 64x64 tiles (4 waves) in groups of 8 tile-rows.
   if (tile == 0) {
     if (tile_batch(b, 128, 128) >= 256) tile = 1288;
+    else if (!std::is_same<TA, __bf16>::value && tile_batch(b, 64, 128) >= 256) tile = 641288;
     else if (!std::is_same<TA, double>::value && tile_batch(b, 128, 64) >= 384) tile = 12864;
     else tile = 64;
     group = tile == 64 ? 8 : 4;
@@ -542,6 +545,9 @@ void launch_bf16(GemmBatch& b, hipStream_t s) {
   if (b.group <= 0) b.group = group;
   if (tile == 128) launch_bf16_tile<TA, TC, SPLIT, 128, 128>(b, s);
   else if (tile == 1288) launch_bf16_tile<TA, TC, SPLIT, 128, 128, 8>(b, s);
+  else if (tile == 641288 && !std::is_same<TA, __bf16>::value) {
+    if constexpr (!std::is_same<TA, __bf16>::value) launch_bf16_tile<TA, TC, SPLIT, 64, 128, 8>(b, s);
+  }
   else if (tile == 12864) launch_bf16_tile<TA, TC, SPLIT, 128, 64>(b, s);
   else launch_bf16_tile<TA, TC, SPLIT, 64, 64>(b, s);
 }
