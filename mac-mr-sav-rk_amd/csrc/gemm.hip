@@ -233,13 +233,15 @@ __device__ inline void store4<double>(double* p, const f4& v) {
 }
 
 // AMODE: how A reaches LDS.  0 = bf16 rows (16-byte loads of 8), 1 = fp32 rows with
-// lda % 4 == 0 and 16-byte aligned bases (float4 loads), 2 = one element per load.
+// lda % 4 == 0 and 16-byte aligned bases (float4 loads), 2 = one element per load,
+// 3 = fp64 rows with even lda and 16-byte aligned bases (two double2 loads per 4).
 template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
 __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
   using C = Cfg<BM, BN, SPLIT, NW>;
   constexpr int BK = C::BK, LDS = C::LDS, FM = C::FM, FN = C::FN, AE = C::AE, BE = C::BE, NT = C::NT;
-  static_assert(AE >= (AMODE == 0 ? 8 : AMODE == 1 ? 4 : 1) && BE >= 8, "tile too small for the block");
-  constexpr int AV = AMODE == 0 ? 8 : (AMODE == 1 ? 4 : 1);   // A elements per load
+  static_assert(AE >= (AMODE == 0 ? 8 : (AMODE == 1 || AMODE == 3) ? 4 : 1) && BE >= 8,
+                "tile too small for the block");
+  constexpr int AV = AMODE == 0 ? 8 : ((AMODE == 1 || AMODE == 3) ? 4 : 1);  // A elements per load group
   constexpr int NA = AE / AV;                                   // A loads per thread
   constexpr int NB = BE / 8;                                    // B loads per thread per part
   constexpr int APR = BK / AV;                                  // A loads per tile row
@@ -314,6 +316,11 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
       } else if constexpr (AMODE == 1) {
         const float4 f = *reinterpret_cast<const float4*>(a_ptr[e] + k0);
         st.ra[e * 4 + 0] = f.x; st.ra[e * 4 + 1] = f.y; st.ra[e * 4 + 2] = f.z; st.ra[e * 4 + 3] = f.w;
+      } else if constexpr (AMODE == 3) {
+        const double2 d0 = *reinterpret_cast<const double2*>(a_ptr[e] + k0);
+        const double2 d1 = *reinterpret_cast<const double2*>(a_ptr[e] + k0 + 2);
+        st.ra[e * 4 + 0] = (float)d0.x; st.ra[e * 4 + 1] = (float)d0.y;
+        st.ra[e * 4 + 2] = (float)d1.x; st.ra[e * 4 + 3] = (float)d1.y;
       } else {
         st.ra[e] = to_f(a_ptr[e][k0]);
       }
@@ -326,21 +333,7 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
   auto gload = [&](int kt, Stage& st) {
     const int k0 = kt * BK;
     if (!(ragged && kt == nk - 1)) {
-#pragma unroll
-      for (int e = 0; e < NA; ++e) {
-        if constexpr (AMODE == 0) {
-          st.rab[e] = *reinterpret_cast<const uint4*>(a_ptr[e] + k0);
-        } else if constexpr (AMODE == 1) {
-          const float4 f = *reinterpret_cast<const float4*>(a_ptr[e] + k0);
-          st.ra[e * 4 + 0] = f.x; st.ra[e * 4 + 1] = f.y; st.ra[e * 4 + 2] = f.z; st.ra[e * 4 + 3] = f.w;
-        } else {
-          st.ra[e] = to_f(a_ptr[e][k0]);
-        }
-      }
-#pragma unroll
-      for (int part = 0; part < C::PARTS; ++part)
-#pragma unroll
-        for (int e = 0; e < NB; ++e) st.rb[part][e] = *reinterpret_cast<const uint4*>(b_ptr[part][e] + k0);
+      gload_full(kt, st);
     } else {
       // Last, partial k-tile: zero beyond K (padding may hold anything).
 #pragma unroll
@@ -367,7 +360,7 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
       const int v = e * NT + tid, r = v / APR, c = (v % APR) * AV;
       if constexpr (AMODE == 0) {
         *reinterpret_cast<uint4*>(&as[lx(r, c)]) = st.rab[e];
-      } else if constexpr (AMODE == 1) {
+      } else if constexpr (AMODE == 1 || AMODE == 3) {
         typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
         bf4 hi, lo;
 #pragma unroll
@@ -516,7 +509,11 @@ void launch_bf16_tile(GemmBatch& b, hipStream_t s) {
     if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 1, NW>(b, total, s);
     else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW>(b, total, s);
   } else {
-    launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW>(b, total, s);
+    bool vec = true;
+    for (int i = 0; i < b.count; ++i)
+      vec = vec && (b.e[i].lda % 2 == 0) && (reinterpret_cast<uintptr_t>(b.e[i].A) % 16 == 0);
+    if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 3, NW>(b, total, s);
+    else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW>(b, total, s);
   }
 }
 
