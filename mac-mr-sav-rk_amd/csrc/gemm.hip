@@ -10,6 +10,8 @@
 //   k_gemm_f64  : v_mfma_f64_16x16x4f64, FP64 in / FP64 out (the plain FP64 solve)
 //   k_gemm_bf16 : v_mfma_f32_16x16x32_bf16, A converted to bf16 on the way into LDS,
 //                 FP32 accumulation, output fp64 / fp32 / bf16 (the mixed solve)
+#include <hip/hip_ext.h>
+
 #include <cstdint>
 #include <type_traits>
 
@@ -485,15 +487,29 @@ int tile_batch(GemmBatch& b, int bm, int bn) {
   return total;
 }
 
+hipEvent_t g_ev_start = nullptr, g_ev_stop = nullptr;
+
+// One launch of `kernel`, with the pending events (if any) recorded by the dispatch
+// itself: the interval is the kernel's, without the gap of separate event records.
+template <typename... Args>
+void launch(void (*kernel)(Args...), int blocks, int threads, hipStream_t s, Args... args) {
+  if (g_ev_start != nullptr) {
+    hipExtLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), 0, s, g_ev_start, g_ev_stop, 0, args...);
+    g_ev_start = g_ev_stop = nullptr;
+  } else {
+    kernel<<<blocks, threads, 0, s>>>(args...);
+  }
+}
+
 template <typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
 void launch_bf16_mode(GemmBatch& b, int total, hipStream_t s) {
   constexpr int nt = 64 * NW;
   switch (b.epi) {
-    case EPI_STORE: k_gemm_bf16<EPI_STORE, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
-    case EPI_SHIFT: k_gemm_bf16<EPI_SHIFT, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
-    case EPI_LAMBDA: k_gemm_bf16<EPI_LAMBDA, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
-    case EPI_ACCUM: k_gemm_bf16<EPI_ACCUM, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
-    default: k_gemm_bf16<EPI_COMBINE, TA, TC, SPLIT, BM, BN, AMODE, NW><<<total, nt, 0, s>>>(b); break;
+    case EPI_STORE: launch(k_gemm_bf16<EPI_STORE, TA, TC, SPLIT, BM, BN, AMODE, NW>, total, nt, s, (const GemmBatch)b); break;
+    case EPI_SHIFT: launch(k_gemm_bf16<EPI_SHIFT, TA, TC, SPLIT, BM, BN, AMODE, NW>, total, nt, s, (const GemmBatch)b); break;
+    case EPI_LAMBDA: launch(k_gemm_bf16<EPI_LAMBDA, TA, TC, SPLIT, BM, BN, AMODE, NW>, total, nt, s, (const GemmBatch)b); break;
+    case EPI_ACCUM: launch(k_gemm_bf16<EPI_ACCUM, TA, TC, SPLIT, BM, BN, AMODE, NW>, total, nt, s, (const GemmBatch)b); break;
+    default: launch(k_gemm_bf16<EPI_COMBINE, TA, TC, SPLIT, BM, BN, AMODE, NW>, total, nt, s, (const GemmBatch)b); break;
   }
 }
 
@@ -571,11 +587,16 @@ void gemm_f64(GemmBatch& b, hipStream_t s) {
   set_scalars(b);
   const int total = tile_batch(b, F64_BM, F64_BN);
   switch (b.epi) {
-    case EPI_STORE: k_gemm_f64<EPI_STORE><<<total, 256, 0, s>>>(b); break;
-    case EPI_SHIFT: k_gemm_f64<EPI_SHIFT><<<total, 256, 0, s>>>(b); break;
-    case EPI_LAMBDA: k_gemm_f64<EPI_LAMBDA><<<total, 256, 0, s>>>(b); break;
-    default: k_gemm_f64<EPI_COMBINE><<<total, 256, 0, s>>>(b); break;
+    case EPI_STORE: launch(k_gemm_f64<EPI_STORE>, total, 256, s, (const GemmBatch)b); break;
+    case EPI_SHIFT: launch(k_gemm_f64<EPI_SHIFT>, total, 256, s, (const GemmBatch)b); break;
+    case EPI_LAMBDA: launch(k_gemm_f64<EPI_LAMBDA>, total, 256, s, (const GemmBatch)b); break;
+    default: launch(k_gemm_f64<EPI_COMBINE>, total, 256, s, (const GemmBatch)b); break;
   }
+}
+
+void set_gemm_events(hipEvent_t start, hipEvent_t stop) {
+  g_ev_start = start;
+  g_ev_stop = stop;
 }
 
 void set_bf16_tiling(int tile, int group) {

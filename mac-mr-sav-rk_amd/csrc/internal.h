@@ -67,6 +67,9 @@ int tile_table_f64(GemmEntry* entries, int count, int* begins);
 // a_type: DT_F64 / DT_F32 / DT_BF16 for A and aux/C per c_type; Bt is always bf16.
 // split: three-product hi/lo form (A must be DT_F32 or DT_F64, Bt_lo set).
 void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s);
+// Events that the next gemm_f64 / gemm_bf16 launch records at its kernel's own start and
+// end (hipExtLaunchKernelGGL), consumed by that launch; null for plain launches.
+void set_gemm_events(hipEvent_t start, hipEvent_t stop);
 // Tile shape (0 auto, 64, 12864, 128) and tile-row grouping of the bf16 GEMM.
 void set_bf16_tiling(int tile, int group);
 
