@@ -186,7 +186,8 @@ struct Mailbox {
   double vals[4 + 64];
 };
 // cg_check (ncols <= 64), then the columns' active / iteration / converged state into mb.
-void cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox* mb, int seq, hipStream_t s);
+void cg_check_post(CgState* st, double* rr, int* flags, int ncols, Mailbox* mb, int seq, hipStream_t s,
+                   const double* rr_partials = nullptr, int nblk = 0);
 // mb->vals = [a[0..na), b[0..nb)] (na + nb <= 68), then mb->seq = seq.
 void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s);
 
@@ -198,6 +199,10 @@ void extrapolate(const double* a, const double* b, const double* c, const double
                  hipStream_t s);
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s);
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s);
+// The same steps from the unfinalized partials of (r, z) / (p, q) (nblk per column, as
+// the reductions leave them); out receives the finalized dots.
+void cg_rho_fin(CgState* st, const double* partials, int nblk, double* out, int ncols, hipStream_t s);
+void cg_alpha_fin(CgState* st, const double* partials, int nblk, double* out, int ncols, hipStream_t s);
 void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,
                  hipStream_t s);
 void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_t n, int64_t stride,

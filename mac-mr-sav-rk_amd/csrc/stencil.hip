@@ -99,16 +99,23 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ 
   if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
 }
 
+// One column's partial results combined in a fixed order (thread-strided, then
+// block_reduce): the value every reduction of this file reports, valid on thread 0.
+template <int MODE>
+__device__ inline double combine_partials(const double* part, int nblk, double* sh) {
+  double accum = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kThreads) {
+    const double v = part[i];
+    accum = MODE == RED_MAXABS ? fmax(accum, v) : accum + v;
+  }
+  return block_reduce<MODE>(accum, sh);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_finalize(const double* partials, int nblk, double* out) {
   const int c = blockIdx.x;
   __shared__ double sh[4];
-  double accum = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += kThreads) {
-    const double v = partials[(int64_t)c * nblk + i];
-    accum = MODE == RED_MAXABS ? fmax(accum, v) : accum + v;
-  }
-  accum = block_reduce<MODE>(accum, sh);
+  const double accum = combine_partials<MODE>(partials + (int64_t)c * nblk, nblk, sh);
   if (threadIdx.x == 0) out[c] = accum;
 }
 
@@ -205,7 +212,19 @@ __global__ void k_extrapolate(const double* __restrict__ a, const double* __rest
     out[i] = c != nullptr ? wa * a[i] + wb * b[i] + wc * c[i] : wa * a[i] + wb * b[i];
 }
 
-__global__ void k_cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox* mb, int seq) {
+// With partials, rr is first finalized from them (one column after another, in
+// k_finalize's order) by this block.
+__global__ __launch_bounds__(kThreads) void k_cg_check_post(CgState* st, double* rr, const double* partials,
+                                                            int nblk, int* flags, int ncols, Mailbox* mb,
+                                                            int seq) {
+  __shared__ double sh[4];
+  if (partials != nullptr) {
+    for (int k = 0; k < ncols; ++k) {
+      const double v = combine_partials<RED_SUM>(partials + (int64_t)k * nblk, nblk, sh);
+      if (threadIdx.x == 0) rr[k] = v;
+      __syncthreads();  // sh is reused; rr[k] is read by thread k below
+    }
+  }
   const int c = threadIdx.x;
   if (c < ncols) {
     if (st[c].active) {
@@ -232,6 +251,32 @@ __global__ void k_post_values(const double* a, int na, const double* b, int nb, 
   __threadfence_system();
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The CG scalar steps fused with the finalize of their dot product's partials.
+__global__ __launch_bounds__(kThreads) void k_cg_rho_fin(CgState* st, const double* partials, int nblk,
+                                                         double* out) {
+  __shared__ double sh[4];
+  const int c = blockIdx.x;
+  const double rz = combine_partials<RED_SUM>(partials + (int64_t)c * nblk, nblk, sh);
+  if (threadIdx.x != 0) return;
+  out[c] = rz;
+  if (!st[c].active) return;
+  st[c].rho = rz;
+  st[c].beta = st[c].iters > 0 ? rz / st[c].rho_prev : 0.0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_cg_alpha_fin(CgState* st, const double* partials, int nblk,
+                                                           double* out) {
+  __shared__ double sh[4];
+  const int c = blockIdx.x;
+  const double pq = combine_partials<RED_SUM>(partials + (int64_t)c * nblk, nblk, sh);
+  if (threadIdx.x != 0) return;
+  out[c] = pq;
+  if (!st[c].active) return;
+  st[c].alpha = st[c].rho / pq;
+  st[c].rho_prev = st[c].rho;
+  st[c].iters += 1;
 }
 
 __global__ void k_cg_rho(CgState* st, const double* rz, int ncols) {
@@ -451,8 +496,15 @@ void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s) {
   k_cg_check<<<(ncols + 63) / 64, 64, 0, s>>>(st, rr, flags, ncols);
 }
-void cg_check_post(CgState* st, const double* rr, int* flags, int ncols, Mailbox* mb, int seq, hipStream_t s) {
-  k_cg_check_post<<<1, 64, 0, s>>>(st, rr, flags, ncols, mb, seq);
+void cg_check_post(CgState* st, double* rr, int* flags, int ncols, Mailbox* mb, int seq, hipStream_t s,
+                   const double* rr_partials, int nblk) {
+  k_cg_check_post<<<1, kThreads, 0, s>>>(st, rr, rr_partials, nblk, flags, ncols, mb, seq);
+}
+void cg_rho_fin(CgState* st, const double* partials, int nblk, double* out, int ncols, hipStream_t s) {
+  k_cg_rho_fin<<<ncols, kThreads, 0, s>>>(st, partials, nblk, out);
+}
+void cg_alpha_fin(CgState* st, const double* partials, int nblk, double* out, int ncols, hipStream_t s) {
+  k_cg_alpha_fin<<<ncols, kThreads, 0, s>>>(st, partials, nblk, out);
 }
 void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s) {
   k_post_values<<<1, 128, 0, s>>>(a, na, b, nb, mb, seq);

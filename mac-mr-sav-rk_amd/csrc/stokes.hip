@@ -754,7 +754,9 @@ class Solver {
     std::vector<int> active;
     bool settled = false;  // every column left the loop through the convergence test
     for (int it = 0; it < opts_.max_iterations; ++it) {
-      cg_check_post(cg_, rr_, flags_, ncols, mailbox_dev_, ++seq_, stream_);
+      // From the second iteration on, (r, r) arrives as the update's partials.
+      cg_check_post(cg_, rr_, flags_, ncols, mailbox_dev_, ++seq_, stream_, it == 0 ? nullptr : partials_,
+                    reduce_blocks(np_));
       const volatile Mailbox* mb = wait_post(seq_);
       active.clear();
       for (int c = 0; c < ncols; ++c)
@@ -774,13 +776,13 @@ class Solver {
       if (mixed_) pressure_inverse_bf16(r_, z_, active, mass, visc);
       else pressure_inverse(r_, np_, z_, np_, active, true, mass, visc);
       reduce(z_, nullptr, np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(sums_));
-      sub_mean_dot(z_, np_, np_, ncols, sums_, r_, partials_, cg_, stream_, fin(sums2_));
-      cg_rho(cg_, sums2_, ncols, stream_);
+      sub_mean_dot(z_, np_, np_, ncols, sums_, r_, partials_, cg_, stream_);
+      cg_rho_fin(cg_, partials_, reduce_blocks(np_), sums2_, ncols, stream_);
       cg_update_p(p_, z_, np_, np_, ncols, cg_, stream_);
       // q = S p; alpha = rho / (p, q)
-      schur(p_, q_, active, mass, visc, p_, sums2_);
-      cg_alpha(cg_, sums2_, ncols, stream_);
-      cg_update_xr(x_, r_, p_, q_, np_, np_, ncols, cg_, partials_, stream_, fin(rr_));
+      schur(p_, q_, active, mass, visc, p_);
+      cg_alpha_fin(cg_, partials_, reduce_blocks(np_), sums2_, ncols, stream_);
+      cg_update_xr(x_, r_, p_, q_, np_, np_, ncols, cg_, partials_, stream_);
       if (accumulate_) accum_velocity(du_, wf_, size_, size_, ncols, cg_, nullptr, nullptr, stream_);
     }
     use_state_ = false;
