@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Stage-solve throughput of the MAC shifted-Stokes solve at 1024^2 on one MI355X.
 
-Workload (the headline configuration): MAC 1024x1024 on the unit square, nu = 1e-3,
+Workload: MAC 1024x1024 on the unit square, nu = 1e-3,
 sdirk3_mrsav with dt = 2e-4 from the seeded stream-function field with modes <= 32, so
 every stage solve has mass 1 and viscosity nu*dt*D[i,i] = 5e-8 and carries the two
 right-hand sides [stage rhs, convection] sharing one operator.  The stage solves of a
@@ -249,7 +249,7 @@ def replay(grid, rec, warmup: int, steps: int, precision: str, warm=None, profil
 
 
 def bench_run(args) -> dict:
-    """End-to-end sdirk3_mrsav run at n^2 (the headline configuration), timed over
+    """End-to-end sdirk3_mrsav run at n^2 (the stage-solve workload's run), timed over
     `--steps` steps after `--warmup` steps: the host model with the plain FP64 backend
     (the reference's arrangement, fields on the host), and the device-resident model with
     the plain FP64 and the mixed + history backends (fields stay in HBM)."""
