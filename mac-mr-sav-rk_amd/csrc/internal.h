@@ -203,6 +203,21 @@ void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s);
 // the reductions leave them); out receives the finalized dots.
 void cg_rho_fin(CgState* st, const double* partials, int nblk, double* out, int ncols, hipStream_t s);
 void cg_alpha_fin(CgState* st, const double* partials, int nblk, double* out, int ncols, hipStream_t s);
+// Mixed-CG steps that fold the mean removals of z and q into their consumers: rho from
+// sum(z), sum(z r), sum(r) (zmean out); p = (z - zmean) + beta p; alpha from the
+// three sums of divergence_sums3 (qmean out); x += alpha p, r -= alpha (q - qmean) with
+// (r, r) partials.  All over n-element columns of stride n.
+void mixed_rho(const double* z, const double* r, int64_t n, int ncols, CgState* st, double* partials,
+               double* zmean, hipStream_t s);
+void mixed_update_p(double* p, const double* z, const double* zmean, int64_t n, int ncols, const CgState* st,
+                    hipStream_t s);
+void mixed_alpha(const double* partials, int64_t n, int ncols, CgState* st, double* qmean, hipStream_t s);
+void mixed_update_xr(double* x, double* r, const double* p, const double* q, const double* qmean, int64_t n,
+                     int ncols, const CgState* st, double* partials, hipStream_t s);
+// out = sign * D vel (no mean removal) with partials of sum(out), sum(out y), sum(y).
+void divergence_sums3(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
+                      int64_t out_stride, double sign, const double* y, int ncols, double* partials,
+                      const CgState* st, hipStream_t s);
 void cg_update_p(double* p, const double* z, int64_t n, int64_t stride, int ncols, const CgState* st,
                  hipStream_t s);
 void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_t n, int64_t stride,

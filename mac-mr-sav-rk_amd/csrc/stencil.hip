@@ -49,6 +49,75 @@ __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, V
   }
 }
 
+// D on the velocity columns like k_div (no add), with three partial sums per block for
+// the mixed CG: sum(out), sum(out * y) and sum(y), at partials[(k * ncols + c) * gx + bx].
+template <typename TIN>
+__global__ __launch_bounds__(kThreads) void k_div_sums3(const TIN* __restrict__ vel, VelLayout L, GridDims g,
+                                                        double* __restrict__ out, int64_t out_stride, double sign,
+                                                        const double* __restrict__ y, double* partials,
+                                                        const CgState* st) {
+  const int c = blockIdx.y;
+  if (st != nullptr && !st[c].active) return;
+  __shared__ double sh[4];
+  const TIN* u = vel + (int64_t)c * L.stride;
+  const TIN* v = u + L.voff;
+  double* o = out + (int64_t)c * out_stride;
+  const double* yc = y + (int64_t)c * out_stride;
+  const int nx = g.nx, ny = g.ny;
+  const int np = nx * ny;
+  const SmallDiv dnx(nx);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int cell = blockIdx.x * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
+    const int j = dnx.quot(cell), i = cell - j * nx;
+    double value = 0.0;
+    if (i < nx - 1) value += ld_d(u, (int64_t)j * L.ldu + i) / g.hx;
+    if (i > 0) value -= ld_d(u, (int64_t)j * L.ldu + i - 1) / g.hx;
+    if (j < ny - 1) value += ld_d(v, (int64_t)j * L.ldv + i) / g.hy;
+    if (j > 0) value -= ld_d(v, (int64_t)(j - 1) * L.ldv + i) / g.hy;
+    value = sign * value;
+    o[cell] = value;
+    const double yv = yc[cell];
+    s0 += value;
+    s1 += value * yv;
+    s2 += yv;
+  }
+  const int64_t gx = gridDim.x, nc = gridDim.y;
+  s0 = block_reduce<RED_SUM>(s0, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) partials[(0 * nc + c) * gx + blockIdx.x] = s0;
+  s1 = block_reduce<RED_SUM>(s1, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) partials[(1 * nc + c) * gx + blockIdx.x] = s1;
+  s2 = block_reduce<RED_SUM>(s2, sh);
+  if (threadIdx.x == 0) partials[(2 * nc + c) * gx + blockIdx.x] = s2;
+}
+
+// sum(a), sum(a * b), sum(b) per column, partials as k_div_sums3.
+__global__ __launch_bounds__(kThreads) void k_sums3(const double* __restrict__ a, const double* __restrict__ b,
+                                                    int64_t n, int64_t stride, const CgState* st,
+                                                    double* partials) {
+  const int c = blockIdx.y;
+  if (!st[c].active) return;
+  __shared__ double sh[4];
+  const int64_t off = (int64_t)c * stride;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double av = a[off + i], bv = b[off + i];
+    s0 += av;
+    s1 += av * bv;
+    s2 += bv;
+  }
+  const int64_t gx = gridDim.x, nc = gridDim.y;
+  s0 = block_reduce<RED_SUM>(s0, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) partials[(0 * nc + c) * gx + blockIdx.x] = s0;
+  s1 = block_reduce<RED_SUM>(s1, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) partials[(1 * nc + c) * gx + blockIdx.x] = s1;
+  s2 = block_reduce<RED_SUM>(s2, sh);
+  if (threadIdx.x == 0) partials[(2 * nc + c) * gx + blockIdx.x] = s2;
+}
+
 template <typename TOUT>
 __global__ __launch_bounds__(kThreads) void k_dt(const double* __restrict__ p, int64_t p_stride,
                                                  const double* mean_sums, const double* __restrict__ add,
@@ -279,6 +348,85 @@ __global__ __launch_bounds__(kThreads) void k_cg_alpha_fin(CgState* st, const do
   st[c].iters += 1;
 }
 
+// Mixed-CG scalar steps on the unprojected vectors: from S_a = sum(a), S_ab = sum(a b),
+// S_b = sum(b) (partials of k_sums3 / k_div_sums3), mean = S_a / n and the projected dot
+// (a - mean, b) = S_ab - mean S_b.  rho from (z, r); alpha from (q, p).
+__device__ inline void sums3(const double* partials, int gx, int ncols, int c, double* sh, double* out3) {
+  for (int k = 0; k < 3; ++k) {
+    const double v = combine_partials<RED_SUM>(partials + ((int64_t)k * ncols + c) * gx, gx, sh);
+    if (threadIdx.x == 0) out3[k] = v;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_cg_rho_mean(CgState* st, const double* partials, int gx, int ncols,
+                                                          double n, double* mean_out) {
+  __shared__ double sh[4];
+  __shared__ double s3[3];
+  const int c = blockIdx.x;
+  sums3(partials, gx, ncols, c, sh, s3);
+  if (threadIdx.x != 0) return;
+  const double mean = s3[0] / n;
+  mean_out[c] = mean;
+  if (!st[c].active) return;
+  const double rz = s3[1] - mean * s3[2];
+  st[c].rho = rz;
+  st[c].beta = st[c].iters > 0 ? rz / st[c].rho_prev : 0.0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_cg_alpha_mean(CgState* st, const double* partials, int gx,
+                                                            int ncols, double n, double* mean_out) {
+  __shared__ double sh[4];
+  __shared__ double s3[3];
+  const int c = blockIdx.x;
+  sums3(partials, gx, ncols, c, sh, s3);
+  if (threadIdx.x != 0) return;
+  const double mean = s3[0] / n;
+  mean_out[c] = mean;
+  if (!st[c].active) return;
+  st[c].alpha = st[c].rho / (s3[1] - mean * s3[2]);
+  st[c].rho_prev = st[c].rho;
+  st[c].iters += 1;
+}
+
+// p = (z - zmean) + beta p (first iteration: z - zmean).
+__global__ __launch_bounds__(kThreads) void k_update_p_mean(double* __restrict__ p, const double* __restrict__ z,
+                                                            const double* __restrict__ zmean, int64_t n,
+                                                            int64_t stride, const CgState* st) {
+  const int c = blockIdx.y;
+  if (!st[c].active) return;
+  const bool first = st[c].iters == 0;
+  const double beta = st[c].beta, zm = zmean[c];
+  double* pc = p + (int64_t)c * stride;
+  const double* zc = z + (int64_t)c * stride;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double zv = zc[i] - zm;
+    pc[i] = first ? zv : pc[i] * beta + zv;
+  }
+}
+
+// x += alpha p; r -= alpha (q - qmean); partial sums of r r.
+__global__ __launch_bounds__(kThreads) void k_update_xr_mean(double* __restrict__ x, double* __restrict__ r,
+                                                             const double* __restrict__ p,
+                                                             const double* __restrict__ q,
+                                                             const double* __restrict__ qmean, int64_t n,
+                                                             int64_t stride, const CgState* st, double* partials) {
+  const int c = blockIdx.y;
+  if (!st[c].active) return;
+  __shared__ double sh[4];
+  const double alpha = st[c].alpha, qm = qmean[c];
+  const int64_t off = (int64_t)c * stride;
+  double accum = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    x[off + i] += alpha * p[off + i];
+    const double rv = r[off + i] - alpha * (q[off + i] - qm);
+    r[off + i] = rv;
+    accum += rv * rv;
+  }
+  accum = block_reduce<RED_SUM>(accum, sh);
+  if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
+}
+
 __global__ void k_cg_rho(CgState* st, const double* rz, int ncols) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols || !st[c].active) return;
@@ -499,6 +647,34 @@ void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t 
 void cg_check_post(CgState* st, double* rr, int* flags, int ncols, Mailbox* mb, int seq, hipStream_t s,
                    const double* rr_partials, int nblk) {
   k_cg_check_post<<<1, kThreads, 0, s>>>(st, rr, rr_partials, nblk, flags, ncols, mb, seq);
+}
+void mixed_rho(const double* z, const double* r, int64_t n, int ncols, CgState* st, double* partials,
+               double* zmean, hipStream_t s) {
+  const dim3 grid = col_grid(n, ncols);
+  k_sums3<<<grid, kThreads, 0, s>>>(z, r, n, n, st, partials);
+  k_cg_rho_mean<<<ncols, kThreads, 0, s>>>(st, partials, (int)grid.x, ncols, (double)n, zmean);
+}
+void mixed_update_p(double* p, const double* z, const double* zmean, int64_t n, int ncols, const CgState* st,
+                    hipStream_t s) {
+  k_update_p_mean<<<col_grid(n, ncols), kThreads, 0, s>>>(p, z, zmean, n, n, st);
+}
+void mixed_alpha(const double* partials, int64_t n, int ncols, CgState* st, double* qmean, hipStream_t s) {
+  k_cg_alpha_mean<<<ncols, kThreads, 0, s>>>(st, partials, reduce_blocks(n), ncols, (double)n, qmean);
+}
+void mixed_update_xr(double* x, double* r, const double* p, const double* q, const double* qmean, int64_t n,
+                     int ncols, const CgState* st, double* partials, hipStream_t s) {
+  k_update_xr_mean<<<col_grid(n, ncols), kThreads, 0, s>>>(x, r, p, q, qmean, n, n, st, partials);
+}
+void divergence_sums3(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
+                      int64_t out_stride, double sign, const double* y, int ncols, double* partials,
+                      const CgState* st, hipStream_t s) {
+  const dim3 grid = col_grid((int64_t)g.nx * g.ny, ncols);
+  if (in_type == DT_F64)
+    k_div_sums3<double><<<grid, kThreads, 0, s>>>((const double*)vel, L, g, out, out_stride, sign, y, partials, st);
+  else if (in_type == DT_F32)
+    k_div_sums3<float><<<grid, kThreads, 0, s>>>((const float*)vel, L, g, out, out_stride, sign, y, partials, st);
+  else
+    k_div_sums3<__bf16><<<grid, kThreads, 0, s>>>((const __bf16*)vel, L, g, out, out_stride, sign, y, partials, st);
 }
 void cg_rho_fin(CgState* st, const double* partials, int nblk, double* out, int ncols, hipStream_t s) {
   k_cg_rho_fin<<<ncols, kThreads, 0, s>>>(st, partials, nblk, out);

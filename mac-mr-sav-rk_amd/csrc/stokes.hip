@@ -772,9 +772,29 @@ class Solver {
         settled = true;
         break;
       }
+      if (mixed_) {
+        // The mixed inner CG folds the mean removals of z and q into their consumers:
+        // rho = (z - mean z, r) from one pass of sums, p = (z - mean z) + beta p, and
+        // alpha = rho / (q - mean q, p), r -= alpha (q - mean q) from sums the
+        // divergence accumulates (the same iteration in exact arithmetic, two fewer
+        // passes and four fewer launches).
+        double* zmean = means_;
+        double* qmean = means_ + kMaxCols;
+        pressure_inverse_bf16(r_, z_, active, mass, visc);
+        mixed_rho(z_, r_, np_, ncols, cg_, partials_, zmean, stream_);
+        mixed_update_p(p_, z_, zmean, np_, ncols, cg_, stream_);
+        const int dt_type = split_velocity() ? DT_F32 : DT_BF16;
+        void* dt_buf = split_velocity() ? static_cast<void*>(tfb_) : static_cast<void*>(tb_);
+        divergence_transpose(p_, np_, nullptr, nullptr, 0, dt_type, dt_buf, lay_b_, g_, ncols, cg_, stream_);
+        velocity_inverse_bf16(dt_buf, dt_type, lay_b_, wf_, DT_F32, size_, active, mass, visc);
+        divergence_sums3(DT_F32, wf_, packed_, g_, q_, np_, 1.0, p_, ncols, partials_, cg_, stream_);
+        mixed_alpha(partials_, np_, ncols, cg_, qmean, stream_);
+        mixed_update_xr(x_, r_, p_, q_, qmean, np_, ncols, cg_, partials_, stream_);
+        if (accumulate_) accum_velocity(du_, wf_, size_, size_, ncols, cg_, nullptr, nullptr, stream_);
+        continue;
+      }
       // z = mass (D D^T)^+ r + visc r, mean removed; rho = (r, z)
-      if (mixed_) pressure_inverse_bf16(r_, z_, active, mass, visc);
-      else pressure_inverse(r_, np_, z_, np_, active, true, mass, visc);
+      pressure_inverse(r_, np_, z_, np_, active, true, mass, visc);
       reduce(z_, nullptr, np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(sums_));
       sub_mean_dot(z_, np_, np_, ncols, sums_, r_, partials_, cg_, stream_);
       cg_rho_fin(cg_, partials_, reduce_blocks(np_), sums2_, ncols, stream_);
@@ -953,6 +973,7 @@ class Solver {
     const int nb = std::max(defect_blocks(size_, ncols), reduce_blocks(np_) * ncols);
     partials_ = (double*)take(sizeof(double) * (4 * (size_t)nb + 16));
     sums_ = (double*)take(sizeof(double) * (ncols + 4));
+    means_ = (double*)take(sizeof(double) * 2 * kMaxCols);
     sums2_ = (double*)take(sizeof(double) * (ncols + 4));
     sums3_ = (double*)take(sizeof(double) * (ncols + 4));
     rr_ = (double*)take(sizeof(double) * (ncols + 4));
@@ -1017,6 +1038,7 @@ class Solver {
   float *warm_x_ = nullptr, *warm_y_ = nullptr;
   double* warm_s_ = nullptr;
   double *partials_ = nullptr, *sums_ = nullptr, *sums2_ = nullptr, *sums3_ = nullptr, *rr_ = nullptr;
+  double* means_ = nullptr;  // mixed CG: column means of z and q (kMaxCols each)
   CgState* cg_ = nullptr;
   int* flags_ = nullptr;
   int* host_flags_ = nullptr;
