@@ -215,18 +215,18 @@ __global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__
   const SmallDiv dw(nx - 1), dnx(nx);
   double md = 0.0, mf = 0.0, mv = 0.0;
   for (int fidx = blockIdx.x * kThreads + threadIdx.x; fidx < S.size; fidx += S.nbv * kThreads) {
-    double kv, lift;
+    FaceRow k;
+    double lift;
     if (fidx < nu) {
       const int w = nx - 1;
       const int j = dw.quot(fidx), i = fidx - j * w;
       const double* u = vc;
       const double diag = 2.0 / hx2 + ((j == 0 || j == ny - 1) ? 3.0 : 2.0) / hy2;
-      kv = 0.0;
-      if (j > 0) kv += oy * u[fidx - w];
-      if (i > 0) kv += ox * u[fidx - 1];
-      kv += diag * u[fidx];
-      if (i < w - 1) kv += ox * u[fidx + 1];
-      if (j < ny - 1) kv += oy * u[fidx + w];
+      if (j > 0) k.add(oy, u[fidx - w]);
+      if (i > 0) k.add(ox, u[fidx - 1]);
+      k.add(diag, u[fidx]);
+      if (i < w - 1) k.add(ox, u[fidx + 1]);
+      if (j < ny - 1) k.add(oy, u[fidx + w]);
       const int left = j * nx + i;
       lift = (pc[left] - pc[left + 1]) / S.hx;
     } else {
@@ -234,18 +234,17 @@ __global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__
       const int j = dnx.quot(e), i = e - j * nx;
       const double* v = vc + nu;
       const double diag = ((i == 0 || i == nx - 1) ? 3.0 : 2.0) / hx2 + 2.0 / hy2;
-      kv = 0.0;
-      if (j > 0) kv += oy * v[e - nx];
-      if (i > 0) kv += ox * v[e - 1];
-      kv += diag * v[e];
-      if (i < nx - 1) kv += ox * v[e + 1];
-      if (j < ny - 2) kv += oy * v[e + nx];
+      if (j > 0) k.add(oy, v[e - nx]);
+      if (i > 0) k.add(ox, v[e - 1]);
+      k.add(diag, v[e]);
+      if (i < nx - 1) k.add(ox, v[e + 1]);
+      if (j < ny - 2) k.add(oy, v[e + nx]);
       const int lower = j * nx + i;
       lift = (pc[lower] - pc[lower + nx]) / S.hy;
     }
     const double fv = fc[fidx];
     const double vv = vc[fidx];
-    const double d = mass * vv + visc * kv - lift - fv;
+    const double d = face_defect(mass, vv, visc, k.kv, lift, fv);
     md = fmax(md, absn(d));
     mf = fmax(mf, absn(fv));
     mv = fmax(mv, absn(vv));

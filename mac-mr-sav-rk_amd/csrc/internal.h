@@ -106,6 +106,18 @@ enum Reduce : int { RED_NONE = 0, RED_SUM = 1, RED_MAXABS = 2 };
 // |x| with NaN mapped to +inf, so max-norm checks catch non-finite values.
 __device__ inline double absn(double x) { return x != x ? (double)INFINITY : fabs(x); }
 
+// One face of the FP64 defect mass u + visc (K u) + G p - f.  K u is the five-point row
+// summed south, west, centre, east, north (a neighbour's coefficient is -1/h^2; absent
+// neighbours pass nothing), with the FMAs written out, so every kernel that evaluates a
+// defect rounds alike whatever the compiler would contract.
+struct FaceRow {
+  double kv = 0.0;
+  __device__ inline void add(double coef, double value) { kv = fma(coef, value, kv); }
+};
+__device__ inline double face_defect(double mass, double vv, double visc, double kv, double lift, double fv) {
+  return (fma(visc, kv, mass * vv) - lift) - fv;
+}
+
 // n / d for 0 <= n < 2^31 and 1 <= d < 2^15, from one FP64 product with the reciprocal
 // (error far below 1, so the truncated quotient is off by at most one) and a correction
 // step: a few VALU operations instead of an integer division sequence.

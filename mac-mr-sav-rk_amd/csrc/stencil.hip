@@ -10,6 +10,16 @@
 namespace macs {
 namespace {
 
+// Workgroups go to the 8 XCDs in turn (block b to XCD b % 8), each XCD with its own L2.
+// Renumbered so that each XCD walks one contiguous eighth of the index range, a
+// stencil's neighbour rows stay in the L2 that loaded them instead of being fetched again
+// for another XCD.  A bijection on [0, nb): partials indexed by the logical block keep
+// every sum's order.
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned per = nb / 8;
+  return b < 8 * per ? (b % 8) * per + b / 8 : b;
+}
+
 __device__ inline double ld_d(const double* p, int64_t i) { return p[i]; }
 __device__ inline double ld_d(const float* p, int64_t i) { return (double)p[i]; }
 __device__ inline double ld_d(const __bf16* p, int64_t i) { return (double)(float)p[i]; }
@@ -29,8 +39,9 @@ __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, V
   const int nx = g.nx, ny = g.ny;
   const int np = nx * ny;
   const SmallDiv dnx(nx);
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   double accum = 0.0;
-  for (int cell = blockIdx.x * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
+  for (int cell = bx * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
     const int j = dnx.quot(cell), i = cell - j * nx;
     double value = 0.0;
     if (i < nx - 1) value += ld_d(u, (int64_t)j * L.ldu + i) / g.hx;
@@ -45,7 +56,7 @@ __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, V
   }
   if (MODE != RED_NONE) {
     accum = block_reduce<MODE>(accum, sh);
-    if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
+    if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + bx] = accum;
   }
 }
 
@@ -66,8 +77,9 @@ __global__ __launch_bounds__(kThreads) void k_div_sums3(const TIN* __restrict__ 
   const int nx = g.nx, ny = g.ny;
   const int np = nx * ny;
   const SmallDiv dnx(nx);
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int cell = blockIdx.x * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
+  for (int cell = bx * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
     const int j = dnx.quot(cell), i = cell - j * nx;
     double value = 0.0;
     if (i < nx - 1) value += ld_d(u, (int64_t)j * L.ldu + i) / g.hx;
@@ -84,12 +96,12 @@ __global__ __launch_bounds__(kThreads) void k_div_sums3(const TIN* __restrict__ 
   const int64_t gx = gridDim.x, nc = gridDim.y;
   s0 = block_reduce<RED_SUM>(s0, sh);
   __syncthreads();
-  if (threadIdx.x == 0) partials[(0 * nc + c) * gx + blockIdx.x] = s0;
+  if (threadIdx.x == 0) partials[(0 * nc + c) * gx + bx] = s0;
   s1 = block_reduce<RED_SUM>(s1, sh);
   __syncthreads();
-  if (threadIdx.x == 0) partials[(1 * nc + c) * gx + blockIdx.x] = s1;
+  if (threadIdx.x == 0) partials[(1 * nc + c) * gx + bx] = s1;
   s2 = block_reduce<RED_SUM>(s2, sh);
-  if (threadIdx.x == 0) partials[(2 * nc + c) * gx + blockIdx.x] = s2;
+  if (threadIdx.x == 0) partials[(2 * nc + c) * gx + bx] = s2;
 }
 
 // sum(a), sum(a * b), sum(b) per column, partials as k_div_sums3.
@@ -123,31 +135,33 @@ __global__ __launch_bounds__(kThreads) void k_dt(const double* __restrict__ p, i
                                                  const double* mean_sums, const double* __restrict__ add,
                                                  int64_t add_stride, TOUT* __restrict__ out, VelLayout L,
                                                  GridDims g, const CgState* st) {
+  // One thread per cell: it writes the u face east of the cell and the v face north of
+  // it, so every pressure value is fetched once (per face the same arithmetic as the
+  // reference loop).
   const int c = blockIdx.y;
   if (st != nullptr && !st[c].active) return;
-  const int nx = g.nx, ny = g.ny;
-  const int64_t nu = (int64_t)(nx - 1) * ny, nv = (int64_t)nx * (ny - 1);
-  const int64_t f = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (f >= nu + nv) return;
+  const int nx = g.nx, ny = g.ny, np = nx * ny;
+  const int cell = (int)xcd_block(blockIdx.x, gridDim.x) * kThreads + (int)threadIdx.x;
+  if (cell >= np) return;
+  const int64_t nu = (int64_t)(nx - 1) * ny;
   const double* pc = p + (int64_t)c * p_stride;
-  const double mean = mean_sums ? mean_sums[c] / (double)((int64_t)nx * ny) : 0.0;
+  const double* ac = add != nullptr ? add + (int64_t)c * add_stride : nullptr;
+  const double mean = mean_sums ? mean_sums[c] / (double)np : 0.0;
   TOUT* oc = out + (int64_t)c * L.stride;
-  double value;
-  int64_t dst;
-  if (f < nu) {
-    const int j = SmallDiv(nx - 1).quot((int)f), i = (int)f - j * (nx - 1);
-    const int64_t left = (int64_t)j * nx + i;
-    value = mean_sums ? ((pc[left] - mean) - (pc[left + 1] - mean)) / g.hx : (pc[left] - pc[left + 1]) / g.hx;
-    dst = (int64_t)j * L.ldu + i;
-  } else {
-    const int e = (int)(f - nu);
-    const int j = SmallDiv(nx).quot(e), i = e - j * nx;
-    const int64_t lower = (int64_t)j * nx + i;
-    value = mean_sums ? ((pc[lower] - mean) - (pc[lower + nx] - mean)) / g.hy : (pc[lower] - pc[lower + nx]) / g.hy;
-    dst = L.voff + (int64_t)j * L.ldv + i;
+  const int j = SmallDiv(nx).quot(cell), i = cell - j * nx;
+  const double pv = pc[cell];
+  if (i < nx - 1) {
+    const double pe = pc[cell + 1];
+    double value = mean_sums ? ((pv - mean) - (pe - mean)) / g.hx : (pv - pe) / g.hx;
+    if (ac != nullptr) value = ac[(int64_t)j * (nx - 1) + i] + value;
+    st_d(oc, (int64_t)j * L.ldu + i, value);
   }
-  if (add != nullptr) value = add[(int64_t)c * add_stride + f] + value;
-  st_d(oc, dst, value);
+  if (j < ny - 1) {
+    const double pn = pc[cell + nx];
+    double value = mean_sums ? ((pv - mean) - (pn - mean)) / g.hy : (pv - pn) / g.hy;
+    if (ac != nullptr) value = ac[nu + cell] + value;
+    st_d(oc, L.voff + (int64_t)j * L.ldv + i, value);
+  }
 }
 
 template <int MODE, bool DOT>
@@ -481,56 +495,62 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
                                                      const double* __restrict__ p, GridDims g, int ncols,
                                                      double mass, double visc, double* partials,
                                                      double* __restrict__ rout) {
+  // One thread per cell: the u face east of the cell and the v face north of it, so the
+  // pressure and both velocity components are each walked once, row by row.  Per face
+  // the arithmetic is the one of K u + G p - f term by term.
   __shared__ double sh[4];
-  const int nx = g.nx, ny = g.ny;
-  const int64_t nu = (int64_t)(nx - 1) * ny, nv = (int64_t)nx * (ny - 1), size = nu + nv;
-  const int64_t np = (int64_t)nx * ny;
+  const int nx = g.nx, ny = g.ny, w = nx - 1;
+  const int nu = w * ny, np = nx * ny;
+  const int64_t size = (int64_t)nu + (int64_t)nx * (ny - 1);
   const double hx2 = g.hx * g.hx, hy2 = g.hy * g.hy;
   const double ox = -1.0 / hx2, oy = -1.0 / hy2;
   double md = 0.0, mf = 0.0, mv = 0.0;
   // Faces of one column are < 2^31 (the grid caps nx, ny at 16384): 32-bit index math,
   // columns walked by the grid's y dimension.
   const int c = blockIdx.y;
-  const unsigned usize = (unsigned)size, unu = (unsigned)nu;
-  const double* vc = vel + (int64_t)c * size;
+  const double* u = vel + (int64_t)c * size;
+  const double* v = u + nu;
+  const double* fc = f + (int64_t)c * size;
   const double* pc = p + (int64_t)c * np;
-  const SmallDiv dw(nx - 1), dnx(nx);
-  for (unsigned fidx = blockIdx.x * kThreads + threadIdx.x; fidx < usize; fidx += gridDim.x * kThreads) {
-    double kv, lift;
-    if (fidx < unu) {
-      const unsigned w = nx - 1;
-      const int j = dw.quot((int)fidx), i = (int)(fidx - (unsigned)j * w);
-      const double* u = vc;
+  double* rc = rout != nullptr ? rout + (int64_t)c * size : nullptr;
+  const SmallDiv dnx(nx);
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
+  for (int cell = bx * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
+    const int j = dnx.quot(cell), i = cell - j * nx;
+    const double pv = pc[cell];
+    if (i < w) {
+      const int fu = j * w + i;
       const double diag = 2.0 / hx2 + ((j == 0 || j == ny - 1) ? 3.0 : 2.0) / hy2;
-      kv = 0.0;
-      if (j > 0) kv += oy * u[fidx - w];
-      if (i > 0) kv += ox * u[fidx - 1];
-      kv += diag * u[fidx];
-      if (i < w - 1) kv += ox * u[fidx + 1];
-      if (j < ny - 1) kv += oy * u[fidx + w];
-      const int64_t left = (int64_t)j * nx + i;
-      lift = (pc[left] - pc[left + 1]) / g.hx;
-    } else {
-      const unsigned e = fidx - unu;
-      const int j = dnx.quot((int)e), i = (int)(e - (unsigned)j * nx);
-      const double* v = vc + nu;
-      const double diag = ((i == 0 || i == nx - 1) ? 3.0 : 2.0) / hx2 + 2.0 / hy2;
-      kv = 0.0;
-      if (j > 0) kv += oy * v[e - nx];
-      if (i > 0) kv += ox * v[e - 1];
-      kv += diag * v[e];
-      if (i < nx - 1) kv += ox * v[e + 1];
-      if (j < ny - 2) kv += oy * v[e + nx];
-      const int64_t lower = (int64_t)j * nx + i;
-      lift = (pc[lower] - pc[lower + nx]) / g.hy;
+      FaceRow k;
+      if (j > 0) k.add(oy, u[fu - w]);
+      if (i > 0) k.add(ox, u[fu - 1]);
+      k.add(diag, u[fu]);
+      if (i < w - 1) k.add(ox, u[fu + 1]);
+      if (j < ny - 1) k.add(oy, u[fu + w]);
+      const double lift = (pv - pc[cell + 1]) / g.hx;
+      const double fv = fc[fu], vv = u[fu];
+      const double d = face_defect(mass, vv, visc, k.kv, lift, fv);
+      if (rc != nullptr) rc[fu] = -d;
+      md = fmax(md, absn(d));
+      mf = fmax(mf, absn(fv));
+      mv = fmax(mv, absn(vv));
     }
-    const double fv = f[(int64_t)c * size + fidx];
-    const double vv = vc[fidx];
-    const double d = mass * vv + visc * kv - lift - fv;
-    if (rout != nullptr) rout[(int64_t)c * size + fidx] = -d;
-    md = fmax(md, absn(d));
-    mf = fmax(mf, absn(fv));
-    mv = fmax(mv, absn(vv));
+    if (j < ny - 1) {
+      const double diag = ((i == 0 || i == nx - 1) ? 3.0 : 2.0) / hx2 + 2.0 / hy2;
+      FaceRow k;
+      if (j > 0) k.add(oy, v[cell - nx]);
+      if (i > 0) k.add(ox, v[cell - 1]);
+      k.add(diag, v[cell]);
+      if (i < nx - 1) k.add(ox, v[cell + 1]);
+      if (j < ny - 2) k.add(oy, v[cell + nx]);
+      const double lift = (pv - pc[cell + nx]) / g.hy;
+      const double fv = fc[nu + cell], vv = v[cell];
+      const double d = face_defect(mass, vv, visc, k.kv, lift, fv);
+      if (rc != nullptr) rc[nu + cell] = -d;
+      md = fmax(md, absn(d));
+      mf = fmax(mf, absn(fv));
+      mv = fmax(mv, absn(vv));
+    }
   }
   md = block_reduce<RED_MAXABS>(md, sh);
   __syncthreads();
@@ -539,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
   mv = block_reduce<RED_MAXABS>(mv, sh);
   const int nb = gridDim.x * gridDim.y;
   if (threadIdx.x == 0) {
-    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    const int b = blockIdx.y * gridDim.x + bx;
     partials[b] = md;
     partials[nb + b] = mf;
     partials[2 * nb + b] = mv;
@@ -585,8 +605,8 @@ void divergence(int in_type, const void* vel, const VelLayout& L, const GridDims
 void divergence_transpose(const double* p, int64_t p_stride, const double* mean_sums, const double* add,
                           int64_t add_stride, int out_type, void* out, const VelLayout& L, const GridDims& g,
                           int ncols, const CgState* st, hipStream_t s) {
-  const int64_t faces = (int64_t)(g.nx - 1) * g.ny + (int64_t)g.nx * (g.ny - 1);
-  const dim3 grid((unsigned)((faces + kThreads - 1) / kThreads), (unsigned)ncols);
+  const int64_t cells = (int64_t)g.nx * g.ny;
+  const dim3 grid((unsigned)((cells + kThreads - 1) / kThreads), (unsigned)ncols);
   if (out_type == DT_F64)
     k_dt<double><<<grid, kThreads, 0, s>>>(p, p_stride, mean_sums, add, add_stride, (double*)out, L, g, st);
   else if (out_type == DT_F32)
@@ -710,8 +730,8 @@ void defect_max(const double* vel, const double* f, const double* p, const GridD
   k_defect<<<col_grid(size, ncols), kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3,
                                                              residual_out);
   if (fin.out != nullptr) {
-    const int nb = defect_blocks(size, ncols);
-    for (int q = 0; q < 3; ++q) finalize(partials3 + q * nb, nb, 1, RED_MAXABS, fin.out + q, s);
+    // The three maxima (defect, f, u) at partials3 + q * nb, one block each.
+    finalize(partials3, defect_blocks(size, ncols), 3, RED_MAXABS, fin.out, s);
   }
 }
 

@@ -28,7 +28,7 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
 
 
-@pytest.mark.parametrize("shape", [(7, 5, 2.0, 1.0), (11, 7, 2.0, 0.7), (130, 97, 1.3, 0.8)])
+@pytest.mark.parametrize("shape", [(7, 5, 2.0, 1.0), (11, 7, 2.0, 0.7), (130, 97, 1.3, 0.8), (1100, 1000, 1.0, 1.0)])
 def test_divergence_and_transpose_are_bitwise_the_reference_kernels(shape):
     g = mo.Grid(*shape)
     rng = np.random.default_rng(79)
