@@ -184,10 +184,20 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ 
 
 // One column's partial results combined in a fixed order (thread-strided, then
 // block_reduce): the value every reduction of this file reports, valid on thread 0.
+// These run as one block per column with nothing else to hide memory latency, so the
+// loads go out eight at a time; each thread still adds its values in index order.
 template <int MODE>
 __device__ inline double combine_partials(const double* part, int nblk, double* sh) {
   double accum = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += kThreads) {
+  int i = threadIdx.x;
+  for (; i + 7 * kThreads < nblk; i += 8 * kThreads) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[i + k * kThreads];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) accum = MODE == RED_MAXABS ? fmax(accum, v[k]) : accum + v[k];
+  }
+  for (; i < nblk; i += kThreads) {
     const double v = part[i];
     accum = MODE == RED_MAXABS ? fmax(accum, v) : accum + v;
   }
