@@ -26,6 +26,26 @@ __device__ inline double ld_d(const __bf16* p, int64_t i) { return (double)(floa
 __device__ inline void st_d(double* p, int64_t i, double v) { p[i] = v; }
 __device__ inline void st_d(float* p, int64_t i, double v) { p[i] = (float)v; }
 __device__ inline void st_d(__bf16* p, int64_t i, double v) { p[i] = (__bf16)(float)v; }
+// D at one cell, term by term as the reference loop (+u_e/hx, -u_w/hx, +v_n/hy, -v_s/hy,
+// each divided before it is added).  The four face loads go out first at clamped
+// indices (an absent face is read from a valid address and left unused), so they
+// overlap instead of each waiting behind its own branch.
+template <typename TIN>
+__device__ inline double cell_divergence(const TIN* u, const TIN* v, const VelLayout& L, const GridDims& g, int j,
+                                         int i) {
+  const int nx = g.nx, ny = g.ny;
+  const double ue = ld_d(u, (int64_t)j * L.ldu + min(i, nx - 2));
+  const double uw = ld_d(u, (int64_t)j * L.ldu + max(i - 1, 0));
+  const double vn = ld_d(v, (int64_t)min(j, ny - 2) * L.ldv + i);
+  const double vs = ld_d(v, (int64_t)max(j - 1, 0) * L.ldv + i);
+  double value = 0.0;
+  if (i < nx - 1) value += ue / g.hx;
+  if (i > 0) value -= uw / g.hx;
+  if (j < ny - 1) value += vn / g.hy;
+  if (j > 0) value -= vs / g.hy;
+  return value;
+}
+
 template <typename TIN, int MODE>
 __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, VelLayout L, GridDims g,
                                                   double* __restrict__ out, int64_t out_stride, double sign,
@@ -43,13 +63,9 @@ __global__ __launch_bounds__(kThreads) void k_div(const TIN* __restrict__ vel, V
   double accum = 0.0;
   for (int cell = bx * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
     const int j = dnx.quot(cell), i = cell - j * nx;
-    double value = 0.0;
-    if (i < nx - 1) value += ld_d(u, (int64_t)j * L.ldu + i) / g.hx;
-    if (i > 0) value -= ld_d(u, (int64_t)j * L.ldu + i - 1) / g.hx;
-    if (j < ny - 1) value += ld_d(v, (int64_t)j * L.ldv + i) / g.hy;
-    if (j > 0) value -= ld_d(v, (int64_t)(j - 1) * L.ldv + i) / g.hy;
-    value = sign * value;
-    if (add != nullptr) value = add[(int64_t)c * out_stride + cell] + value;
+    const double a = add != nullptr ? add[(int64_t)c * out_stride + cell] : 0.0;
+    double value = sign * cell_divergence(u, v, L, g, j, i);
+    if (add != nullptr) value = a + value;
     o[cell] = value;
     if (MODE == RED_SUM) accum += value;
     if (MODE == RED_MAXABS) accum = fmax(accum, absn(value));
@@ -81,14 +97,9 @@ __global__ __launch_bounds__(kThreads) void k_div_sums3(const TIN* __restrict__ 
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   for (int cell = bx * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
     const int j = dnx.quot(cell), i = cell - j * nx;
-    double value = 0.0;
-    if (i < nx - 1) value += ld_d(u, (int64_t)j * L.ldu + i) / g.hx;
-    if (i > 0) value -= ld_d(u, (int64_t)j * L.ldu + i - 1) / g.hx;
-    if (j < ny - 1) value += ld_d(v, (int64_t)j * L.ldv + i) / g.hy;
-    if (j > 0) value -= ld_d(v, (int64_t)(j - 1) * L.ldv + i) / g.hy;
-    value = sign * value;
-    o[cell] = value;
     const double yv = yc[cell];
+    const double value = sign * cell_divergence(u, v, L, g, j, i);
+    o[cell] = value;
     s0 += value;
     s1 += value * yv;
     s2 += yv;
@@ -149,17 +160,18 @@ __global__ __launch_bounds__(kThreads) void k_dt(const double* __restrict__ p, i
   const double mean = mean_sums ? mean_sums[c] / (double)np : 0.0;
   TOUT* oc = out + (int64_t)c * L.stride;
   const int j = SmallDiv(nx).quot(cell), i = cell - j * nx;
-  const double pv = pc[cell];
+  // Loads first, at clamped indices (unused where the face does not exist).
+  const double pv = pc[cell], pe = pc[min(cell + 1, np - 1)], pn = pc[min(cell + nx, np - 1)];
+  const double au = ac != nullptr ? ac[(int64_t)j * (nx - 1) + min(i, nx - 2)] : 0.0;
+  const double av = ac != nullptr ? ac[nu + min(cell, np - nx - 1)] : 0.0;
   if (i < nx - 1) {
-    const double pe = pc[cell + 1];
     double value = mean_sums ? ((pv - mean) - (pe - mean)) / g.hx : (pv - pe) / g.hx;
-    if (ac != nullptr) value = ac[(int64_t)j * (nx - 1) + i] + value;
+    if (ac != nullptr) value = au + value;
     st_d(oc, (int64_t)j * L.ldu + i, value);
   }
   if (j < ny - 1) {
-    const double pn = pc[cell + nx];
     double value = mean_sums ? ((pv - mean) - (pn - mean)) / g.hy : (pv - pn) / g.hy;
-    if (ac != nullptr) value = ac[nu + cell] + value;
+    if (ac != nullptr) value = av + value;
     st_d(oc, L.voff + (int64_t)j * L.ldv + i, value);
   }
 }
@@ -210,6 +222,19 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const double* partials, i
   __shared__ double sh[4];
   const double accum = combine_partials<MODE>(partials + (int64_t)c * nblk, nblk, sh);
   if (threadIdx.x == 0) out[c] = accum;
+}
+
+// Two reductions' final combines in one launch: blocks [0, n1) combine (p1, len1) into
+// out1 as k_finalize does, the others (p2, len2) into out2.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_finalize2(const double* p1, int len1, int n1, double* out1,
+                                                        const double* p2, int len2, double* out2) {
+  __shared__ double sh[4];
+  const bool first = (int)blockIdx.x < n1;
+  const int c = first ? (int)blockIdx.x : (int)blockIdx.x - n1;
+  const int len = first ? len1 : len2;
+  const double accum = combine_partials<MODE>((first ? p1 : p2) + (int64_t)c * len, len, sh);
+  if (threadIdx.x == 0) (first ? out1 : out2)[c] = accum;
 }
 
 template <bool DOT>
@@ -504,20 +529,27 @@ __global__ __launch_bounds__(kThreads) void k_update_xr(double* __restrict__ x, 
 __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ vel, const double* __restrict__ f,
                                                      const double* __restrict__ p, GridDims g, int ncols,
                                                      double mass, double visc, double* partials,
-                                                     double* __restrict__ rout) {
+                                                     double* __restrict__ rout, int want_div,
+                                                     double* __restrict__ div_out, double div_sign) {
   // One thread per cell: the u face east of the cell and the v face north of it, so the
   // pressure and both velocity components are each walked once, row by row.  Per face
-  // the arithmetic is the one of K u + G p - f term by term.
+  // the arithmetic is the one of K u + G p - f term by term.  With want_div the thread
+  // also forms div_sign * (D u) at its cell from the same loads, term by term as k_div.
   __shared__ double sh[4];
   const int nx = g.nx, ny = g.ny, w = nx - 1;
-  const int nu = w * ny, np = nx * ny;
-  const int64_t size = (int64_t)nu + (int64_t)nx * (ny - 1);
+  const int nu = w * ny, nv = nx * (ny - 1), np = nx * ny;  // nx, ny >= 3
+  const int64_t size = (int64_t)nu + nv;
   const double hx2 = g.hx * g.hx, hy2 = g.hy * g.hy;
   const double ox = -1.0 / hx2, oy = -1.0 / hy2;
-  double md = 0.0, mf = 0.0, mv = 0.0;
+  // Diagonals of K: tangential walls (the u faces of the first and last rows, the v
+  // faces of the first and last columns) carry 3 / h^2 instead of 2 / h^2.
+  const double diag_u_in = 2.0 / hx2 + 2.0 / hy2, diag_u_wall = 2.0 / hx2 + 3.0 / hy2;
+  const double diag_v_in = 2.0 / hx2 + 2.0 / hy2, diag_v_wall = 3.0 / hx2 + 2.0 / hy2;
+  double md = 0.0, mf = 0.0, mv = 0.0, mdiv = 0.0;
   // Faces of one column are < 2^31 (the grid caps nx, ny at 16384): 32-bit index math,
   // columns walked by the grid's y dimension.
   const int c = blockIdx.y;
+  double* dc = div_out != nullptr ? div_out + (int64_t)c * np : nullptr;
   const double* u = vel + (int64_t)c * size;
   const double* v = u + nu;
   const double* fc = f + (int64_t)c * size;
@@ -527,39 +559,57 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
   const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   for (int cell = bx * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
     const int j = dnx.quot(cell), i = cell - j * nx;
-    const double pv = pc[cell];
-    if (i < w) {
-      const int fu = j * w + i;
-      const double diag = 2.0 / hx2 + ((j == 0 || j == ny - 1) ? 3.0 : 2.0) / hy2;
+    // Every load goes out first, at an index clamped into its array (a value whose face
+    // or neighbour does not exist is read from a valid address and left unused), so the
+    // loads overlap instead of each waiting behind its own branch.
+    const bool east = i < w, west = i > 0, south = j > 0, north = j < ny - 1;
+    const int fu = min(j * w + i, nu - 1);  // the u face east of the cell, when east
+    const int e = min(cell, nv - 1);        // the v face north of the cell, when north
+    const double uc = u[fu], uw = u[max(j * w + i - 1, 0)], ue = u[min(fu + 1, nu - 1)];
+    const double us = u[max(fu - w, 0)], un = u[min(fu + w, nu - 1)];
+    const double vc = v[e], vs = v[max(cell - nx, 0)], vw = v[max(e - 1, 0)];
+    const double ve = v[min(e + 1, nv - 1)], vn = v[min(e + nx, nv - 1)];
+    const double pv = pc[cell], pe = pc[min(cell + 1, np - 1)], pn = pc[min(cell + nx, np - 1)];
+    const double fu_v = fc[fu], fv_v = fc[nu + e];
+    if (east) {
+      const double diag = south && north ? diag_u_in : diag_u_wall;
       FaceRow k;
-      if (j > 0) k.add(oy, u[fu - w]);
-      if (i > 0) k.add(ox, u[fu - 1]);
-      k.add(diag, u[fu]);
-      if (i < w - 1) k.add(ox, u[fu + 1]);
-      if (j < ny - 1) k.add(oy, u[fu + w]);
-      const double lift = (pv - pc[cell + 1]) / g.hx;
-      const double fv = fc[fu], vv = u[fu];
-      const double d = face_defect(mass, vv, visc, k.kv, lift, fv);
+      if (south) k.add(oy, us);
+      if (west) k.add(ox, uw);
+      k.add(diag, uc);
+      if (i < w - 1) k.add(ox, ue);
+      if (north) k.add(oy, un);
+      const double lift = (pv - pe) / g.hx;
+      const double d = face_defect(mass, uc, visc, k.kv, lift, fu_v);
       if (rc != nullptr) rc[fu] = -d;
       md = fmax(md, absn(d));
-      mf = fmax(mf, absn(fv));
-      mv = fmax(mv, absn(vv));
+      mf = fmax(mf, absn(fu_v));
+      mv = fmax(mv, absn(uc));
     }
-    if (j < ny - 1) {
-      const double diag = ((i == 0 || i == nx - 1) ? 3.0 : 2.0) / hx2 + 2.0 / hy2;
+    if (north) {
+      const double diag = west && east ? diag_v_in : diag_v_wall;
       FaceRow k;
-      if (j > 0) k.add(oy, v[cell - nx]);
-      if (i > 0) k.add(ox, v[cell - 1]);
-      k.add(diag, v[cell]);
-      if (i < nx - 1) k.add(ox, v[cell + 1]);
-      if (j < ny - 2) k.add(oy, v[cell + nx]);
-      const double lift = (pv - pc[cell + nx]) / g.hy;
-      const double fv = fc[nu + cell], vv = v[cell];
-      const double d = face_defect(mass, vv, visc, k.kv, lift, fv);
-      if (rc != nullptr) rc[nu + cell] = -d;
+      if (south) k.add(oy, vs);
+      if (west) k.add(ox, vw);
+      k.add(diag, vc);
+      if (east) k.add(ox, ve);
+      if (j < ny - 2) k.add(oy, vn);
+      const double lift = (pv - pn) / g.hy;
+      const double d = face_defect(mass, vc, visc, k.kv, lift, fv_v);
+      if (rc != nullptr) rc[nu + e] = -d;
       md = fmax(md, absn(d));
-      mf = fmax(mf, absn(fv));
-      mv = fmax(mv, absn(vv));
+      mf = fmax(mf, absn(fv_v));
+      mv = fmax(mv, absn(vc));
+    }
+    if (want_div) {
+      double value = 0.0;
+      if (east) value += uc / g.hx;
+      if (west) value -= uw / g.hx;
+      if (north) value += vc / g.hy;
+      if (south) value -= vs / g.hy;
+      value = div_sign * value;
+      if (dc != nullptr) dc[cell] = value;
+      mdiv = fmax(mdiv, absn(value));
     }
   }
   md = block_reduce<RED_MAXABS>(md, sh);
@@ -567,12 +617,17 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
   mf = block_reduce<RED_MAXABS>(mf, sh);
   __syncthreads();
   mv = block_reduce<RED_MAXABS>(mv, sh);
+  if (want_div) {
+    __syncthreads();
+    mdiv = block_reduce<RED_MAXABS>(mdiv, sh);
+  }
   const int nb = gridDim.x * gridDim.y;
   if (threadIdx.x == 0) {
     const int b = blockIdx.y * gridDim.x + bx;
     partials[b] = md;
     partials[nb + b] = mf;
     partials[2 * nb + b] = mv;
+    if (want_div) partials[3 * nb + b] = mdiv;  // per column: [c * gx + bx]
   }
 }
 
@@ -735,14 +790,23 @@ void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_
 }
 
 void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols, double mass,
-                double visc, double* partials3, hipStream_t s, double* residual_out, Fin fin) {
+                double visc, double* partials3, hipStream_t s, double* residual_out, Fin fin, double* div_out,
+                double div_sign, Fin div_fin) {
   const int64_t size = (int64_t)(g.nx - 1) * g.ny + (int64_t)g.nx * (g.ny - 1);
-  k_defect<<<col_grid(size, ncols), kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3,
-                                                             residual_out);
-  if (fin.out != nullptr) {
-    // The three maxima (defect, f, u) at partials3 + q * nb, one block each.
-    finalize(partials3, defect_blocks(size, ncols), 3, RED_MAXABS, fin.out, s);
-  }
+  const dim3 grid = col_grid(size, ncols);
+  const int want_div = div_out != nullptr || div_fin.out != nullptr;
+  k_defect<<<grid, kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3, residual_out, want_div, div_out,
+                                     div_sign);
+  // The three maxima (defect, f, u) at partials3 + q * nb, one block each; the
+  // divergence's per-column maxima at partials3 + 3 nb + c * grid.x.
+  const int nb = defect_blocks(size, ncols);
+  if (fin.out != nullptr && div_fin.out != nullptr)
+    k_finalize2<RED_MAXABS><<<3 + ncols, kThreads, 0, s>>>(partials3, nb, 3, fin.out, partials3 + 3 * nb,
+                                                           (int)grid.x, div_fin.out);
+  else if (fin.out != nullptr)
+    finalize(partials3, nb, 3, RED_MAXABS, fin.out, s);
+  else if (div_fin.out != nullptr)
+    finalize(partials3 + 3 * nb, grid.x, ncols, RED_MAXABS, div_fin.out, s);
 }
 
 __global__ void k_add_cols(double* __restrict__ x, const double* __restrict__ y, int64_t total) {

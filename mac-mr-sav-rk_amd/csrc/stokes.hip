@@ -621,13 +621,11 @@ class Solver {
         out.warm = true;
       }
     }
-    const int nb = defect_blocks(size_, ncols);
     std::vector<int> inner(ncols);
     for (int sweep = 0;; ++sweep) {
       // FP64 residuals of the full system at the current iterate.
-      defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_, fin(sums_));
-      divergence(DT_F64, uacc_, packed_, g_, rc_, np_, -1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr,
-                 stream_, nullptr, fin(sums2_));
+      defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_, fin(sums_), rc_, -1.0,
+                 fin(sums2_));
       post_values(sums_, 3, sums2_, ncols, mailbox_dev_, ++seq_, stream_);
       const volatile Mailbox* mb = wait_post(seq_);
       out.residual = mb->vals[0] / (1.0 + mb->vals[1]);
@@ -833,10 +831,8 @@ class Solver {
       copy_cols(vel, vel_stride, w2_, size_, size_, ncols, stream_);
       vc = w2_;
     }
-    const int nb = defect_blocks(size_, ncols);
-    defect_max(vc, fc, p, g_, ncols, mass, visc, partials_, stream_, nullptr, fin(sums_));
-    divergence(DT_F64, vc, packed_, g_, q_, np_, 1.0, ncols, RED_MAXABS, partials_ + 3 * nb, nullptr, stream_,
-               nullptr, fin(sums2_));
+    defect_max(vc, fc, p, g_, ncols, mass, visc, partials_, stream_, nullptr, fin(sums_), nullptr, 1.0,
+               fin(sums2_));
     post_values(sums_, 3, sums2_, ncols, mailbox_dev_, ++seq_, stream_);
     const volatile Mailbox* mb = wait_post(seq_);
     out.residual = mb->vals[0] / (1.0 + mb->vals[1]);
