@@ -204,7 +204,14 @@ void cg_check_post(CgState* st, double* rr, int* flags, int ncols, Mailbox* mb, 
 void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s);
 
 // CG scalar steps (one thread per column).
-void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s);
+// rr_out (optional) also receives bb, the first recurrence residual (r, r).
+void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s,
+             double* rr_out = nullptr);
+// A mixed sweep's inner-CG start on b (n per column, contiguous): b -= sums[c] / n,
+// r = b, x = 0; (b, b) into bb[c] and max|b| into bmax[c] (partials: 2 * ncols *
+// reduce_blocks(n) entries).
+void sweep_start(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,
+                 double* bb, double* bmax, hipStream_t s);
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s);
 // out = w[0] a + w[1] b (+ w[2] c when c is given), over n elements.
 void extrapolate(const double* a, const double* b, const double* c, const double w[3], double* out, int64_t n,

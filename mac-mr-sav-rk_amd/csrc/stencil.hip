@@ -226,15 +226,42 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const double* partials, i
 
 // Two reductions' final combines in one launch: blocks [0, n1) combine (p1, len1) into
 // out1 as k_finalize does, the others (p2, len2) into out2.
-template <int MODE>
+template <int MODE1, int MODE2>
 __global__ __launch_bounds__(kThreads) void k_finalize2(const double* p1, int len1, int n1, double* out1,
                                                         const double* p2, int len2, double* out2) {
   __shared__ double sh[4];
   const bool first = (int)blockIdx.x < n1;
   const int c = first ? (int)blockIdx.x : (int)blockIdx.x - n1;
-  const int len = first ? len1 : len2;
-  const double accum = combine_partials<MODE>((first ? p1 : p2) + (int64_t)c * len, len, sh);
+  const double accum = first ? combine_partials<MODE1>(p1 + (int64_t)c * len1, len1, sh)
+                             : combine_partials<MODE2>(p2 + (int64_t)c * len2, len2, sh);
   if (threadIdx.x == 0) (first ? out1 : out2)[c] = accum;
+}
+
+// Start of a mixed sweep's inner CG on b: b -= mean(b) (sums[c] = sum of b), r = b,
+// x = 0, with partial (b, b) at partials[c * gx + bx] (the order of k_sub_mean_dot) and
+// partial max|b| at partials[(ncols + c) * gx + bx].
+__global__ __launch_bounds__(kThreads) void k_sweep_start(double* __restrict__ b, double* __restrict__ r,
+                                                          double* __restrict__ x, int64_t n, const double* sums,
+                                                          double* partials) {
+  const int c = blockIdx.y;
+  __shared__ double sh[4];
+  const int64_t off = (int64_t)c * n;
+  const double mean = sums[c] / (double)n;
+  double dot = 0.0, mx = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double v = b[off + i] - mean;
+    b[off + i] = v;
+    r[off + i] = v;
+    x[off + i] = 0.0;
+    dot += v * v;
+    mx = fmax(mx, absn(v));
+  }
+  const int64_t gx = gridDim.x, nc = gridDim.y;
+  dot = block_reduce<RED_SUM>(dot, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) partials[c * gx + blockIdx.x] = dot;
+  mx = block_reduce<RED_MAXABS>(mx, sh);
+  if (threadIdx.x == 0) partials[(nc + c) * gx + blockIdx.x] = mx;
 }
 
 template <bool DOT>
@@ -295,9 +322,11 @@ __global__ void k_copy_cols(const double* src, int64_t ss, double* dst, int64_t 
 
 // ---- CG scalars (scipy 1.15.3 cg loop: stop when ||r|| < rtol*||b||, tested at the
 // top of each iteration; rho = (r,z); beta = rho/rho_prev; alpha = rho/(p,q)).
-__global__ void k_cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols) {
+__global__ void k_cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols,
+                          double* rr_out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
+  if (rr_out != nullptr) rr_out[c] = bb[c];
   CgState s{};
   s.bnorm = sqrt(bb[c]);
   s.tol = rtol * s.bnorm;
@@ -723,8 +752,17 @@ void copy_cols(const double* src, int64_t ss, double* dst, int64_t ds, int64_t n
   k_copy_cols<<<col_grid(n, ncols), kThreads, 0, s>>>(src, ss, dst, ds, n);
 }
 
-void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s) {
-  k_cg_init<<<(ncols + 63) / 64, 64, 0, s>>>(st, bb, bmax, rtol, ncols);
+void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s,
+             double* rr_out) {
+  k_cg_init<<<(ncols + 63) / 64, 64, 0, s>>>(st, bb, bmax, rtol, ncols, rr_out);
+}
+void sweep_start(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,
+                 double* bb, double* bmax, hipStream_t s) {
+  const dim3 grid = col_grid(n, ncols);
+  k_sweep_start<<<grid, kThreads, 0, s>>>(b, r, x, n, sums, partials);
+  k_finalize2<RED_SUM, RED_MAXABS><<<2 * ncols, kThreads, 0, s>>>(partials, (int)grid.x, ncols, bb,
+                                                                  partials + (int64_t)ncols * grid.x,
+                                                                  (int)grid.x, bmax);
 }
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s) {
   k_cg_check<<<(ncols + 63) / 64, 64, 0, s>>>(st, rr, flags, ncols);
@@ -801,8 +839,9 @@ void defect_max(const double* vel, const double* f, const double* p, const GridD
   // divergence's per-column maxima at partials3 + 3 nb + c * grid.x.
   const int nb = defect_blocks(size, ncols);
   if (fin.out != nullptr && div_fin.out != nullptr)
-    k_finalize2<RED_MAXABS><<<3 + ncols, kThreads, 0, s>>>(partials3, nb, 3, fin.out, partials3 + 3 * nb,
-                                                           (int)grid.x, div_fin.out);
+    k_finalize2<RED_MAXABS, RED_MAXABS><<<3 + ncols, kThreads, 0, s>>>(partials3, nb, 3, fin.out,
+                                                                       partials3 + 3 * nb, (int)grid.x,
+                                                                       div_fin.out);
   else if (fin.out != nullptr)
     finalize(partials3, nb, 3, RED_MAXABS, fin.out, s);
   else if (div_fin.out != nullptr)

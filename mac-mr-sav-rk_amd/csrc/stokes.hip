@@ -641,12 +641,9 @@ class Solver {
       velocity_inverse_bf16(ru_, DT_F64, packed_, du_, DT_F32, size_, all, mass, visc);
       divergence(DT_F32, du_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_,
                  fin(sums_));
-      sub_mean_dot(b_, np_, np_, ncols, sums_, b_, partials_, nullptr, stream_, fin(sums2_));
-      reduce(b_, nullptr, np_, np_, ncols, RED_MAXABS, partials_, nullptr, stream_, fin(sums3_));
-      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_);
-      HIP_CHECK(hipMemsetAsync(x_, 0, sizeof(double) * np_ * ncols, stream_));
-      copy_cols(b_, np_, r_, np_, np_, ncols, stream_);
-      HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
+      // b -= mean, r = b, x = 0, with (b, b) and max|b| for the start test.
+      sweep_start(b_, r_, x_, np_, ncols, sums_, partials_, sums2_, sums3_, stream_);
+      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_, rr_);
       accumulate_ = true;
       pcg(ncols, mass, visc, inner, false);
       accumulate_ = false;
