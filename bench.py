@@ -88,7 +88,10 @@ def record_stage_solves(grid, count: int, seed: int = 0, modes: int = 32):
     original = backend.solve
 
     def recording(rhs, *, mass, viscosity, pressure_guess=None):
-        rec.append((torch.as_tensor(np.array(rhs), device="cuda:0"), mass, viscosity))
+        # Held as (size, k) views of one (k, size) tensor, the layout the device-resident
+        # model hands the solver (no transpose copy on the way in).
+        cols = np.ascontiguousarray(np.asarray(rhs).T)
+        rec.append((torch.as_tensor(cols, device="cuda:0").t(), mass, viscosity))
         return original(rhs, mass=mass, viscosity=viscosity, pressure_guess=pressure_guess)
 
     backend.solve = recording
