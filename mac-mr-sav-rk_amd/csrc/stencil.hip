@@ -124,7 +124,19 @@ __global__ __launch_bounds__(kThreads) void k_sums3(const double* __restrict__ a
   __shared__ double sh[4];
   const int64_t off = (int64_t)c * stride;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+  // Two elements per step with their loads issued first, summed in index order.
+  const int64_t step = (int64_t)gridDim.x * kThreads;
+  int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  for (; i + step < n; i += 2 * step) {
+    const double a0 = a[off + i], b0 = b[off + i], a1 = a[off + i + step], b1 = b[off + i + step];
+    s0 += a0;
+    s1 += a0 * b0;
+    s2 += b0;
+    s0 += a1;
+    s1 += a1 * b1;
+    s2 += b1;
+  }
+  for (; i < n; i += step) {
     const double av = a[off + i], bv = b[off + i];
     s0 += av;
     s1 += av * bv;
@@ -600,36 +612,30 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
     const double ve = v[min(e + 1, nv - 1)], vn = v[min(e + nx, nv - 1)];
     const double pv = pc[cell], pe = pc[min(cell + 1, np - 1)], pn = pc[min(cell + nx, np - 1)];
     const double fu_v = fc[fu], fv_v = fc[nu + e];
-    if (east) {
-      const double diag = south && north ? diag_u_in : diag_u_wall;
-      FaceRow k;
-      if (south) k.add(oy, us);
-      if (west) k.add(ox, uw);
-      k.add(diag, uc);
-      if (i < w - 1) k.add(ox, ue);
-      if (north) k.add(oy, un);
-      const double lift = (pv - pe) / g.hx;
-      const double d = face_defect(mass, uc, visc, k.kv, lift, fu_v);
-      if (rc != nullptr) rc[fu] = -d;
-      md = fmax(md, absn(d));
-      mf = fmax(mf, absn(fu_v));
-      mv = fmax(mv, absn(uc));
+    // Both faces are evaluated on every lane and enter the maxima through selects (a
+    // face that does not exist adds 0 to a maximum of magnitudes), so no load is sunk
+    // into a branch; only the stores are conditional.
+    FaceRow ku;
+    if (south) ku.add(oy, us);
+    if (west) ku.add(ox, uw);
+    ku.add(south && north ? diag_u_in : diag_u_wall, uc);
+    if (i < w - 1) ku.add(ox, ue);
+    if (north) ku.add(oy, un);
+    const double du_ = face_defect(mass, uc, visc, ku.kv, (pv - pe) / g.hx, fu_v);
+    FaceRow kv;
+    if (south) kv.add(oy, vs);
+    if (west) kv.add(ox, vw);
+    kv.add(west && east ? diag_v_in : diag_v_wall, vc);
+    if (east) kv.add(ox, ve);
+    if (j < ny - 2) kv.add(oy, vn);
+    const double dv_ = face_defect(mass, vc, visc, kv.kv, (pv - pn) / g.hy, fv_v);
+    if (rc != nullptr) {
+      if (east) rc[fu] = -du_;
+      if (north) rc[nu + e] = -dv_;
     }
-    if (north) {
-      const double diag = west && east ? diag_v_in : diag_v_wall;
-      FaceRow k;
-      if (south) k.add(oy, vs);
-      if (west) k.add(ox, vw);
-      k.add(diag, vc);
-      if (east) k.add(ox, ve);
-      if (j < ny - 2) k.add(oy, vn);
-      const double lift = (pv - pn) / g.hy;
-      const double d = face_defect(mass, vc, visc, k.kv, lift, fv_v);
-      if (rc != nullptr) rc[nu + e] = -d;
-      md = fmax(md, absn(d));
-      mf = fmax(mf, absn(fv_v));
-      mv = fmax(mv, absn(vc));
-    }
+    md = fmax(md, fmax(east ? absn(du_) : 0.0, north ? absn(dv_) : 0.0));
+    mf = fmax(mf, fmax(east ? absn(fu_v) : 0.0, north ? absn(fv_v) : 0.0));
+    mv = fmax(mv, fmax(east ? absn(uc) : 0.0, north ? absn(vc) : 0.0));
     if (want_div) {
       double value = 0.0;
       if (east) value += uc / g.hx;
