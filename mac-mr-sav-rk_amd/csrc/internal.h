@@ -248,9 +248,12 @@ void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_
 // With div_out and/or div_fin the same launch forms div_sign * (D vel) per cell, bitwise
 // as divergence(): written to div_out (np per column) when given, per-column max|.|
 // into div_fin.  partials3 then needs 4 * defect_blocks entries.
+// residual_out (optional) receives r_u = -defect as residual_type (DT_F64, DT_F32 or
+// DT_BF16) in residual_layout (the packed layout when null).
 void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols,
-                double mass, double visc, double* partials3, hipStream_t s, double* residual_out = nullptr,
-                Fin fin = {}, double* div_out = nullptr, double div_sign = 1.0, Fin div_fin = {});
+                double mass, double visc, double* partials3, hipStream_t s, void* residual_out = nullptr,
+                Fin fin = {}, double* div_out = nullptr, double div_sign = 1.0, Fin div_fin = {},
+                int residual_type = DT_F64, const VelLayout* residual_layout = nullptr);
 // x[c] += y[c] (FP64, contiguous columns).
 // du[c] += a_c w[c] (FP32) on the columns whose CG runs: a_c = alpha of the CG state
 // (gb == gq == nullptr), gb[c]/gq[c] (0 when gq[c] <= 0; the Galerkin start), or 1 (gb

@@ -567,10 +567,11 @@ __global__ __launch_bounds__(kThreads) void k_update_xr(double* __restrict__ x, 
 // defect = mass*vel + visc*(K vel) - D^T p - f on every free face (operators.py:6-17 for
 // K: 5-point, tangential wall diagonal 3, normal diagonal 2), max-abs partials of the
 // defect, of f and of vel (three per block).
+template <typename TR>
 __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ vel, const double* __restrict__ f,
                                                      const double* __restrict__ p, GridDims g, int ncols,
                                                      double mass, double visc, double* partials,
-                                                     double* __restrict__ rout, int want_div,
+                                                     TR* __restrict__ rout, VelLayout Lr, int want_div,
                                                      double* __restrict__ div_out, double div_sign) {
   // One thread per cell: the u face east of the cell and the v face north of it, so the
   // pressure and both velocity components are each walked once, row by row.  Per face
@@ -595,7 +596,7 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
   const double* v = u + nu;
   const double* fc = f + (int64_t)c * size;
   const double* pc = p + (int64_t)c * np;
-  double* rc = rout != nullptr ? rout + (int64_t)c * size : nullptr;
+  TR* rc = rout != nullptr ? rout + (int64_t)c * Lr.stride : nullptr;
   const SmallDiv dnx(nx);
   const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   for (int cell = bx * kThreads + threadIdx.x; cell < np; cell += gridDim.x * kThreads) {
@@ -629,9 +630,9 @@ __global__ __launch_bounds__(kThreads) void k_defect(const double* __restrict__ 
     if (east) kv.add(ox, ve);
     if (j < ny - 2) kv.add(oy, vn);
     const double dv_ = face_defect(mass, vc, visc, kv.kv, (pv - pn) / g.hy, fv_v);
-    if (rc != nullptr) {
-      if (east) rc[fu] = -du_;
-      if (north) rc[nu + e] = -dv_;
+    if (rc != nullptr) {  // -defect = r_u, in the output layout and type
+      if (east) st_d(rc, (int64_t)j * Lr.ldu + i, -du_);
+      if (north) st_d(rc, Lr.voff + (int64_t)j * Lr.ldv + i, -dv_);
     }
     md = fmax(md, fmax(east ? absn(du_) : 0.0, north ? absn(dv_) : 0.0));
     mf = fmax(mf, fmax(east ? absn(fu_v) : 0.0, north ? absn(fv_v) : 0.0));
@@ -834,13 +835,26 @@ void cg_update_xr(double* x, double* r, const double* p, const double* q, int64_
 }
 
 void defect_max(const double* vel, const double* f, const double* p, const GridDims& g, int ncols, double mass,
-                double visc, double* partials3, hipStream_t s, double* residual_out, Fin fin, double* div_out,
-                double div_sign, Fin div_fin) {
-  const int64_t size = (int64_t)(g.nx - 1) * g.ny + (int64_t)g.nx * (g.ny - 1);
+                double visc, double* partials3, hipStream_t s, void* residual_out, Fin fin, double* div_out,
+                double div_sign, Fin div_fin, int residual_type, const VelLayout* residual_layout) {
+  const int64_t nu = (int64_t)(g.nx - 1) * g.ny, size = nu + (int64_t)g.nx * (g.ny - 1);
   const dim3 grid = col_grid(size, ncols);
   const int want_div = div_out != nullptr || div_fin.out != nullptr;
-  k_defect<<<grid, kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3, residual_out, want_div, div_out,
-                                     div_sign);
+  VelLayout Lr{};  // packed unless given
+  Lr.ldu = g.nx - 1;
+  Lr.ldv = g.nx;
+  Lr.voff = nu;
+  Lr.stride = size;
+  if (residual_layout != nullptr) Lr = *residual_layout;
+  if (residual_type == DT_F32)
+    k_defect<float><<<grid, kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3, (float*)residual_out, Lr,
+                                              want_div, div_out, div_sign);
+  else if (residual_type == DT_BF16)
+    k_defect<__bf16><<<grid, kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3, (__bf16*)residual_out,
+                                               Lr, want_div, div_out, div_sign);
+  else
+    k_defect<double><<<grid, kThreads, 0, s>>>(vel, f, p, g, ncols, mass, visc, partials3, (double*)residual_out,
+                                               Lr, want_div, div_out, div_sign);
   // The three maxima (defect, f, u) at partials3 + q * nb, one block each; the
   // divergence's per-column maxima at partials3 + 3 nb + c * grid.x.
   const int nb = defect_blocks(size, ncols);

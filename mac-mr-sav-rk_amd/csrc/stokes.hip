@@ -563,7 +563,8 @@ class Solver {
 
   // MIXED: FP64 defect correction around bf16-transform inner solves.
   //   u = 0, p = guess (or 0)
-  //   repeat: r_u = f - (mass u + visc K u) + D^T p, r_c = -D u   (FP64)
+  //   repeat: r_u = f - (mass u + visc K u) + D^T p, r_c = -D u   (FP64; r_u kept in the
+  //           precision the transform reads, FP32 for the split form)
   //           stop when max|r_u|/(1+max|f|) <= refine_tol and max|D u| <= refine_tol
   //           W = L~^-1 r_u; b = r_c - D W; dp = PCG(S~, b, inner_tol); p += dp
   //           u += L~^-1 (r_u + D^T dp) = W + sum_k alpha_k w_k, where w_k = L~^-1 D^T p_k
@@ -622,10 +623,14 @@ class Solver {
       }
     }
     std::vector<int> inner(ncols);
+    // r_u goes straight into the transform's input buffer, in the precision the split
+    // (or plain bf16) transform keeps of it: FP32 holds every bit of its bf16 hi + lo.
+    void* ru_buf = split_velocity() ? static_cast<void*>(tfb_) : static_cast<void*>(tb_);
+    const int ru_type = split_velocity() ? DT_F32 : DT_BF16;
     for (int sweep = 0;; ++sweep) {
       // FP64 residuals of the full system at the current iterate.
-      defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_, fin(sums_), rc_, -1.0,
-                 fin(sums2_));
+      defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_buf, fin(sums_), rc_, -1.0,
+                 fin(sums2_), ru_type, &lay_b_);
       post_values(sums_, 3, sums2_, ncols, mailbox_dev_, ++seq_, stream_);
       const volatile Mailbox* mb = wait_post(seq_);
       out.residual = mb->vals[0] / (1.0 + mb->vals[1]);
@@ -638,7 +643,7 @@ class Solver {
         break;
       // Inner solve for the correction (dp, du).
       mixed_ = true;
-      velocity_inverse_bf16(ru_, DT_F64, packed_, du_, DT_F32, size_, all, mass, visc);
+      velocity_inverse_bf16(ru_buf, ru_type, lay_b_, du_, DT_F32, size_, all, mass, visc);
       divergence(DT_F32, du_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_,
                  fin(sums_));
       // b -= mean, r = b, x = 0, with (b, b) and max|b| for the start test.
@@ -952,7 +957,6 @@ class Solver {
     prev_ncols_ = 0;
     prev_b_ok_ = false;
     uacc_ = (double*)take(vb);
-    ru_ = (double*)take(vb);
     pacc_ = (double*)take(pb);
     rc_ = (double*)take(pb);
     wf_ = (float*)take(sizeof(float) * size_ * ncols);
@@ -998,7 +1002,7 @@ class Solver {
   bool use_state_ = false;
   double *w_ = nullptr, *w2_ = nullptr, *s1_ = nullptr, *s2_ = nullptr;
   double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;
-  double *fcomp_ = nullptr, *uacc_ = nullptr, *ru_ = nullptr, *pacc_ = nullptr, *rc_ = nullptr;
+  double *fcomp_ = nullptr, *uacc_ = nullptr, *pacc_ = nullptr, *rc_ = nullptr;
   float* wf_ = nullptr;
   static Fin fin(double* out) { return Fin{out}; }
   float* du_ = nullptr;       // mixed sweeps: velocity correction, accumulated in the CG
