@@ -209,9 +209,9 @@ void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int
              double* rr_out = nullptr);
 // A mixed sweep's inner-CG start on b (n per column, contiguous): b -= sums[c] / n,
 // r = b, x = 0; (b, b) into bb[c] and max|b| into bmax[c] (partials: 2 * ncols *
-// reduce_blocks(n) entries).
+// reduce_blocks(n) entries); r32 (optional) receives r rounded to FP32.
 void sweep_start(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,
-                 double* bb, double* bmax, hipStream_t s);
+                 double* bb, double* bmax, hipStream_t s, float* r32 = nullptr);
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s);
 // out = w[0] a + w[1] b (+ w[2] c when c is given), over n elements.
 void extrapolate(const double* a, const double* b, const double* c, const double w[3], double* out, int64_t n,
@@ -231,8 +231,9 @@ void mixed_rho(const double* z, const double* r, int64_t n, int ncols, CgState* 
 void mixed_update_p(double* p, const double* z, const double* zmean, int64_t n, int ncols, const CgState* st,
                     hipStream_t s);
 void mixed_alpha(const double* partials, int64_t n, int ncols, CgState* st, double* qmean, hipStream_t s);
+// r32 (optional) receives the updated r rounded to FP32.
 void mixed_update_xr(double* x, double* r, const double* p, const double* q, const double* qmean, int64_t n,
-                     int ncols, const CgState* st, double* partials, hipStream_t s);
+                     int ncols, const CgState* st, double* partials, hipStream_t s, float* r32 = nullptr);
 // out = sign * D vel (no mean removal) with partials of sum(out), sum(out y), sum(y).
 void divergence_sums3(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
                       int64_t out_stride, double sign, const double* y, int ncols, double* partials,

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void k_finalize2(const double* p1, int le
 // partial max|b| at partials[(ncols + c) * gx + bx].
 __global__ __launch_bounds__(kThreads) void k_sweep_start(double* __restrict__ b, double* __restrict__ r,
                                                           double* __restrict__ x, int64_t n, const double* sums,
-                                                          double* partials) {
+                                                          double* partials, float* __restrict__ r32) {
   const int c = blockIdx.y;
   __shared__ double sh[4];
   const int64_t off = (int64_t)c * n;
@@ -264,6 +264,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_start(double* __restrict__ b
     const double v = b[off + i] - mean;
     b[off + i] = v;
     r[off + i] = v;
+    if (r32 != nullptr) r32[off + i] = (float)v;
     x[off + i] = 0.0;
     dot += v * v;
     mx = fmax(mx, absn(v));
@@ -500,7 +501,8 @@ __global__ __launch_bounds__(kThreads) void k_update_xr_mean(double* __restrict_
                                                              const double* __restrict__ p,
                                                              const double* __restrict__ q,
                                                              const double* __restrict__ qmean, int64_t n,
-                                                             int64_t stride, const CgState* st, double* partials) {
+                                                             int64_t stride, const CgState* st, double* partials,
+                                                             float* __restrict__ r32) {
   const int c = blockIdx.y;
   if (!st[c].active) return;
   __shared__ double sh[4];
@@ -511,6 +513,7 @@ __global__ __launch_bounds__(kThreads) void k_update_xr_mean(double* __restrict_
     x[off + i] += alpha * p[off + i];
     const double rv = r[off + i] - alpha * (q[off + i] - qm);
     r[off + i] = rv;
+    if (r32 != nullptr) r32[off + i] = (float)rv;
     accum += rv * rv;
   }
   accum = block_reduce<RED_SUM>(accum, sh);
@@ -764,9 +767,9 @@ void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int
   k_cg_init<<<(ncols + 63) / 64, 64, 0, s>>>(st, bb, bmax, rtol, ncols, rr_out);
 }
 void sweep_start(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,
-                 double* bb, double* bmax, hipStream_t s) {
+                 double* bb, double* bmax, hipStream_t s, float* r32) {
   const dim3 grid = col_grid(n, ncols);
-  k_sweep_start<<<grid, kThreads, 0, s>>>(b, r, x, n, sums, partials);
+  k_sweep_start<<<grid, kThreads, 0, s>>>(b, r, x, n, sums, partials, r32);
   k_finalize2<RED_SUM, RED_MAXABS><<<2 * ncols, kThreads, 0, s>>>(partials, (int)grid.x, ncols, bb,
                                                                   partials + (int64_t)ncols * grid.x,
                                                                   (int)grid.x, bmax);
@@ -792,8 +795,8 @@ void mixed_alpha(const double* partials, int64_t n, int ncols, CgState* st, doub
   k_cg_alpha_mean<<<ncols, kThreads, 0, s>>>(st, partials, reduce_blocks(n), ncols, (double)n, qmean);
 }
 void mixed_update_xr(double* x, double* r, const double* p, const double* q, const double* qmean, int64_t n,
-                     int ncols, const CgState* st, double* partials, hipStream_t s) {
-  k_update_xr_mean<<<col_grid(n, ncols), kThreads, 0, s>>>(x, r, p, q, qmean, n, n, st, partials);
+                     int ncols, const CgState* st, double* partials, hipStream_t s, float* r32) {
+  k_update_xr_mean<<<col_grid(n, ncols), kThreads, 0, s>>>(x, r, p, q, qmean, n, n, st, partials, r32);
 }
 void divergence_sums3(int in_type, const void* vel, const VelLayout& L, const GridDims& g, double* out,
                       int64_t out_stride, double sign, const double* y, int ncols, double* partials,

@@ -232,7 +232,10 @@ class Solver {
   }
 
   // z = s0 * (D D^T)^+ r + s1 * r with bf16 transforms; r and z FP64 (np per column).
-  void pressure_inverse_bf16(const double* in, double* out, const std::vector<int>& cols, double s0, double s1) {
+  // in32 (optional): the same input rounded to FP32, for the first pass; the last pass's
+  // visc * in term reads the FP64 input either way.
+  void pressure_inverse_bf16(const double* in, double* out, const std::vector<int>& cols, double s0, double s1,
+                             const float* in32 = nullptr) {
     const bool sp = split_pressure();
     const int mid = sp ? DT_F32 : DT_BF16;
     const size_t mid_bytes = sp ? 4 : 2;
@@ -246,10 +249,12 @@ class Solver {
       for (size_t t = 0; t < count; ++t) {
         const int c = cols[base + t];
         const double* src = in + (int64_t)c * np_;
+        const void* a0 = in32 != nullptr ? static_cast<const void*>(in32 + (int64_t)c * np_)
+                                         : static_cast<const void*>(src);
         double* dst = out + (int64_t)c * np_;
         char* t1 = t1base + (int64_t)c * tstride_ * mid_bytes;
         char* t2 = t2base + (int64_t)c * tstride_ * mid_bytes;
-        add_pass_any(ga, src, B.nx, B.mx->qb, B.mx->ld8, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr,
+        add_pass_any(ga, a0, B.nx, B.mx->qb, B.mx->ld8, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr,
                      sp ? B.mx->qb_lo : nullptr);
         add_pass_any(gb, t1, ry, B.my->qb, B.my->ld8, t2, ry, B.nx, B.ny, B.ny, B.lam_x, B.lam_y,
                      sp ? B.my->qb_lo : nullptr);
@@ -263,7 +268,7 @@ class Solver {
       gb.epi = EPI_LAMBDA; gb.trans_c = 0; gb.thresh = p_thresh_;
       gc.epi = EPI_STORE; gc.trans_c = 1;
       gd.epi = EPI_COMBINE; gd.trans_c = 0; gd.s0 = s0; gd.s1 = s1;
-      run_gemm(ga, DT_F64, mid, true, sp);
+      run_gemm(ga, in32 != nullptr ? DT_F32 : DT_F64, mid, true, sp);
       run_gemm(gb, mid, mid, true, sp);
       run_gemm(gc, mid, mid, true, sp);
       run_gemm(gd, mid, DT_F64, true, sp);
@@ -647,11 +652,13 @@ class Solver {
       divergence(DT_F32, du_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_,
                  fin(sums_));
       // b -= mean, r = b, x = 0, with (b, b) and max|b| for the start test.
-      sweep_start(b_, r_, x_, np_, ncols, sums_, partials_, sums2_, sums3_, stream_);
+      sweep_start(b_, r_, x_, np_, ncols, sums_, partials_, sums2_, sums3_, stream_, tfb_);
+      r32_ = tfb_;
       cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_, rr_);
       accumulate_ = true;
       pcg(ncols, mass, visc, inner, false);
       accumulate_ = false;
+      r32_ = nullptr;
       for (int c = 0; c < ncols; ++c) out.iterations[c] += inner[c];
       add_cols(pacc_, x_, np_, ncols, stream_);
       // du = L~^-1 (r_u + D^T dp) = W + sum_k alpha_k L~^-1 D^T p_k, already summed in du_.
@@ -780,7 +787,7 @@ class Solver {
         // passes and four fewer launches).
         double* zmean = means_;
         double* qmean = means_ + kMaxCols;
-        pressure_inverse_bf16(r_, z_, active, mass, visc);
+        pressure_inverse_bf16(r_, z_, active, mass, visc, r32_);
         mixed_rho(z_, r_, np_, ncols, cg_, partials_, zmean, stream_);
         mixed_update_p(p_, z_, zmean, np_, ncols, cg_, stream_);
         const int dt_type = split_velocity() ? DT_F32 : DT_BF16;
@@ -789,7 +796,7 @@ class Solver {
         velocity_inverse_bf16(dt_buf, dt_type, lay_b_, wf_, DT_F32, size_, active, mass, visc);
         divergence_sums3(DT_F32, wf_, packed_, g_, q_, np_, 1.0, p_, ncols, partials_, cg_, stream_);
         mixed_alpha(partials_, np_, ncols, cg_, qmean, stream_);
-        mixed_update_xr(x_, r_, p_, q_, qmean, np_, ncols, cg_, partials_, stream_);
+        mixed_update_xr(x_, r_, p_, q_, qmean, np_, ncols, cg_, partials_, stream_, r32_);
         if (accumulate_) accum_velocity(du_, wf_, size_, size_, ncols, cg_, nullptr, nullptr, stream_);
         continue;
       }
@@ -1011,6 +1018,10 @@ class Solver {
   float *tf1_ = nullptr, *tf2_ = nullptr, *tfb_ = nullptr;
   int64_t tstride_ = 0, toff_v_ = 0;
   VelLayout lay_b_{};
+  // During a mixed sweep's inner CG: r rounded to FP32 (np per column), kept in tfb_
+  // between the kernels that update r and the next pressure inverse (tfb_ takes D^T p
+  // only after that inverse has read it).
+  float* r32_ = nullptr;
   bool mixed_ = false;
   void* cnn_ = nullptr;
   double cnn_alpha_ = 1.0;
