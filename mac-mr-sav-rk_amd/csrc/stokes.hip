@@ -391,19 +391,21 @@ class Solver {
   const double* schur_rhs() const { return b_; }
   int last_ncols() const { return ncols_cur_; }
 
-  // Fills io_g_ with the warm start's pressure guess (previous solve, history, or the
-  // network's correction of a base guess), if there is one.  The network reads the
+  // Fills out (io_g_ unless given) with the warm start's pressure guess (previous solve,
+  // history, or the network's correction of a base guess), if there is one; nothing is
+  // written when there is none.  The network reads the
   // base's residual, b - b_base (the Schur right-hand sides recorded with the base's
   // pressures, combined like them): with one operator S p_m = b_m, so the correction
   // it predicts, p - base, solves S (p - base) = b - b_base.
-  bool learned_guess(const double* b, int ncols) {
+  bool learned_guess(const double* b, int ncols, double* out = nullptr) {
+    if (out == nullptr) out = io_g_;
     if (warm_mode_ == MAC_WARM_PREVIOUS) {
       // Previous solve's pressures as the guess (same column count only).
       if (prev_ncols_ != ncols) return false;
-      copy_cols(pprev_, np_, io_g_, np_, np_, ncols, stream_);
+      copy_cols(pprev_, np_, out, np_, np_, ncols, stream_);
       return true;
     }
-    if (warm_mode_ == MAC_WARM_HISTORY) return history_guess(ncols, io_g_);
+    if (warm_mode_ == MAC_WARM_HISTORY) return history_guess(ncols, out);
     if (warm_mode_ != MAC_WARM_NETWORK || cnn_ == nullptr || !warm_shape_ok(g_.nx, g_.ny)) return false;
     // The network corrects a base guess: the history extrapolation when a period is set,
     // else the previous solve's pressures.  Without a base there is no guess (zero start),
@@ -420,7 +422,7 @@ class Solver {
     if (base == nullptr) return false;
     const double w[3] = {1.0, -1.0, 0.0};
     extrapolate(b, warm_db_, nullptr, w, warm_db_, n, stream_);  // b - b_base
-    warm_guess(cnn_, warm_db_, np_, base, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, io_g_,
+    warm_guess(cnn_, warm_db_, np_, base, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, out,
                np_, stream_);
     return true;
   }
@@ -600,8 +602,8 @@ class Solver {
     bool bzero_ok = false;  // bzero_ holds this solve's zero-start Schur right-hand side
     if (guess) {
       copy_cols(guess, guess_stride, pacc_, np_, np_, ncols, stream_);
-    } else if (warm_mode_ != MAC_WARM_OFF && warm_mode_ != MAC_WARM_NETWORK && learned_guess(nullptr, ncols)) {
-      copy_cols(io_g_, np_, pacc_, np_, np_, ncols, stream_);
+    } else if (warm_mode_ != MAC_WARM_OFF && warm_mode_ != MAC_WARM_NETWORK &&
+               learned_guess(nullptr, ncols, pacc_)) {
       // With a history, the velocities start from their own extrapolation too: a good
       // pressure with u = 0 leaves an O(1) velocity defect, whose bf16 correction (after
       // D) would swamp the pressure's accuracy.
@@ -621,8 +623,7 @@ class Solver {
       mixed_ = false;
       copy_cols(b_, np_, bzero_, np_, np_, ncols, stream_);  // the sweeps reuse b_
       bzero_ok = true;
-      if (learned_guess(b_, ncols)) {
-        copy_cols(io_g_, np_, pacc_, np_, np_, ncols, stream_);
+      if (learned_guess(b_, ncols, pacc_)) {
         if (history_period_ > 0) history_velocity(ncols, uacc_);
         out.warm = true;
       }
