@@ -55,7 +55,11 @@ typedef struct mac_stokes_opts {
   int32_t mixed_split;     /* MIXED: bit 0 -> velocity inverses, bit 1 -> pressure
                               preconditioner use the split bf16x3 GEMM (hi/lo parts
                               of data and matrix, ~16-bit operands; default 3)      */
-  int32_t pad0;
+  int32_t fp64_refine;     /* FP64: after the reference solve, up to this many FP64
+                              defect-correction sweeps (inner Schur-CG rtol
+                              inner_tolerance) until the residual and max|D u| are
+                              both <= refine_tolerance; 0 (default) = the reference
+                              algorithm alone                                        */
 } mac_stokes_opts;
 
 typedef struct mac_solve_report {
@@ -67,6 +71,8 @@ typedef struct mac_solve_report {
   int32_t total_iterations; /* Schur-CG iterations summed over columns and sweeps  */
   int32_t warm_started;   /* 1 if the attached network supplied the CG start        */
   int32_t pad0;
+  double first_residual;   /* MIXED / refined FP64: residual and max|D u| of the first */
+  double first_divergence; /* FP64 defect (the starting iterate's)                     */
 } mac_solve_report;
 
 /* Fills `opts` with the reference defaults (tensor_stokes.py:128-136). */
@@ -196,6 +202,11 @@ int32_t mac_grid_divergence(int32_t nx, int32_t ny, double lx, double ly, const 
 /* sum a*b (fixed-order two-level reduction) and max|a| (NaN reported as inf). */
 int32_t mac_dot(int64_t n, const double* a, const double* b, double* result);
 int32_t mac_max_abs(int64_t n, const double* a, double* result);
+/* The same two reductions without waiting: mode 0 writes sum a*b, mode 1 max|a|, into
+ * the device double *out, in the null stream's order (the reference computes these
+ * inline, solver/mac/grid.py:67-68 and the schemes' max-norm gates,
+ * solver/schemes/sdirk3_mrsav.py:35-57); callers read many such slots in one copy. */
+int32_t mac_reduce_into(int64_t n, const double* a, const double* b, int32_t mode, double* out);
 
 /* ---- batched cavities ----
  * Replaces a host loop of `TensorStokes.solve` (tensor_stokes.py:245-364) over many

@@ -302,4 +302,18 @@ int32_t mac_max_abs(int64_t n, const double* a, double* result) {
   });
 }
 
+int32_t mac_reduce_into(int64_t n, const double* a, const double* b, int32_t mode, double* out) {
+  return model_guard([&] {
+    if (n < 1 || !a || !out || mode < 0 || mode > 1 || (mode == 0 && !b))
+      throw ArgFailure("Invalid reduction arguments");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    const int nb = reduce_blocks(n);
+    const int red = mode == 0 ? RED_SUM : RED_MAXABS;
+    double* part = scratch(nb);
+    reduce(a, mode == 0 ? b : nullptr, n, n, 1, red, part, nullptr, 0);
+    finalize(part, nb, 1, red, out, 0);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
 }  // extern "C"

@@ -365,6 +365,7 @@ class Solver {
     std::vector<int> iterations;
     int sweeps = 1;
     bool warm = false;
+    double first_residual = 0, first_divergence = 0;  // of the first FP64 defect (mixed / refined)
   };
 
   // ------------------------------------------------------- learned warm start
@@ -559,6 +560,10 @@ class Solver {
     // p = x - mean(x)
     reduce(x_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_, fin(sums_));
     sub_mean_dot(x_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
+    if (opts_.fp64_refine > 0) {
+      refine_fp64(f, f_stride, ncols, mass, visc, vel, vel_stride, pres, pres_stride, out);
+      return out;
+    }
     copy_cols(x_, np_, pres, pres_stride, np_, ncols, stream_);
     remember_pressure(x_, ncols, nullptr, b_);
     // u = L^-1 (f + D^T p)
@@ -566,6 +571,69 @@ class Solver {
     velocity_inverse(w_, size_, vel, vel_stride, all, mass, visc);
     checks(f, f_stride, vel, vel_stride, x_, ncols, mass, visc, out);
     return out;
+  }
+
+  // FP64 defect correction after the reference solve (p in x_, mean-zero): with the
+  // FP64 residuals r_u = f - (mass u + visc K u) + D^T p and r_c = -D u of the current
+  // iterate, each sweep solves the Stokes system for the correction exactly as the
+  // reference solves for u and p, (W = L^-1 r_u, S dp = r_c - D W by the Schur CG to
+  // inner_tolerance, du = L^-1 (r_u + D^T dp)), all in FP64.  Exact operators make each
+  // sweep contract the defect by about the inner tolerance; the loop stops once the
+  // reference's residual and max|D u| are both <= refine_tolerance.
+  void refine_fp64(const double* f, int64_t f_stride, int ncols, double mass, double visc, double* vel,
+                   int64_t vel_stride, double* pres, int64_t pres_stride, Outcome& out) {
+    std::vector<int> all(ncols), inner(ncols);
+    for (int c = 0; c < ncols; ++c) all[c] = c;
+    const double* fc = f;
+    if (f_stride != size_) {
+      copy_cols(f, f_stride, fcomp_, size_, size_, ncols, stream_);
+      fc = fcomp_;
+    }
+    const double* b0 = b_;
+    copy_cols(b0, np_, bzero_, np_, np_, ncols, stream_);  // the zero start's b, for the history
+    copy_cols(x_, np_, pacc_, np_, np_, ncols, stream_);
+    divergence_transpose(pacc_, np_, nullptr, fc, size_, DT_F64, w_, packed_, g_, ncols, nullptr, stream_);
+    velocity_inverse(w_, size_, uacc_, size_, all, mass, visc);
+    out.sweeps = 1;
+    for (int sweep = 0;; ++sweep) {
+      defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru64_, fin(sums_), rc_, -1.0,
+                 fin(sums2_), DT_F64, nullptr);
+      post_values(sums_, 3, sums2_, ncols, mailbox_dev_, ++seq_, stream_);
+      const volatile Mailbox* mb = wait_post(seq_);
+      out.residual = mb->vals[0] / (1.0 + mb->vals[1]);
+      out.fmax = mb->vals[1];
+      out.vmax = mb->vals[2];
+      out.divergence = 0.0;
+      for (int c = 0; c < ncols; ++c) out.divergence = std::max(out.divergence, (double)mb->vals[3 + c]);
+      if (sweep == 0) {
+        out.first_residual = out.residual;
+        out.first_divergence = out.divergence;
+      }
+      const bool done = out.residual <= opts_.refine_tolerance && out.divergence <= opts_.refine_tolerance;
+      if (done || sweep >= opts_.fp64_refine || !std::isfinite(out.residual) || !std::isfinite(out.divergence))
+        break;
+      velocity_inverse(ru64_, size_, w_, size_, all, mass, visc);  // W = L^-1 r_u
+      divergence(DT_F64, w_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_,
+                 fin(sums_));                                      // b = r_c - D W
+      sub_mean_dot(b_, np_, np_, ncols, sums_, b_, partials_, nullptr, stream_, fin(sums2_));
+      reduce(b_, nullptr, np_, np_, ncols, RED_MAXABS, partials_, nullptr, stream_, fin(sums3_));
+      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_);
+      HIP_CHECK(hipMemsetAsync(x_, 0, sizeof(double) * np_ * ncols, stream_));
+      copy_cols(b_, np_, r_, np_, np_, ncols, stream_);
+      HIP_CHECK(hipMemcpyAsync(rr_, sums2_, sizeof(double) * ncols, hipMemcpyDeviceToDevice, stream_));
+      pcg(ncols, mass, visc, inner, false);
+      for (int c = 0; c < ncols; ++c) out.iterations[c] += inner[c];
+      add_cols(pacc_, x_, np_, ncols, stream_);
+      divergence_transpose(x_, np_, nullptr, ru64_, size_, DT_F64, w2_, packed_, g_, ncols, nullptr, stream_);
+      velocity_inverse(w2_, size_, w_, size_, all, mass, visc);    // du = L^-1 (r_u + D^T dp)
+      add_cols(uacc_, w_, size_, ncols, stream_);
+      out.sweeps = sweep + 2;
+    }
+    reduce(pacc_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_, fin(sums_));
+    sub_mean_dot(pacc_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
+    copy_cols(pacc_, np_, pres, pres_stride, np_, ncols, stream_);
+    copy_cols(uacc_, size_, vel, vel_stride, size_, ncols, stream_);
+    remember_pressure(pacc_, ncols, nullptr, bzero_);
   }
 
   // MIXED: FP64 defect correction around bf16-transform inner solves.
@@ -644,6 +712,10 @@ class Solver {
       out.vmax = mb->vals[2];
       out.divergence = 0.0;
       for (int c = 0; c < ncols; ++c) out.divergence = std::max(out.divergence, (double)mb->vals[3 + c]);
+      if (sweep == 0) {
+        out.first_residual = out.residual;
+        out.first_divergence = out.divergence;
+      }
       const bool done = out.residual <= opts_.refine_tolerance && out.divergence <= opts_.refine_tolerance;
       if (done || sweep >= opts_.max_sweeps || !std::isfinite(out.residual) || !std::isfinite(out.divergence))
         break;
@@ -965,6 +1037,7 @@ class Solver {
     prev_ncols_ = 0;
     prev_b_ok_ = false;
     uacc_ = (double*)take(vb);
+    ru64_ = (double*)take(vb);
     pacc_ = (double*)take(pb);
     rc_ = (double*)take(pb);
     wf_ = (float*)take(sizeof(float) * size_ * ncols);
@@ -1011,6 +1084,7 @@ class Solver {
   double *w_ = nullptr, *w2_ = nullptr, *s1_ = nullptr, *s2_ = nullptr;
   double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;
   double *fcomp_ = nullptr, *uacc_ = nullptr, *pacc_ = nullptr, *rc_ = nullptr;
+  double* ru64_ = nullptr;   // refined FP64 solve: the FP64 velocity residual r_u
   float* wf_ = nullptr;
   static Fin fin(double* out) { return Fin{out}; }
   float* du_ = nullptr;       // mixed sweeps: velocity correction, accumulated in the CG
@@ -1132,7 +1206,7 @@ void* mac_stokes_create(int32_t nx, int32_t ny, double lx, double ly, const mac_
   if (!(o.tolerance > 0) || !(o.krylov_tolerance > 0) || !is_finite(o.tolerance) || !is_finite(o.krylov_tolerance) ||
       o.max_iterations < 1 || (o.precision != MAC_PRECISION_FP64 && o.precision != MAC_PRECISION_MIXED) ||
       !(o.refine_tolerance > 0) || !(o.inner_tolerance > 0) || o.max_sweeps < 1 || o.mixed_split < 0 ||
-      o.mixed_split > 3) {
+      o.mixed_split > 3 || o.fp64_refine < 0 || o.fp64_refine > 30) {
     g_create_error = "Invalid solver controls";
     return nullptr;
   }
@@ -1222,6 +1296,8 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
       report->sweeps = out.sweeps;
       report->total_iterations = total;
       report->warm_started = out.warm ? 1 : 0;
+      report->first_residual = out.first_residual;
+      report->first_divergence = out.first_divergence;
     }
     const double tol = s->opts().tolerance;
     if (!is_finite(out.residual) || !is_finite(out.divergence) || !is_finite(out.vmax) || out.residual > tol ||

@@ -63,6 +63,7 @@ EXPORTED = (
     "mac_grid_divergence",
     "mac_dot",
     "mac_max_abs",
+    "mac_reduce_into",
     "mac_batch_create",
     "mac_batch_create_error",
     "mac_batch_destroy",
@@ -84,7 +85,7 @@ class StokesOpts(ctypes.Structure):
         ("max_sweeps", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("mixed_split", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("fp64_refine", ctypes.c_int32),
     ]
 
 
@@ -98,6 +99,8 @@ class SolveReport(ctypes.Structure):
         ("total_iterations", ctypes.c_int32),
         ("warm_started", ctypes.c_int32),
         ("pad0", ctypes.c_int32),
+        ("first_residual", ctypes.c_double),
+        ("first_divergence", ctypes.c_double),
     ]
 
 
@@ -197,6 +200,8 @@ def load() -> ctypes.CDLL:
     lib.mac_dot.restype = _I32
     lib.mac_max_abs.argtypes = [_I64, _P, ctypes.POINTER(_D)]
     lib.mac_max_abs.restype = _I32
+    lib.mac_reduce_into.argtypes = [_I64, _P, _P, _I32, _P]
+    lib.mac_reduce_into.restype = _I32
     lib.mac_batch_create.argtypes = [_I32, _P, _P, _P, _P, _I32, ctypes.POINTER(StokesOpts)]
     lib.mac_batch_create.restype = _P
     lib.mac_batch_create_error.argtypes = []

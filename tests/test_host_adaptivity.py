@@ -110,3 +110,92 @@ def test_controller_factors_follow_their_formulas_and_validate():
         PIController(proportional=0.3, integral=0.4)
     sc = ScalarErrorControl(atol=1e-6, gamma=2.0)
     assert sc.defect(1.0, np.exp(-2.0 * 0.1), 0.1) == pytest.approx(0.0, abs=1e-15)
+
+
+def _mms_model(backend_kind):
+    from macsav import MACGrid
+    from macsav.mac_ns import MACNavierStokes
+    from oracle import mms
+    grid = MACGrid(12, 10, 1.3, 0.8)
+    og = mo.Grid(12, 10, 1.3, 0.8)
+    backend = (mo.DirectStokes(og, tolerance=1e-9) if backend_kind == "direct"
+               else mo.TensorStokes(og, tolerance=1e-9, krylov_tolerance=1e-12))
+    model = MACNavierStokes(grid, 0.1, backend=backend)
+    model.force = lambda t: mms.forcing(grid, 0.1, 0.2, t)
+    return model, model.state(0.0, mms.initial_velocity(grid, 0.2))
+
+
+def test_tensor_and_direct_backends_take_the_same_adaptive_decisions():
+    """The reference's tests/core/test_tensor_stokes_adaptive.py:12-37 on the oracle
+    backends: SDIRK3 under I control rejects the first trial, then accepts two; both
+    backends take the same decisions and step sizes and end at the same state."""
+    from macsav.adaptivity import IController, integrate_adaptive
+    from macsav.schemes import SDIRK3
+    runs = []
+    for kind in ("direct", "tensor"):
+        model, initial = _mms_model(kind)
+        res = integrate_adaptive(model, SDIRK3(), initial, 0.002, 0.002,
+                                 controller=IController(atol=1e-12, rtol=1e-8, min_step=1e-7, max_step=0.002))
+        assert res.status == "complete", res.error
+        assert res.final.t == 0.002
+        assert model.max_abs(model.apply_D(model.vector(res.final))) < 1e-11
+        assert max(s.continuity_residual for s in res.final_stages) < 1e-11
+        runs.append((model, res))
+    (dm, direct), (tm, tensor) = runs
+    assert [a["accepted"] for a in direct.attempts] == [False, True, True]
+    assert [a["accepted"] for a in tensor.attempts] == [False, True, True]
+    np.testing.assert_allclose([a["step"] for a in tensor.attempts], [a["step"] for a in direct.attempts],
+                               rtol=2e-8)
+    np.testing.assert_allclose(tm.vector(tensor.final), dm.vector(direct.final), rtol=2e-11, atol=2e-11)
+
+
+def test_strict_snapshots_become_nodes_and_slivers_are_split():
+    from macsav.adaptivity import IController, integrate_adaptive
+    from macsav.schemes import make_scheme
+    m = _ode()
+    x0 = np.array([1.0, -0.5])
+    outputs = [0.0, 0.13, 0.2001, 0.5]
+    control = IController(atol=1e-6, rtol=1e-4, min_step=1e-3, max_step=0.1)
+    res = integrate_adaptive(m, make_scheme("sdirk3"), m.state(0.0, x0), 0.5, 0.05, controller=control,
+                             snapshots=outputs, strict_snapshots=True)
+    assert res.status == "complete", res.error
+    for t in outputs:
+        assert min(abs(np.array(res.times) - t)) <= 1e-15
+    assert res.snapshot_times == pytest.approx(outputs, abs=1e-15)
+    parts = [a for a in res.attempts if a["boundary_partition"]]
+    assert parts, "a sliver in front of an output is replaced by equal partition steps"
+    with pytest.raises(ValueError):
+        integrate_adaptive(m, make_scheme("sdirk3"), m.state(0.0, x0), 0.5, 0.05, snapshots=[0.3, 0.2],
+                           strict_snapshots=True)
+
+
+def test_continuation_resumes_with_the_controller_history():
+    from macsav.adaptivity import PIController, integrate_adaptive
+    from macsav.schemes import make_scheme
+    m = _ode()
+    x0 = np.array([1.0, -0.5])
+    control = PIController(atol=1e-7, rtol=1e-5, max_step=0.2)
+    whole = integrate_adaptive(m, make_scheme("sdirk3"), m.state(0.0, x0), 0.6, 0.1, controller=control)
+    first = integrate_adaptive(m, make_scheme("sdirk3"), m.state(0.0, x0), 0.3, 0.1, controller=control)
+    assert first.continuation.accepted_steps == first.accepted_steps
+    rest = integrate_adaptive(m, make_scheme("sdirk3"), first.final, 0.3, first.continuation.step,
+                              controller=control, continuation=first.continuation)
+    assert rest.status == "complete" and rest.final.t == pytest.approx(0.6)
+    assert rest.continuation.attempted_steps == first.attempted_steps + rest.attempted_steps
+    assert np.linalg.norm(rest.final.x - whole.final.x) < 1e-5
+
+
+def test_observer_failures_end_the_run_with_status_failed():
+    from macsav.adaptivity import integrate_adaptive
+    from macsav.integrate import integrate
+    from macsav.schemes import make_scheme
+    m = _ode()
+    initial = m.state(0.0, np.array([1.0, -0.5]))
+
+    def boom(*_):
+        raise KeyError("observer")
+    for kw in ("on_attempt", "on_advance", "on_accept"):
+        res = integrate_adaptive(m, make_scheme("sdirk3"), initial, 0.5, 0.1, **{kw: boom})
+        assert res.status == "failed" and "observer failed" in res.error and "KeyError" in res.error
+    res = integrate(m, make_scheme("sdirk3"), initial, [0.1] * 3, on_accept=boom)
+    assert res.status == "failed" and len(res.times) == 2 and "observer failed" in res.error

@@ -106,12 +106,14 @@ def test_schemes_reach_their_order_on_a_manufactured_solution(name, order):
 
 @pytest.mark.parametrize("order,count", [(1, 1), (2, 1), (3, 2)])
 def test_scalar_updates_match_an_independent_block_solve(order, count):
-    from macsav.schemes.scalar_correction import solve_scalar_stage
+    from macsav.schemes import sav_update
+    from oracle import scheme_oracle as so
     L = np.array([[1.8, 0.2], [0.2, 1.4]])
     B, rhs_u = np.array([0.7, -0.4]), np.array([0.6, -0.2])
     dt, d, rhs_r, z = 0.2, 1.7, 0.23, -0.25
     U, V = np.linalg.solve(L, rhs_u), np.linalg.solve(L, B)
-    res = solve_scalar_stage(z, dt, d, rhs_r, B @ U, B @ V, order=order)
+    res = sav_update(z, dt, d, rhs_r, B @ U, B @ V, order=order, count=count)
+    oracle_values, _ = so.scalar_stage(z, dt, d, rhs_r, B @ U, B @ V, order)
     want = []
     for _ in range(count):
         q = sum(z**j for j in range(order))
@@ -120,6 +122,7 @@ def test_scalar_updates_match_an_independent_block_solve(order, count):
         z = float(sol[-1])
         want.append(z)
     np.testing.assert_allclose(res.values, want, atol=2e-16, rtol=0)
+    np.testing.assert_allclose(oracle_values, want, atol=2e-16, rtol=0)
     np.testing.assert_allclose(U - dt * res.coupling * V, sol[:-1], atol=2e-16, rtol=0)
 
 
@@ -219,3 +222,58 @@ def test_failing_step_keeps_the_accepted_prefix():
     res = integrate(m, FailsThird(), m.state(0.0, x0), [1e-3] * 5, snapshots=[0.0, 2e-3, 4e-3])
     assert res.status == "failed" and "boom" in res.error
     assert len(res.times) == 3 and res.snapshot_times == [0.0, 2e-3]
+
+
+# ------------------------------------- the engine against the per-scheme restatement
+def _engine_vs_oracle(model_factory, name, steps, dt, x0, r0=0.3):
+    from macsav.schemes import make_scheme
+    from oracle import scheme_oracle as so
+    m = model_factory()
+    scheme = make_scheme(name)
+    st = m.state(0.0, x0, r0)
+    ost = m.state(0.0, x0, r0)
+    for _ in range(steps):
+        trial = scheme.step(m, st, dt)
+        t, u, r, stages, emb = so.STEPS[name](m, ost, dt)
+        ost = m.state(t, u, r)
+        assert trial.state.t == t
+        scale = max(np.abs(u).max(), 1.0)
+        np.testing.assert_allclose(m.vector(trial.state), u, rtol=0, atol=1e-13 * scale)
+        assert abs(trial.state.r - r) <= 1e-13 * max(abs(r), 1.0)
+        if emb is None:
+            assert trial.embedded_velocity is None
+        else:
+            np.testing.assert_allclose(trial.embedded_velocity, emb, rtol=0, atol=1e-12 * scale)
+        assert len(trial.stages) == len(stages)
+        for a, b in zip(trial.stages, stages):
+            np.testing.assert_allclose(np.ravel(a.pressure), np.ravel(b["pressure"]), rtol=0,
+                                       atol=1e-11 * max(np.abs(b["pressure"]).max(), 1.0))
+            assert a.r == pytest.approx(b["r"], abs=1e-13)
+            for key in ("residual", "divergence_inf", "continuity_residual"):
+                # record values agree to the rounding level of the quantities they measure
+                assert abs(getattr(a, key) - b[key]) <= 1e-12 * (1 + abs(b[key])) + 1e-14
+            if "corrections" in b:
+                np.testing.assert_allclose(a.corrections, b["corrections"], rtol=0, atol=1e-13)
+                assert a.frozen_factor == pytest.approx(b["frozen_factor"], abs=1e-13)
+                assert a.scalar_residual <= 1e-8
+        st = trial.state
+
+
+@pytest.mark.parametrize("name", ["sdirk2", "sdirk2_mrsav", "sdirk3", "sdirk3_mrsav", "imex_euler",
+                                  "imex_euler_mrsav"])
+def test_engine_matches_the_per_scheme_restatement_on_the_mac_model(name):
+    from macsav.fields import stream_function_velocity
+    factory = lambda: _mac_model(n=12, nu=0.05)  # noqa: E731
+    m = factory()
+    x0 = stream_function_velocity(m.grid, seed=5, modes=4, amplitude=0.5)
+    _engine_vs_oracle(factory, name, steps=3, dt=2e-3, x0=x0)
+
+
+@pytest.mark.parametrize("name", ["sdirk2", "sdirk2_mrsav", "sdirk3", "sdirk3_mrsav", "imex_euler",
+                                  "imex_euler_mrsav"])
+def test_engine_matches_the_per_scheme_restatement_on_a_forced_ode(name):
+    def factory():
+        m = ODEModel()
+        m.force = lambda t: np.array([np.cos(3 * t), np.sin(t) - 0.2])
+        return m
+    _engine_vs_oracle(factory, name, steps=4, dt=0.05, x0=np.array([0.4, -0.7]), r0=0.8)
