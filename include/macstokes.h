@@ -208,6 +208,19 @@ int32_t mac_max_abs(int64_t n, const double* a, double* result);
  * solver/schemes/sdirk3_mrsav.py:35-57); callers read many such slots in one copy. */
 int32_t mac_reduce_into(int64_t n, const double* a, const double* b, int32_t mode, double* out);
 
+/* ---- allocation-free inner product ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops that forbid hipMalloc): mac_dot's
+ * result (same kernels, same summation order, bitwise equal) with its partial sums in
+ * a file-scope device array of 4160 doubles (33 KB per device, held for the life of the
+ * process) instead of library-allocated scratch.  1 <= n <= 2^31 - 1, else
+ * MAC_INVALID_ARGUMENT.  Synchronous (the result is a host double).  Calls that overlap
+ * on one device share the array and race; mac_noalloc_last_error() holds the calling
+ * thread's last message. */
+#define MAC_NOALLOC_MAX_N 2147483647LL
+int32_t mac_dot_noalloc(int64_t n, const double* a, const double* b, double* result);
+const char* mac_noalloc_last_error(void);
+
 /* ---- batched cavities ----
  * Replaces a host loop of `TensorStokes.solve` (tensor_stokes.py:245-364) over many
  * independent cavities, e.g. one `MACNavierStokes` per cavity stepped in lockstep: every

@@ -64,6 +64,8 @@ EXPORTED = (
     "mac_dot",
     "mac_max_abs",
     "mac_reduce_into",
+    "mac_dot_noalloc",
+    "mac_noalloc_last_error",
     "mac_batch_create",
     "mac_batch_create_error",
     "mac_batch_destroy",
@@ -202,6 +204,10 @@ def load() -> ctypes.CDLL:
     lib.mac_max_abs.restype = _I32
     lib.mac_reduce_into.argtypes = [_I64, _P, _P, _I32, _P]
     lib.mac_reduce_into.restype = _I32
+    lib.mac_dot_noalloc.argtypes = [_I64, _P, _P, ctypes.POINTER(_D)]
+    lib.mac_dot_noalloc.restype = _I32
+    lib.mac_noalloc_last_error.argtypes = []
+    lib.mac_noalloc_last_error.restype = ctypes.c_char_p
     lib.mac_batch_create.argtypes = [_I32, _P, _P, _P, _P, _I32, ctypes.POINTER(StokesOpts)]
     lib.mac_batch_create.restype = _P
     lib.mac_batch_create_error.argtypes = []
