@@ -1,27 +1,37 @@
 #!/usr/bin/env python3
 """Stage-solve throughput of the MAC shifted-Stokes solve at 1024^2 on one MI355X.
 
-Workload: MAC 1024x1024 on the unit square, nu = 1e-3,
-sdirk3_mrsav with dt = 2e-4 from the seeded stream-function field with modes <= 32, so
-every stage solve has mass 1 and viscosity nu*dt*D[i,i] = 5e-8 and carries the two
-right-hand sides [stage rhs, convection] sharing one operator.  The stage solves of a
-run are recorded (an FP64 run through the MAC model, one step of start-up first) and
-replayed in order from HBM: W untimed solves, then K timed ones.  One "step" is one
-two-RHS stage solve to residual <= 1e-10 and max|D u| <= 1e-10 (absolute, the
-reference's definitions).
+Workload: MAC 1024x1024 on the unit square, nu = 1e-3, sdirk3_mrsav with dt = 2e-4 from
+the seeded stream-function field with modes <= 32, so every stage solve has mass 1 and
+viscosity nu dt D[i,i] = 5e-8 and carries the two right-hand sides [stage rhs,
+convection] sharing one operator.  The stage solves of a run are recorded on the
+device-resident model (FP64) and replayed in order from HBM.  Each form first solves
+the run's first six steps untimed (this fills the same-stage history the warm starts
+extrapolate), then W untimed and K timed stage solves.  One "step" is one two-RHS stage
+solve to residual <= 1e-10 and max|D u| <= 1e-10, absolute, in the reference's
+definitions; each form's worst values are reported and checked against that bar.
 
 Forms, all on the same recorded right-hand sides:
-  plain FP64      the reference algorithm (zero start; Schur-CG rtol 1e-14, which this
-                  size needs for the 1e-10 divergence bar)
-  FP64 + history  the same, started from the same-stage extrapolation of past steps
-  mixed           bf16x3 MFMA transforms inside FP64 defect correction, zero start
-  mixed + history the mixed form started from the extrapolated velocity and pressure
-  mixed + network the mixed form started from the trained CNN's correction of the
-                  extrapolated pressure (trained here on another seed's run)
-`value` is the fastest mixed form's stage solves/s; the others are reported beside it.
-The dominant kernel (the bf16 transform GEMM) is timed live with HIP events on the
-solver's own stream over a second pass of the same K solves.  The CPU baseline times
-the oracle (NumPy/SciPy restatement of the reference path) on a bounded sample.
+  fp64             the reference algorithm: zero start, Schur-CG rtol 1e-14.  Its
+                   max|D u| stops near 2e-10 at this size (the oracle's too), above the bar
+  fp64+refine      the same with rtol 1e-12 and FP64 defect correction to the bar: the
+                   plain FP64 GPU solve at equal final residual (the speedup baseline)
+  fp64+history     fp64+refine started from the same-stage history extrapolation
+  mixed            bf16x3 MFMA transforms inside FP64 defect correction, zero start
+  mixed+history    the mixed form started from the extrapolated velocity and pressure
+                   (degree 5 through the last six steps)
+  mixed+network    mixed+history whose early solves (fewer than three recorded steps)
+                   start from the trained CNN's correction of the extrapolation
+`value` is mixed+network's stage solves/s (the metric's form).  The same forms are also
+timed over every stage solve of the run (cold start included: `whole_run`).  The
+dominant kernel (the split bf16 transform GEMM) is timed live with HIP events on the
+solver's own stream over a second pass of the K solves.  The CPU baseline times the
+oracle (NumPy/SciPy restatement of the reference path) on a bounded sample, on one core
+and on the host's cores.
+
+Other workloads (--workload): learned (the learned warm start at 512^2, 40 training and
+10 held-out seeds), cavities (batched uneven cavities), run (end-to-end steps),
+combined (adaptive 2048^2 with lid), sizes (solve time against grid size).
 """
 from __future__ import annotations
 
@@ -37,11 +47,23 @@ for _p in (str(ROOT), str(ROOT / "mac-mr-sav-rk_amd")):
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
-PEAK_TFLOPS = {"fp64": 78.6, "mixed": 2500.0}  # MI355X dense: FP64 MFMA, BF16 MFMA
+PEAK_TFLOPS = {"fp64": 78.6, "bf16": 2500.0}     # MI355X dense: FP64 MFMA, BF16 MFMA
 NU, DT, STAGE_DIAG = 1e-3, 2e-4, 0.25            # sdirk3 D[i,i] = 1/4
 STAGES = 4                                       # sdirk3_mrsav stage solves per step
+HISTORY_STEPS = 6                                # extrapolation depth of the history forms
+FILL_STEPS = 6                                   # run steps solved before the replay
+BAR = 1e-10
 CALIBRATION_BYTES = 512 << 20                    # one plain copy of known size, before timing
 TRAFFIC_FILE = ROOT / "profiles" / "k_gemm_traffic_1024.md"
+
+FORMS = {
+    "fp64": dict(precision="fp64", krylov=1e-14),
+    "fp64+refine": dict(precision="fp64", krylov=1e-12, refine=3),
+    "fp64+history": dict(precision="fp64", krylov=1e-12, refine=3, warm="history"),
+    "mixed": dict(precision="mixed"),
+    "mixed+history": dict(precision="mixed", warm="history"),
+    "mixed+network": dict(precision="mixed", warm="network"),
+}
 
 
 def calibration_copy():
@@ -69,72 +91,240 @@ def measured_traffic(precision: str):
     return None
 
 
-def record_stage_solves(grid, count: int, seed: int = 0, modes: int = 32):
-    """Right-hand sides (device tensors), mass and viscosity of `count` consecutive
-    sdirk3_mrsav stage solves of a plain FP64 run through the MAC model, after one step
-    of start-up."""
-    import numpy as np
-    import torch
+def record_run(grid, steps: int, seed: int = 0, modes: int = 32, nu: float = NU, dt: float = DT):
+    """Right-hand sides ((size, 2) device views), mass and viscosity of every stage solve
+    of an sdirk3_mrsav run on the device-resident model with the refined FP64 backend."""
     from macsav import GpuTensorStokes
     from macsav.fields import stream_function_velocity
-    from macsav.mac_ns import MACNavierStokes
+    from macsav.mac_ns_device import DeviceMACNavierStokes
     from macsav.schemes import make_scheme
-    backend = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14)
-    model = MACNavierStokes(grid, NU, backend=backend)
-    scheme = make_scheme("sdirk3_mrsav")
-    state = model.state(0.0, stream_function_velocity(grid, seed=seed, modes=modes))
-    state = scheme.step(model, state, DT).state
+    backend = GpuTensorStokes(grid, tolerance=BAR, krylov_tolerance=1e-12, fp64_refine=3)
+    model = DeviceMACNavierStokes(grid, nu, backend=backend)
     rec = []
-    original = backend.solve
+    inner = backend.solve
 
     def recording(rhs, *, mass, viscosity, pressure_guess=None):
-        # Held as (size, k) views of one (k, size) tensor, the layout the device-resident
-        # model hands the solver (no transpose copy on the way in).
-        cols = np.ascontiguousarray(np.asarray(rhs).T)
-        rec.append((torch.as_tensor(cols, device="cuda:0").t(), mass, viscosity))
-        return original(rhs, mass=mass, viscosity=viscosity, pressure_guess=pressure_guess)
+        rec.append((rhs.clone(), mass, viscosity))
+        return inner(rhs, mass=mass, viscosity=viscosity, pressure_guess=pressure_guess)
 
     backend.solve = recording
-    while len(rec) < count:
-        state = scheme.step(model, state, DT).state
-    backend.solve = original
-    return rec[:count]
+    scheme = make_scheme("sdirk3_mrsav")
+    state = model.state(0.0, stream_function_velocity(grid, seed=seed, modes=modes))
+    for _ in range(steps):
+        state = scheme.step(model, state, dt).state
+    return rec
 
 
-def train_warm_start(grid, seed: int = 7, steps: int = 6):
-    """The CNN warm start trained on (b, base, p) pairs of another seed's FP64 run."""
+def make_solver(grid, form: str, net=None):
+    from macsav import GpuTensorStokes
+    f = FORMS[form]
+    s = GpuTensorStokes(grid, tolerance=BAR, krylov_tolerance=f.get("krylov", 1e-14), precision=f["precision"],
+                        refine_tolerance=BAR, fp64_refine=f.get("refine", 0))
+    if f.get("warm") == "history":
+        s.set_warm_start("history", period=STAGES, history_steps=HISTORY_STEPS)
+    elif f.get("warm") == "network":
+        s.set_warm_start(net, period=STAGES, history_steps=HISTORY_STEPS, network_depth=3)
+    return s
+
+
+def _tally(solver):
+    return (solver.total_iterations, solver.sweeps, solver.warm_starts, solver.network_starts)
+
+
+def _stats(solver, before, n, elapsed, worst):
+    it, sw, w, nw = (a - b for a, b in zip(_tally(solver), before))
+    return {"stage_solves_per_s": n / elapsed, "ms_per_step": 1e3 * elapsed / n,
+            "cg_iterations_per_rhs": it / (2 * n), "sweeps_per_solve": sw / n, "warm_started_solves": w,
+            "network_solves": nw, "worst_residual": worst[0], "worst_divergence_inf": worst[1],
+            "meets_bar": worst[0] <= BAR and worst[1] <= BAR}
+
+
+def replay(grid, rec, fill: int, warmup: int, steps: int, form: str, net=None, profile: bool = False):
+    """rec[:fill + warmup] untimed (the history fill, then the warm-up), then
+    rec[fill + warmup:][:steps] timed; optionally a second pass with HIP events on the
+    timed solves' transform GEMMs."""
+    import torch
+    solver = make_solver(grid, form, net)
+    first = fill + warmup
+
+    def run_once(keep=None, profiled=False):
+        solver.reset_history()
+        for f, m, v in rec[:first]:
+            solver.solve(f, mass=m, viscosity=v)
+            solver.commit_history()
+        torch.cuda.synchronize()
+        if profiled:
+            solver.set_profiling(True)
+        before = _tally(solver)
+        worst = [0.0, 0.0]
+        t0 = time.perf_counter()
+        for f, m, v in rec[first:first + steps]:
+            res = solver.solve(f, mass=m, viscosity=v)
+            solver.commit_history()
+            worst = [max(worst[0], res.residual), max(worst[1], res.divergence_inf)]
+            if keep is not None:
+                keep.append(res.velocity)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        return _stats(solver, before, steps, elapsed, worst), elapsed
+
+    velocities = []
+    stats, _ = run_once(velocities)
+    prof = None
+    if profile:
+        _, prof_elapsed = run_once(profiled=True)
+        prof = solver.kernel_profile()
+        solver.set_profiling(False)
+        stats["gemm_share_of_step"] = prof["gemm_ms"] * 1e-3 / (prof_elapsed + 1e-30)
+    return stats, prof, velocities
+
+
+def whole_run(grid, rec, form: str, net=None):
+    """Every recorded stage solve timed in order from a fresh solver (cold start
+    included), after one untimed solve of the last record on a throwaway solver."""
+    import torch
+    scratch = make_solver(grid, form, net)
+    f, m, v = rec[-1]
+    scratch.solve(f, mass=m, viscosity=v)
+    del scratch
+    solver = make_solver(grid, form, net)
+    torch.cuda.synchronize()
+    before = _tally(solver)
+    worst = [0.0, 0.0]
+    t0 = time.perf_counter()
+    for f, m, v in rec:
+        res = solver.solve(f, mass=m, viscosity=v)
+        solver.commit_history()
+        worst = [max(worst[0], res.residual), max(worst[1], res.divergence_inf)]
+    torch.cuda.synchronize()
+    return _stats(solver, before, len(rec), time.perf_counter() - t0, worst)
+
+
+def train_cold_start_network(grid, seeds=(11, 12, 13, 14, 15, 16), steps: int = 3):
+    """The CNN of the mixed+network form, trained on the solves it serves: those with one
+    or two recorded steps (steps 2 and 3 of a run), pairs recorded from FP64 solves of
+    other seeds' runs, their base the history extrapolation through the recorded steps."""
     from macsav import GpuTensorStokes
     from macsav.fields import stream_function_velocity
-    from macsav.learned import WarmStartNet, collect_pairs, fit_alpha, train
-    from macsav.mac_ns import MACNavierStokes
+    from macsav.learned import Pairs, WarmStartNet, collect_pairs, fit_alpha, train
+    from macsav.mac_ns_device import DeviceMACNavierStokes
     from macsav.schemes import make_scheme
-    plain = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14)
-    model = MACNavierStokes(grid, NU, backend=plain)
-    scheme = make_scheme("sdirk3_mrsav")
-    state = scheme.step(model, model.state(0.0, stream_function_velocity(grid, seed=seed, modes=32)), DT).state
-    pairs = collect_pairs(model, scheme, state, steps=steps, dt=DT, period=STAGES)
+    pairs = Pairs([], [], [])
+    for seed in seeds:
+        backend = GpuTensorStokes(grid, tolerance=BAR, krylov_tolerance=1e-12, fp64_refine=3)
+        model = DeviceMACNavierStokes(grid, NU, backend=backend)
+        x0 = stream_function_velocity(grid, seed=seed, modes=32)
+        got = collect_pairs(model, make_scheme("sdirk3_mrsav"), model.state(0.0, x0), steps=steps, dt=DT,
+                            period=STAGES, depth=2)
+        pairs.b += got.b
+        pairs.base += got.base
+        pairs.p += got.p
     net = WarmStartNet(width=32, seed=0, alpha=fit_alpha(grid, pairs))
-    losses = train(net, grid, pairs, epochs=40, lr=1e-3, batch=8)
-    return net, {"pairs": len(pairs.b), "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1]}
+    losses = train(net, grid, pairs, epochs=30, lr=1e-3, batch=8)
+    return net, {"pairs": len(pairs.b), "seeds": list(seeds), "loss_first_epoch": losses[0],
+                 "loss_last_epoch": losses[-1]}
 
 
-def cpu_baseline(grid, pair, kappa: float, budget_s: float) -> dict:
-    """Oracle (scipy.fft + scipy-cg loop) on host cores, single-threaded FFTs."""
+def cpu_baseline(grid, pair, kappa: float, budget_s: float, workers: int) -> dict:
+    """Oracle (scipy.fft transforms + the scipy-cg loop) on `workers` FFT threads."""
     import scipy.fft
     from oracle import mac_oracle as mo
     g = mo.Grid(grid.nx, grid.ny, grid.lx, grid.ly)
-    solver = mo.TensorStokes(g, tolerance=1e-10, krylov_tolerance=1e-14)
+    solver = mo.TensorStokes(g, tolerance=1e-9, krylov_tolerance=1e-14)
     done, t0 = 0, time.perf_counter()
-    with scipy.fft.set_workers(1):
+    with scipy.fft.set_workers(workers):
         while True:
             solver.solve(pair, mass=1.0, viscosity=kappa)
             done += 1
             if time.perf_counter() - t0 > budget_s or done >= 20:
                 break
     secs = time.perf_counter() - t0
-    return {"value": done / secs, "unit": "stage solves/s", "cores": 1, "kind": "port",
-            "sample": f"{done} two-RHS stage solves at {grid.nx}x{grid.ny} (oracle TensorStokes, "
-                      f"scipy.fft workers=1), {secs:.1f} s"}
+    return {"value": done / secs, "unit": "stage solves/s", "cores": workers, "kind": "port",
+            "sample": f"{done} two-RHS stage solves at {grid.nx}x{grid.ny} (oracle TensorStokes, scipy.fft "
+                      f"workers={workers}, host nproc={os.cpu_count()}), {secs:.1f} s"}
+
+
+def bench_learned(args) -> dict:
+    """The learned warm start at 512^2: sdirk3_mrsav runs with nu alternating 1e-3 / 3e-4
+    (dt = 2e-4, modes <= 32); the network trained on the (b - b_prev, previous-stage
+    pressure, pressure) pairs of the training seeds' FP64 solves (13 steps each, about
+    100 pairs per seed), then every held-out seed's run solved with the zero start (the
+    reference), the previous-stage pressure (Galerkin-scaled) and the network's
+    two-term Galerkin start over {previous, correction}, all to Schur-CG rtol 1e-12 and
+    the 1e-10 gate; CG iterations and wall time per start."""
+    import numpy as np
+    import torch
+    from macsav import GpuTensorStokes, MACGrid
+    from macsav.fields import stream_function_velocity
+    from macsav.learned import Pairs, WarmStartNet, collect_pairs, fit_alpha, train
+    from macsav.mac_ns_device import DeviceMACNavierStokes
+    from macsav.schemes import make_scheme
+    n = args.n or 512
+    grid = MACGrid(n, n)
+    nus = (1e-3, 3e-4)
+    train_seeds = list(range(100, 100 + args.train_seeds))
+    test_seeds = list(range(200, 200 + args.test_seeds))
+    steps = 13
+
+    def backend(warm=None, net=None):
+        b = GpuTensorStokes(grid, tolerance=BAR, krylov_tolerance=1e-12)
+        if warm == "previous":
+            b.set_warm_start("previous")
+        elif warm == "network":
+            b.set_warm_start(net)
+        return b
+
+    t0 = time.perf_counter()
+    pairs = Pairs([], [], [])
+    for k, seed in enumerate(train_seeds):
+        model = DeviceMACNavierStokes(grid, nus[k % 2], backend=backend())
+        x0 = stream_function_velocity(grid, seed=seed, modes=32)
+        got = collect_pairs(model, make_scheme("sdirk3_mrsav"), model.state(0.0, x0), steps=steps, dt=DT)
+        pairs.b += got.b
+        pairs.base += got.base
+        pairs.p += got.p
+    collect_s = time.perf_counter() - t0
+    n_pairs = sum(int(b.shape[1]) for b in pairs.b)
+    print(f"[learned] {n_pairs} pairs from {len(train_seeds)} seeds in {collect_s:.1f}s", file=sys.stderr,
+          flush=True)
+    net = WarmStartNet(width=32, seed=0, alpha=fit_alpha(grid, pairs))
+    t0 = time.perf_counter()
+    losses = train(net, grid, pairs, epochs=args.epochs, lr=1e-3, batch=16)
+    train_s = time.perf_counter() - t0
+    print(f"[learned] trained {args.epochs} epochs in {train_s:.1f}s, loss {losses[0]:.3g} -> {losses[-1]:.3g}",
+          file=sys.stderr, flush=True)
+    del pairs
+    rows = {}
+    for start in ("zero", "previous", "network"):
+        its = solves = 0
+        secs = 0.0
+        for k, seed in enumerate(test_seeds):
+            b = backend(None if start == "zero" else start, net)
+            model = DeviceMACNavierStokes(grid, nus[k % 2], backend=b)
+            x0 = stream_function_velocity(grid, seed=seed, modes=32)
+            state = model.state(0.0, x0)
+            scheme = make_scheme("sdirk3_mrsav")
+            torch.cuda.synchronize()
+            for _ in range(steps):
+                t1 = b.solve_seconds
+                state = scheme.step(model, state, DT).state
+                secs += b.solve_seconds - t1
+            its += b.total_iterations
+            solves += b.solves
+        rows[start] = {"cg_iterations": its, "stage_solves": solves, "cg_iterations_per_rhs": its / (2 * solves),
+                       "solve_seconds": secs, "ms_per_solve": 1e3 * secs / solves}
+        print(f"[learned] {start}: {rows[start]}", file=sys.stderr, flush=True)
+    gain = 1.0 - rows["network"]["cg_iterations"] / rows["previous"]["cg_iterations"]
+    return {"metric": "Schur-CG iterations per RHS to 1e-10, learned warm start vs zero / previous-stage guess",
+            "value": rows["network"]["cg_iterations_per_rhs"], "unit": "CG iterations per RHS", "n_gpus": 1,
+            "steps": steps, "warmup": 0, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 (network bf16 weights, fp32 accumulation)",
+            "data": f"synthetic: sdirk3_mrsav runs at {n}^2, nu alternating {nus}, seeded stream-function fields",
+            "config": {"workload": f"MAC {n}x{n}, {len(train_seeds)} training seeds, {len(test_seeds)} held-out "
+                                   f"seeds, {steps} steps each"},
+            "starts": rows, "iteration_saving_vs_previous": gain,
+            "training": {"pairs": n_pairs, "collect_seconds": collect_s, "train_seconds": train_s,
+                         "epochs": args.epochs, "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1]}}
 
 
 ETA2 = 1.0 - 0.5 ** 0.5                          # sdirk2 diagonal
@@ -204,51 +394,6 @@ def bench_cavities(args) -> dict:
             "speedup_vs_loop": value / loop, "launch_shape": list(batch.launch),
             "autotune_seconds": {f"{k[0]},{k[1]}": v for k, v in timings.items()},
             "cg_iterations_per_rhs": its / (2 * n * args.steps), "setup_seconds": setup}
-
-
-def replay(grid, rec, warmup: int, steps: int, precision: str, warm=None, profile: bool = False):
-    """Solve rec[:warmup] untimed (they also fill the warm-start history), then time
-    rec[warmup:warmup+steps]; optionally a second pass with HIP events on the timed
-    solves' transform GEMMs."""
-    import torch
-    from macsav import GpuTensorStokes
-    solver = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14, precision=precision,
-                             refine_tolerance=1e-10)
-    if warm is not None:
-        solver.set_warm_start(warm, period=STAGES)
-
-    def run_once(timed_results=None, profiled=False):
-        solver.reset_history()
-        for f, m, v in rec[:warmup]:
-            solver.solve(f, mass=m, viscosity=v)
-        torch.cuda.synchronize()
-        if profiled:  # HIP events on the timed solves only
-            solver.set_profiling(True)
-        it0, sw0, w0 = solver.total_iterations, solver.sweeps, solver.warm_starts
-        worst = [0.0, 0.0]
-        t0 = time.perf_counter()
-        for f, m, v in rec[warmup:warmup + steps]:
-            res = solver.solve(f, mass=m, viscosity=v)
-            worst = [max(worst[0], res.residual), max(worst[1], res.divergence_inf)]
-            if timed_results is not None:
-                timed_results.append(res.velocity)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        return elapsed, solver.total_iterations - it0, solver.sweeps - sw0, solver.warm_starts - w0, worst
-
-    velocities = []
-    elapsed, its, sweeps, warm_used, worst = run_once(velocities)
-    stats = {"stage_solves_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
-             "cg_iterations_per_rhs": its / (2 * steps), "sweeps_per_solve": sweeps / steps,
-             "warm_started_solves": warm_used, "worst_residual": worst[0], "worst_divergence_inf": worst[1]}
-    prof = None
-    if profile:
-        prof_elapsed = run_once(profiled=True)[0]
-        prof = solver.kernel_profile()
-        solver.set_profiling(False)
-        # Share of the profiled pass's own wall time (its event pairs slow it a little).
-        stats["gemm_share_of_step"] = prof["gemm_ms"] * 1e-3 / (prof_elapsed + 1e-30)
-    return stats, prof, velocities
 
 
 def bench_run(args) -> dict:
@@ -385,7 +530,7 @@ def bench_sizes(args) -> dict:
     rows = {}
     for n in (64, 128, 256, 384, 512, 768, 1024):
         grid = MACGrid(n, n)
-        rec = record_stage_solves(grid, 12, modes=min(32, n // 4))
+        rec = record_run(grid, 3, modes=min(32, n // 4))
         row = {}
         for precision in ("fp64", "mixed"):
             s = GpuTensorStokes(grid, tolerance=1e-10, krylov_tolerance=1e-14, precision=precision,
@@ -409,81 +554,57 @@ def bench_sizes(args) -> dict:
             "sizes": rows}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed stage solves")
-    ap.add_argument("--warmup", type=int, default=12, help="untimed stage solves before them (3 steps)")
-    ap.add_argument("--n", type=int, default=None, help="grid size (n x n); 1024, or 2048 for the combined run")
-    ap.add_argument("--modes", type=int, default=32)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-network", action="store_true")
-    ap.add_argument("--workload", choices=["stage1024", "cavities", "run", "combined", "sizes"], default="stage1024")
-    ap.add_argument("--t-end", type=float, default=0.05, help="combined run: final time")
-    ap.add_argument("--max-steps", type=int, default=2000, help="combined run: accepted-step cap")
-    ap.add_argument("--cavities", type=int, default=256)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
-              file=sys.stderr)
-        return 2
-
-    if args.workload == "cavities":
-        print(json.dumps(bench_cavities(args)), flush=True)
-        return 0
-    if args.workload == "sizes":
-        print(json.dumps(bench_sizes(args)), flush=True)
-        return 0
-    if args.workload == "combined":
-        print(json.dumps(bench_combined(args)), flush=True)
-        return 0
-    args.n = args.n or 1024
-    if args.workload == "run":
-        print(json.dumps(bench_run(args)), flush=True)
-        return 0
-
+def bench_stage1024(args) -> dict:
     import torch
     from macsav import MACGrid
-
     grid = MACGrid(args.n, args.n)
     kappa = NU * DT * STAGE_DIAG
     calibration_copy()
-    t0 = time.perf_counter()
-    rec = record_stage_solves(grid, args.warmup + args.steps, modes=args.modes)
-    record_s = time.perf_counter() - t0
     W, K = args.warmup, args.steps
-
-    plain, _, plain_v = replay(grid, rec, W, K, "fp64")
-    forms = {"fp64": plain}
-    forms["fp64+history"], _, _ = replay(grid, rec, W, K, "fp64", warm="history")
-    forms["mixed"], prof_mixed, mixed_v = replay(grid, rec, W, K, "mixed", profile=True)
-    forms["mixed+history"], prof_hist, hist_v = replay(grid, rec, W, K, "mixed", warm="history", profile=True)
-    net_info = None
+    fill = FILL_STEPS * STAGES
+    run_steps = max(args.run_steps, FILL_STEPS + (W + K + STAGES - 1) // STAGES)
+    t0 = time.perf_counter()
+    rec = record_run(grid, run_steps, modes=args.modes)
+    record_s = time.perf_counter() - t0
+    print(f"[stage1024] recorded {len(rec)} stage solves in {record_s:.1f}s", file=sys.stderr, flush=True)
+    net_info, net = None, None
     if not args.no_network:
-        net, net_info = train_warm_start(grid)
-        forms["mixed+network"], _, _ = replay(grid, rec, W, K, "mixed", warm=net)
+        net, net_info = train_cold_start_network(grid)
+        print(f"[stage1024] network {net_info}", file=sys.stderr, flush=True)
+    forms, velocities, profs = {}, {}, {}
+    for form in FORMS:
+        if form == "mixed+network" and net is None:
+            continue
+        profile = form == "mixed+network" or (net is None and form == "mixed+history")
+        forms[form], profs[form], velocities[form] = replay(grid, rec, fill, W, K, form, net, profile)
+        print(f"[stage1024] {form}: {forms[form]}", file=sys.stderr, flush=True)
 
     def rel(a_list, b_list):
-        return max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)) for a, b in zip(a_list, b_list))
-
-    forms["mixed"]["rel_l2_vs_fp64_velocity"] = rel(mixed_v, plain_v)
-    forms["mixed+history"]["rel_l2_vs_fp64_velocity"] = rel(hist_v, plain_v)
-    best = max(("mixed", "mixed+history"), key=lambda k: forms[k]["stage_solves_per_s"])
-    main_stats = forms[best]
-    prof = prof_hist if best == "mixed+history" else prof_mixed
+        return max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+                   for a, b in zip(a_list, b_list))
+    ref_v = velocities["fp64+refine"]
+    for form in forms:
+        forms[form]["rel_l2_vs_fp64_refine_velocity"] = rel(velocities[form], ref_v)
+    base = forms["fp64+refine"]["stage_solves_per_s"]
+    for form in forms:
+        forms[form]["speedup_vs_fp64_refine"] = forms[form]["stage_solves_per_s"] / base
+        forms[form]["speedup_vs_fp64"] = forms[form]["stage_solves_per_s"] / forms["fp64"]["stage_solves_per_s"]
+    runs = {}
+    for form in ("fp64+refine", "mixed+history", "mixed+network"):
+        if form in forms:
+            runs[form] = whole_run(grid, rec, form, net)
+            print(f"[stage1024] whole run {form}: {runs[form]}", file=sys.stderr, flush=True)
+    head = "mixed+network" if "mixed+network" in forms else "mixed+history"
+    main_stats, prof = forms[head], profs[head]
     gemm_s = prof["gemm_ms"] * 1e-3
-    # Flops of the MFMA work the algorithm issues: 2MNK per product, times 3 for the
-    # split bf16x3 products (A_lo B_hi + A_hi B_lo + A_hi B_hi).
-    achieved = prof["gemm_flops"] / gemm_s / 1e12
-    split = 3.0
-    value = main_stats["stage_solves_per_s"]
-    for k, f in forms.items():
-        f["speedup_vs_plain_fp64"] = f["stage_solves_per_s"] / plain["stage_solves_per_s"]
-
+    launches = max(prof["gemm_launches"], 1)
+    # The profile counts the MFMA work issued: three bf16 products per k-step in the
+    # split form.  The algorithmic work is one product, 2MNK, per transform pass.
+    executed = prof["gemm_flops"] / gemm_s / 1e12
+    algorithmic = executed / 3.0
     line = {
         "metric": "stage solves/s at 1024^2 MAC, mixed+learned vs plain FP64 GPU, equal residual",
-        "value": value,
+        "value": main_stats["stage_solves_per_s"],
         "unit": "stage solves/s",
         "n_gpus": 1,
         "steps": K,
@@ -493,30 +614,67 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16-gemm/f64-refine",
-        "data": f"synthetic: {W}+{K} consecutive stage solves of a recorded sdirk3_mrsav run from the seeded "
-                f"stream-function field (modes<={args.modes}), after one start-up step",
+        "data": f"synthetic: stage solves {fill + W}..{fill + W + K - 1} of a recorded {run_steps}-step "
+                f"sdirk3_mrsav run from the seeded stream-function field (modes<={args.modes}); the first "
+                f"{fill} (history fill) and the next {W} are solved untimed",
         "config": {"workload": f"MAC {grid.nx}x{grid.ny} stage solve, 2 RHS, mass=1, viscosity={kappa:g} "
-                               f"(nu=1e-3, dt=2e-4, sdirk3_mrsav)", "form": best,
-                   "bar": "residual<=1e-10 and divergence_inf<=1e-10 (absolute)"},
+                               f"(nu=1e-3, dt=2e-4, sdirk3_mrsav)", "form": head,
+                   "bar": "residual<=1e-10 and divergence_inf<=1e-10 (absolute)",
+                   "baseline_form": "fp64+refine"},
         "solve": main_stats,
-        "plain_fp64": plain,
-        "speedup_vs_plain_fp64": value / plain["stage_solves_per_s"],
+        "speedup_vs_plain_fp64_equal_bar": main_stats["stage_solves_per_s"] / base,
+        "speedup_vs_reference_fp64": main_stats["speedup_vs_fp64"],
         "forms": forms,
+        "whole_run": {"stage_solves": len(rec), "forms": runs},
         "warm_start_network": net_info,
         "record_seconds": record_s,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS["mixed"], "traffic": measured_traffic("mixed"),
-                     "kernel": "k_gemm_bf16",
-                     "launches": prof["gemm_launches"],
-                     "avg_launch_us": 1e6 * gemm_s / max(prof["gemm_launches"], 1),
-                     "flops_per_launch": prof["gemm_flops"] / max(prof["gemm_launches"], 1),
-                     "transform_flops_per_launch": prof["gemm_flops"] / split / max(prof["gemm_launches"], 1),
-                     "algorithmic_bytes_per_launch": prof["gemm_bytes"] / max(prof["gemm_launches"], 1)},
+        "roofline": {"bound": "mfma", "achieved": algorithmic, "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
+                     "frac": algorithmic / PEAK_TFLOPS["bf16"], "traffic": measured_traffic("mixed"),
+                     "kernel": "k_gemm_bf16", "work": "transform flops 2MNK per launch (one product per pass)",
+                     "mfma_executed_tflops": executed, "mfma_executed_frac": executed / PEAK_TFLOPS["bf16"],
+                     "launches": prof["gemm_launches"], "avg_launch_us": 1e6 * gemm_s / launches,
+                     "flops_per_launch": prof["gemm_flops"] / 3.0 / launches,
+                     "algorithmic_bytes_per_launch": prof["gemm_bytes"] / launches},
     }
     if not args.no_cpu:
-        f0, m0, v0 = rec[W]
-        line["cpu_baseline"] = cpu_baseline(grid, f0.cpu().numpy(), v0, args.cpu_seconds)
-    print(json.dumps(line), flush=True)
+        f0, m0, v0 = rec[fill + W]
+        pair = f0.cpu().numpy()
+        line["cpu_baseline"] = cpu_baseline(grid, pair, v0, args.cpu_seconds, 1)
+        cores = min(os.cpu_count() or 1, 16)
+        line["cpu_baseline"]["all_cores"] = cpu_baseline(grid, pair, v0, args.cpu_seconds, cores)
+    return line
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed stage solves")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed stage solves before them (after the fill)")
+    ap.add_argument("--run-steps", type=int, default=12, help="steps of the recorded run (at least the fill "
+                                                             "plus the warm-up and timed solves)")
+    ap.add_argument("--n", type=int, default=None, help="grid size (n x n)")
+    ap.add_argument("--modes", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-network", action="store_true")
+    ap.add_argument("--workload", choices=["stage1024", "learned", "cavities", "run", "combined", "sizes"],
+                    default="stage1024")
+    ap.add_argument("--t-end", type=float, default=0.05, help="combined run: final time")
+    ap.add_argument("--max-steps", type=int, default=2000, help="combined run: accepted-step cap")
+    ap.add_argument("--cavities", type=int, default=256)
+    ap.add_argument("--train-seeds", type=int, default=40)
+    ap.add_argument("--test-seeds", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        return 2
+    runners = {"cavities": bench_cavities, "sizes": bench_sizes, "combined": bench_combined, "learned": bench_learned,
+               "run": bench_run, "stage1024": bench_stage1024}
+    if args.workload in ("run", "stage1024"):
+        args.n = args.n or 1024
+    print(json.dumps(runners[args.workload](args)), flush=True)
     return 0
 
 

@@ -69,7 +69,8 @@ typedef struct mac_solve_report {
   double seconds;         /* wall time of the call, device-synchronised            */
   int32_t sweeps;         /* outer defect-correction sweeps (1 for FP64)           */
   int32_t total_iterations; /* Schur-CG iterations summed over columns and sweeps  */
-  int32_t warm_started;   /* 1 if the attached network supplied the CG start        */
+  int32_t warm_started;   /* 1: a warm start (history / previous) supplied the start;
+                              2: the network's correction entered it; 0: zero start  */
   int32_t pad0;
   double first_residual;   /* MIXED / refined FP64: residual and max|D u| of the first */
   double first_divergence; /* FP64 defect (the starting iterate's)                     */
@@ -124,6 +125,11 @@ int64_t mac_cnn_num_params(void* cnn);
 /* Parameters as FP32, per conv layer c0..c6: weights (co, ci, 3, 3) then bias (co). */
 int32_t mac_cnn_set_params(void* cnn, const float* host_params);
 int32_t mac_cnn_get_params(void* cnn, float* host_params);
+/* Element counts (FP32 units) of the 29 activation / work buffers one forward and
+ * backward pass of n inputs of H x W needs, for a network of `width` channels: the
+ * plan the network sizes its buffers by before any pass (host arithmetic only, no
+ * device call).  MAC_INVALID_ARGUMENT unless H, W are multiples of 8. */
+int32_t mac_cnn_plan(int32_t width, int32_t n, int32_t H, int32_t W, int64_t* counts);
 /* x: [n][2][H][W], y: [n][1][H][W] device FP32, H and W multiples of 8. */
 int32_t mac_cnn_forward(void* cnn, const float* x, float* y, int32_t n, int32_t H, int32_t W);
 /* Mean squared error against target [n][1][H][W] and its gradient (device, FP32). */
@@ -139,8 +145,9 @@ int32_t mac_cnn_train_step(void* cnn, const float* x, const float* target, int32
  *                      pressures); alpha is the pressure / right-hand-side scale it was
  *                      trained with;
  *   MAC_WARM_PREVIOUS  the previous solve's pressures;
- *   MAC_WARM_HISTORY   the same stage of the last three steps extrapolated in time,
- *                      3 p[-P] - 3 p[-2P] + p[-3P] (2 p[-P] - p[-2P] or p[-P] earlier), P
+ *   MAC_WARM_HISTORY   the same stage of the last K steps extrapolated polynomially in
+ *                      time (K = mac_stokes_set_history_steps, fewer while fewer are
+ *                      recorded; 3 p[-P] - 3 p[-2P] + p[-3P] for K = 3 and equal steps), P
  *                      the stage solves per step (mac_stokes_set_warm_history); a MIXED
  *                      handle extrapolates the velocities the same way.
  * FP64 handles start the Schur CG from theta g, theta = (g, b)/(g, S g); MIXED handles
@@ -155,6 +162,18 @@ int32_t mac_stokes_reset_history(void* handle);
 /* Stage solves per step for MAC_WARM_HISTORY and the network's base (0..16; 0 = off);
  * forgets the recorded history.  sdirk3_mrsav: 4, sdirk2_mrsav: 2. */
 int32_t mac_stokes_set_warm_history(void* handle, int32_t period);
+/* Steps the history extrapolation reaches back, 1..6 (default 6: degree 5 in time);
+ * forgets the recorded history. */
+int32_t mac_stokes_set_history_steps(void* handle, int32_t steps);
+/* MAC_WARM_NETWORK with a history period: the network corrects the extrapolation only
+ * while fewer than `steps` steps are recorded (1..7, default 3); from then on the
+ * extrapolation alone is the guess.  FP64 handles combine base g and correction c as
+ * a g + beta c with the smallest starting Schur residual (never worse than g alone). */
+int32_t mac_stokes_set_network_depth(void* handle, int32_t steps);
+/* Trial bookkeeping for the history: rollback = 0 commits the solves recorded since the
+ * last mark (an accepted step), rollback = 1 forgets them (a rejected or failed trial),
+ * so guesses extrapolate accepted steps only. */
+int32_t mac_stokes_history_mark(void* handle, int32_t rollback);
 /* Schur right-hand sides b of the last FP64 solve, ncols columns of np (device). */
 int32_t mac_stokes_last_schur_rhs(void* handle, double* out, int64_t ld, int32_t ncols);
 /* Network inputs / targets / scales exactly as the solver prepares them (device):

@@ -327,6 +327,29 @@ inline int nblocks(int64_t n) { return (int)((n + 255) / 256); }
 
 }  // namespace
 
+// ============================================================== buffer plan
+// Element counts of the activation buffers of one forward + backward pass of n inputs
+// of H x W, in the order of the Acts members (e0 ... de0, part, lossp, wt), from the
+// width and the shapes alone (no device calls), so the plan can be checked on a host.
+// Every count is the largest extent any kernel touches in that buffer.
+enum { kPlanBuffers = 29 };
+bool cnn_plan(int width, int n, int H, int W, int64_t* count) {
+  if (H % 8 || W % 8 || H < 8 || W < 8 || n < 1 || width < 1) return false;
+  const int64_t w = width, f = (int64_t)n * w * H * W, h2 = f / 4, h4 = f / 16, h8 = f / 64;
+  const int64_t one = (int64_t)n * H * W;
+  // e0 p1 e1 p2 e2 p3 e3 u2 g2 u1 g1 u0 y dy du0 dg1 du1 dg2 du2 de3 dp3 de2 dp2 de1 dp1 de0
+  const int64_t acts[26] = {f, h2, h2, h4, h4, h8, h8, h4, h4, h2, h2, f, one, one,
+                            f, h2, h2, h4, h4, h8, h8, h4, h4, h2, h2, f};
+  for (int k = 0; k < 26; ++k) count[k] = acts[k];
+  // The weight-gradient partials: one row of co*ci*9 + co per (spatial block, input) of
+  // the widest layer at full resolution, the largest of the seven layers' needs.
+  const int64_t nblk = (int64_t)((W + TS - 1) / TS) * ((H + TS - 1) / TS) * n;
+  count[26] = nblk * (w * w * 9 + w);
+  count[27] = 256;                       // loss partials (at most 256 blocks)
+  count[28] = (w * w * 9 + 1) / 2;       // flipped bf16 weights, in float units
+  return true;
+}
+
 // ============================================================== network object
 class Cnn {
  public:
@@ -380,6 +403,20 @@ class Cnn {
   // x: [n][2][H][W] device, y: [n][1][H][W] device.
   void forward(const float* x, float* y, int n, int H, int W) {
     ensure(n, H, W);
+    forward_planned(x, y, n, H, W);
+  }
+
+  // Loss = mean (y - t)^2 over all n*H*W outputs; gradient into g_ (and host copy).
+  float loss_grad(const float* x, const float* t, int n, int H, int W) {
+    ensure(n, H, W);
+    return loss_grad_planned(x, t, n, H, W);
+  }
+
+ private:
+  // The passes proper.  They use the activation buffers as planned by the caller's
+  // ensure() and never re-plan, so no buffer address they hold (a_.y passed to the
+  // forward pass, for one) can go stale while they run.
+  void forward_planned(const float* x, float* y, int n, int H, int W) {
     const int w = width_;
     conv(x, 0, a_.e0, n, H, W, true);
     pool(a_.e0, a_.p1, (int64_t)n * w, H, W);
@@ -396,11 +433,8 @@ class Cnn {
     conv(a_.u0, 6, y, n, H, W, false);
   }
 
-  // Loss = mean (y - t)^2 over all n*H*W outputs; gradient into g_ (and host copy).
-  float loss_grad(const float* x, const float* t, int n, int H, int W) {
-    // Size the activation buffers before taking any of their addresses.
-    ensure(n, H, W);
-    forward(x, a_.y, n, H, W);
+  float loss_grad_planned(const float* x, const float* t, int n, int H, int W) {
+    forward_planned(x, a_.y, n, H, W);
     const int w = width_;
     const int64_t cnt = (int64_t)n * H * W;
     const int mb = std::min<int64_t>(nblocks(cnt), 256);
@@ -433,6 +467,7 @@ class Cnn {
     return (float)(s / (double)cnt);
   }
 
+ public:
   float train_step(const float* x, const float* t, int n, int H, int W, float lr) {
     const float loss = loss_grad(x, t, n, H, W);
     ++step_;
@@ -519,8 +554,12 @@ class Cnn {
     __bf16* wt;
   };
 
+  // Sizes the activation buffers for (n, H, W) by the plan; called once at the top of
+  // each entry point, before any buffer address is taken.  (Re-planning inside a pass
+  // once freed buffers whose addresses the pass already held.)
   void ensure(int n, int H, int W) {
-    if (H % 8 || W % 8 || H < 8 || W < 8) throw std::invalid_argument("network input H, W must be multiples of 8");
+    int64_t plan[kPlanBuffers];
+    if (!cnn_plan(width_, n, H, W, plan)) throw std::invalid_argument("network input H, W must be multiples of 8");
     if (n == cap_n_ && H == cap_h_ && W == cap_w_) return;
     HIP_CHECK_CNN(hipStreamSynchronize(stream_));
     for (void* p : act_allocs_) {
@@ -528,22 +567,17 @@ class Cnn {
       allocs_.erase(std::remove(allocs_.begin(), allocs_.end(), p), allocs_.end());
     }
     act_allocs_.clear();
-    const int64_t w = width_, f = (int64_t)n * w * H * W, h2 = f / 4, h4 = f / 16, h8 = f / 64;
-    auto take = [&](int64_t count) {
-      float* p = alloc<float>(count);
+    int k = 0;
+    auto take = [&]() {
+      float* p = alloc<float>(plan[k++]);
       act_allocs_.push_back(p);
       return p;
     };
-    a_.e0 = take(f); a_.p1 = take(h2); a_.e1 = take(h2); a_.p2 = take(h4); a_.e2 = take(h4);
-    a_.p3 = take(h8); a_.e3 = take(h8); a_.u2 = take(h4); a_.g2 = take(h4); a_.u1 = take(h2);
-    a_.g1 = take(h2); a_.u0 = take(f); a_.y = take((int64_t)n * H * W); a_.dy = take((int64_t)n * H * W);
-    a_.du0 = take(f); a_.dg1 = take(h2); a_.du1 = take(h2); a_.dg2 = take(h4); a_.du2 = take(h4);
-    a_.de3 = take(h8); a_.dp3 = take(h8); a_.de2 = take(h4); a_.dp2 = take(h4); a_.de1 = take(h2);
-    a_.dp1 = take(h2); a_.de0 = take(f);
-    const int64_t nblk = (int64_t)((W + TS - 1) / TS) * ((H + TS - 1) / TS) * n;
-    a_.part = take(nblk * (w * w * 9 + w));
-    a_.lossp = take(256);
-    a_.wt = reinterpret_cast<__bf16*>(take(w * w * 9));
+    float** order[28] = {&a_.e0, &a_.p1, &a_.e1, &a_.p2, &a_.e2, &a_.p3, &a_.e3, &a_.u2, &a_.g2, &a_.u1,
+                         &a_.g1, &a_.u0, &a_.y, &a_.dy, &a_.du0, &a_.dg1, &a_.du1, &a_.dg2, &a_.du2, &a_.de3,
+                         &a_.dp3, &a_.de2, &a_.dp2, &a_.de1, &a_.dp1, &a_.de0, &a_.part, &a_.lossp};
+    for (float** slot : order) *slot = take();
+    a_.wt = reinterpret_cast<__bf16*>(take());
     cap_n_ = n;
     cap_h_ = H;
     cap_w_ = W;
@@ -645,6 +679,11 @@ int32_t mac_cnn_get_params(void* h, float* host_params) {
   Cnn* c = static_cast<Cnn*>(h);
   if (!c || !host_params) return MAC_INVALID_ARGUMENT;
   return cnn_guard(c, [&] { c->get_params(host_params); });
+}
+
+int32_t mac_cnn_plan(int32_t width, int32_t n, int32_t H, int32_t W, int64_t* counts) {
+  if (!counts) return MAC_INVALID_ARGUMENT;
+  return macs::cnn_plan(width, n, H, W, counts) ? MAC_OK : MAC_INVALID_ARGUMENT;
 }
 
 int32_t mac_cnn_forward(void* h, const float* x, float* y, int32_t n, int32_t H, int32_t W) {

@@ -180,6 +180,11 @@ void fill(double* x, int64_t n, double v, hipStream_t s);
 // x = theta g, r = b - theta q with theta = gb/gq per column (x holds g, q holds S g).
 void theta_start(double* x, double* r, const double* b, const double* q, const double* gb, const double* gq,
                  int64_t n, int64_t stride, int ncols, const CgState* st, hipStream_t s);
+// x = a g + beta c, r = b - a bg - beta q with (a, beta) minimising ||r||_2 per column,
+// from dots = [(bg,bg) (bg,q) (q,q) (bg,b) (q,b)] x ncols (columns of n contiguous
+// elements; inactive columns skipped).
+void galerkin2_start(double* x, double* r, const double* g, const double* c, const double* b, const double* bg,
+                     const double* q, const double* dots, int64_t n, int ncols, const CgState* st, hipStream_t s);
 // out = a - b column-wise (skips inactive columns when st is given).
 void sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t stride, int ncols,
               const CgState* st, hipStream_t s);
@@ -213,9 +218,15 @@ void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int
 void sweep_start(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,
                  double* bb, double* bmax, hipStream_t s, float* r32 = nullptr);
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s);
-// out = w[0] a + w[1] b (+ w[2] c when c is given), over n elements.
-void extrapolate(const double* a, const double* b, const double* c, const double w[3], double* out, int64_t n,
-                 hipStream_t s);
+// out = sum_k w[k] src[k] over n elements, 1 <= count <= kMaxExtrapolation (the time
+// extrapolation of recorded solutions, and differences of two fields).
+constexpr int kMaxExtrapolation = 6;
+struct Extrapolation {
+  const double* src[kMaxExtrapolation];
+  double w[kMaxExtrapolation];
+  int count;
+};
+void extrapolate(const double* const* src, const double* w, int count, double* out, int64_t n, hipStream_t s);
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s);
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s);
 // The same steps from the unfinalized partials of (r, z) / (p, q) (nblk per column, as

@@ -323,6 +323,38 @@ __global__ void k_theta_start(double* __restrict__ x, double* __restrict__ r, co
   }
 }
 
+// Two-term start over span{g, c} (base guess g, correction c), per column: the
+// coefficients minimise the starting residual ||b - a S g - beta S c||_2, the quantity
+// the CG stopping test measures, so the start is never worse in it than any multiple of
+// the base alone:
+//   [(v1,v1) (v1,v2)] [a   ]   [(v1,b)]
+//   [(v1,v2) (v2,v2)] [beta] = [(v2,b)],   v1 = S g (taken as bg), v2 = S c = q.
+// A near-singular pair falls back to the base alone (beta = 0), a useless base to zero.
+// dots holds (v1,v1), (v1,v2), (v2,v2), (v1,b), (v2,b), ncols each.
+__global__ void k_galerkin2(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ g,
+                            const double* __restrict__ cc, const double* __restrict__ b,
+                            const double* __restrict__ bg, const double* __restrict__ q, const double* dots,
+                            int ncols, int64_t n, const CgState* st) {
+  const int c = blockIdx.y;
+  if (st != nullptr && !st[c].active) return;
+  const double v11 = dots[c], v12 = dots[ncols + c], v22 = dots[2 * ncols + c];
+  const double v1b = dots[3 * ncols + c], v2b = dots[4 * ncols + c];
+  const double det = v11 * v22 - v12 * v12;
+  double a = 0.0, beta = 0.0;
+  if (v11 > 0.0 && v22 > 0.0 && det > 1e-12 * v11 * v22) {
+    a = (v1b * v22 - v12 * v2b) / det;
+    beta = (v11 * v2b - v12 * v1b) / det;
+  } else if (v11 > 0.0) {
+    a = v1b / v11;
+  }
+  const int64_t off = (int64_t)c * n;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double gi = g[off + i], ci = cc[off + i];
+    x[off + i] = a * gi + beta * ci;
+    r[off + i] = b[off + i] - a * bg[off + i] - beta * q[off + i];
+  }
+}
+
 __global__ void k_fill(double* x, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) x[i] = v;
 }
@@ -365,11 +397,17 @@ __global__ void k_cg_check(CgState* st, const double* rr, int* flags, int ncols)
 
 // Polynomial extrapolation in time from equally spaced past values (newest first):
 // out = 2 a - b (linear), or 3 a - 3 b + c (quadratic) when c is given.
-__global__ void k_extrapolate(const double* __restrict__ a, const double* __restrict__ b,
-                              const double* __restrict__ c, double wa, double wb, double wc,
-                              double* __restrict__ out, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-    out[i] = c != nullptr ? wa * a[i] + wb * b[i] + wc * c[i] : wa * a[i] + wb * b[i];
+__global__ void k_extrapolate(Extrapolation e, double* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    double v[kMaxExtrapolation];
+#pragma unroll
+    for (int k = 0; k < kMaxExtrapolation; ++k) v[k] = k < e.count ? e.src[k][i] : 0.0;  // loads first
+    double acc = e.w[0] * v[0];
+#pragma unroll
+    for (int k = 1; k < kMaxExtrapolation; ++k)
+      if (k < e.count) acc += e.w[k] * v[k];
+    out[i] = acc;
+  }
 }
 
 // With partials, rr is first finalized from them (one column after another, in
@@ -754,6 +792,11 @@ void theta_start(double* x, double* r, const double* b, const double* q, const d
   k_theta_start<<<col_grid(n, ncols), kThreads, 0, s>>>(x, r, b, q, gb, gq, n, stride, st);
 }
 
+void galerkin2_start(double* x, double* r, const double* g, const double* c, const double* b, const double* bg,
+                     const double* q, const double* dots, int64_t n, int ncols, const CgState* st, hipStream_t s) {
+  k_galerkin2<<<col_grid(n, ncols), kThreads, 0, s>>>(x, r, g, c, b, bg, q, dots, ncols, n, st);
+}
+
 void fill(double* x, int64_t n, double v, hipStream_t s) {
   k_fill<<<reduce_blocks(n), kThreads, 0, s>>>(x, n, v);
 }
@@ -818,9 +861,14 @@ void cg_alpha_fin(CgState* st, const double* partials, int nblk, double* out, in
 void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s) {
   k_post_values<<<1, 128, 0, s>>>(a, na, b, nb, mb, seq);
 }
-void extrapolate(const double* a, const double* b, const double* c, const double w[3], double* out, int64_t n,
-                 hipStream_t s) {
-  k_extrapolate<<<reduce_blocks(n), kThreads, 0, s>>>(a, b, c, w[0], w[1], w[2], out, n);
+void extrapolate(const double* const* src, const double* w, int count, double* out, int64_t n, hipStream_t s) {
+  Extrapolation e{};
+  e.count = count;
+  for (int k = 0; k < count; ++k) {
+    e.src[k] = src[k];
+    e.w[k] = w[k];
+  }
+  k_extrapolate<<<reduce_blocks(n), kThreads, 0, s>>>(e, out, n);
 }
 void cg_rho(CgState* st, const double* rz, int ncols, hipStream_t s) { k_cg_rho<<<(ncols + 63) / 64, 64, 0, s>>>(st, rz, ncols); }
 void cg_alpha(CgState* st, const double* pq, int ncols, hipStream_t s) {

@@ -366,6 +366,7 @@ class Solver {
     int sweeps = 1;
     bool warm = false;
     double first_residual = 0, first_divergence = 0;  // of the first FP64 defect (mixed / refined)
+    bool network = false;                             // the network's correction entered the start
   };
 
   // ------------------------------------------------------- learned warm start
@@ -377,14 +378,32 @@ class Solver {
   void reset_history() {
     prev_ncols_ = 0;
     hist_count_ = 0;
+    hist_committed_ = 0;
   }
+  // The solves recorded since the last commit belong to a trial in progress: commit keeps
+  // them, rollback forgets them (a rejected or failed trial), so every later guess
+  // extrapolates accepted steps only.  The ring holds one step more than the depth, so a
+  // trial's records never overwrite the committed steps the extrapolation reads.
+  void commit_history() { hist_committed_ = hist_count_; }
+  void rollback_history() {
+    hist_count_ = hist_committed_;
+    prev_ncols_ = 0;
+  }
+  // Steps the extrapolation reaches back (1 = repeat the last step's solution, k = the
+  // degree k-1 polynomial through the last k steps).  Forgets the recorded history.
+  void set_history_steps(int steps) {
+    hist_steps_ = steps;
+    hist_slots_ = 0;
+    reset_history();
+  }
+  int history_steps() const { return hist_steps_; }
   // Same-slot history: with a scheme of `period` stage solves per step, the solve `period`
   // back is the same stage of the previous step; the guess extrapolates polynomially in
-  // time over the last three steps (two after two steps, the last one after one).  The
+  // time through the last `hist_steps_` steps (fewer while fewer are recorded).  The
   // step sizes come from the solves themselves: a stage's viscosity/mass is nu a_ii h,
   // so same-slot ratios are step ratios.  Consecutive same-slot solves are taken
   // (h_m + h_m-1) / 2 apart (the stage at mid-step); the weights are the Lagrange ones
-  // on those nodes, 3 p[-P] - 3 p[-2P] + p[-3P] for equal steps.
+  // on those nodes, (-1)^(k+1) C(K, k) for K equal steps (3, -3, 1 for K = 3).
   void set_history(int period) {
     history_period_ = period;
     reset_history();
@@ -406,11 +425,15 @@ class Solver {
       copy_cols(pprev_, np_, out, np_, np_, ncols, stream_);
       return true;
     }
+    net_base_ = nullptr;
     if (warm_mode_ == MAC_WARM_HISTORY) return history_guess(ncols, out);
     if (warm_mode_ != MAC_WARM_NETWORK || cnn_ == nullptr || !warm_shape_ok(g_.nx, g_.ny)) return false;
     // The network corrects a base guess: the history extrapolation when a period is set,
     // else the previous solve's pressures.  Without a base there is no guess (zero start),
-    // as in training.
+    // as in training.  Once the history reaches back net_depth_limit_ steps its
+    // extrapolation alone is the guess: there the network's correction (at a quarter of
+    // the resolution, bf16 weights) is coarser than the extrapolation's error.
+    if (history_period_ > 0 && recorded_steps(ncols) >= net_depth_limit_) return history_guess(ncols, out);
     const double* base = nullptr;
     const int64_t n = np_ * ncols;
     if (history_period_ > 0) {
@@ -421,17 +444,25 @@ class Solver {
       copy_cols(bprev_, np_, warm_db_, np_, np_, ncols, stream_);
     }
     if (base == nullptr) return false;
-    const double w[3] = {1.0, -1.0, 0.0};
-    extrapolate(b, warm_db_, nullptr, w, warm_db_, n, stream_);  // b - b_base
+    const double w[2] = {1.0, -1.0};
+    const double* src[2] = {b, warm_db_};
+    extrapolate(src, w, 2, warm_db_, n, stream_);  // b - b_base
     warm_guess(cnn_, warm_db_, np_, base, np_, ncols, g_.nx, g_.ny, cnn_alpha_, warm_x_, warm_y_, warm_s_, out,
                np_, stream_);
+    net_base_ = base;
     return true;
   }
+  // Same-slot steps recorded for `ncols`-column solves (0 without a history).
+  int64_t recorded_steps(int ncols) const {
+    if (history_period_ <= 0 || hist_ncols_ != ncols) return 0;
+    return std::min<int64_t>(hist_count_ / history_period_, hist_steps_);
+  }
+  void set_net_depth_limit(int k) { net_depth_limit_ = k; }
   // Every slot the history extrapolation reads has its Schur right-hand side recorded.
   bool history_b_ok(int ncols) const {
     if (history_period_ <= 0 || hist_ncols_ != ncols || hist_b_ == nullptr) return false;
-    const int P = history_period_, slots = kHistSteps * P;
-    const int64_t steps = hist_count_ / P >= 3 ? 3 : hist_count_ / P;
+    const int P = history_period_, slots = hist_ring_steps() * P;
+    const int64_t steps = std::min<int64_t>(hist_count_ / P, hist_steps_);
     for (int64_t k = 1; k <= steps; ++k)
       if (!hist_b_ok_[(int)((hist_count_ - k * P) % slots)]) return false;
     return steps > 0;
@@ -443,46 +474,49 @@ class Solver {
   // out = the same-slot polynomial extrapolation from ring `ring` (elements per slot n).
   bool extrapolate_history(const double* ring, int64_t n, double* out) {
     const int P = history_period_;
-    const int64_t steps = hist_count_ / P >= 3 ? 3 : hist_count_ / P;
+    const int64_t steps = std::min<int64_t>(hist_count_ / P, hist_steps_);
     if (steps == 0) return false;
-    const int slots = kHistSteps * P;
+    const int slots = hist_ring_steps() * P;
     auto index = [&](int back) { return (int)((hist_count_ - (int64_t)back * P) % slots); };
-    auto slot = [&](int back) { return ring + index(back) * n; };
     if (steps == 1) {
-      copy_cols(slot(1), n, out, n, n, 1, stream_);
+      copy_cols(ring + index(1) * n, n, out, n, n, 1, stream_);
       return true;
     }
-    double w[3];
+    double w[kMaxHistSteps];
+    const double* src[kMaxHistSteps];
     history_weights((int)steps, index, w);
-    extrapolate(slot(1), slot(2), steps == 3 ? slot(3) : nullptr, w, out, n, stream_);
+    for (int k = 0; k < steps; ++k) src[k] = ring + index(k + 1) * n;
+    extrapolate(src, w, (int)steps, out, n, stream_);
     return true;
   }
   // Lagrange weights at the new solve's time for the last `steps` same-slot solves.
   template <class Index>
-  void history_weights(int steps, Index index, double w[3]) const {
-    double h[4] = {shift_cur_, 0.0, 0.0, 0.0};
+  void history_weights(int steps, Index index, double* w) const {
+    double h[kMaxHistSteps + 1] = {shift_cur_};
     for (int k = 1; k <= steps; ++k) h[k] = hist_shift_[index(k)];
     bool equal = true, known = std::isfinite(h[0]) && h[0] > 0.0;
     for (int k = 1; k <= steps; ++k) {
       equal = equal && h[k] == h[0];
       known = known && std::isfinite(h[k]) && h[k] > 0.0;
     }
-    if (equal || !known) {  // equal steps (or no step information): the fixed weights
-      w[0] = steps == 3 ? 3.0 : 2.0;
-      w[1] = steps == 3 ? -3.0 : -1.0;
-      w[2] = steps == 3 ? 1.0 : 0.0;
+    if (equal || !known) {  // equal steps (or no step information): binomial weights
+      double c = 1.0;       // C(steps, k), built up k by k
+      for (int k = 1; k <= steps; ++k) {
+        c = c * (steps - k + 1) / k;
+        w[k - 1] = (k % 2 == 1) ? c : -c;
+      }
       return;
     }
-    double x[4] = {0.0, 0.0, 0.0, 0.0};  // node times relative to the new solve
+    double x[kMaxHistSteps + 1] = {0.0};  // node times relative to the new solve
     for (int k = 1; k <= steps; ++k) x[k] = x[k - 1] - 0.5 * (h[k - 1] + h[k]);
-    for (int k = 1; k <= 3; ++k) {
+    for (int k = 1; k <= steps; ++k) {
       double num = 1.0, den = 1.0;
       for (int m = 1; m <= steps; ++m) {
         if (m == k) continue;
         num *= -x[m];
         den *= x[k] - x[m];
       }
-      w[k - 1] = k <= steps ? num / den : 0.0;
+      w[k - 1] = num / den;
     }
   }
   // Records the solve's pressures (and, for the mixed solve, velocities: its outer
@@ -494,7 +528,7 @@ class Solver {
     prev_b_ok_ = b != nullptr;
     if (b) copy_cols(b, np_, bprev_, np_, np_, ncols, stream_);
     if (history_period_ > 0) {
-      const int slots = kHistSteps * history_period_;
+      const int slots = hist_ring_steps() * history_period_;
       const bool with_u = u != nullptr;
       if (hist_ncols_ != ncols || hist_slots_ < slots || hist_has_u_ != with_u) {
         HIP_CHECK(hipStreamSynchronize(stream_));
@@ -509,6 +543,7 @@ class Solver {
         hist_slots_ = slots;
         hist_has_u_ = with_u;
         hist_count_ = 0;
+        hist_committed_ = 0;
       }
       const int slot = (int)(hist_count_ % slots);
       hist_shift_[slot] = shift_cur_;
@@ -554,7 +589,12 @@ class Solver {
       start_from(guess, guess_stride, ncols, mass, visc);
     } else if (learned_guess(b_, ncols)) {
       out.warm = true;
-      start_from(io_g_, np_, ncols, mass, visc, true);
+      if (net_base_ != nullptr) {
+        out.network = true;
+        start_two_term(net_base_, ncols, mass, visc);
+      } else {
+        start_from(io_g_, np_, ncols, mass, visc, true);
+      }
     }
     pcg(ncols, mass, visc, out.iterations, true);
     // p = x - mean(x)
@@ -634,6 +674,7 @@ class Solver {
     copy_cols(pacc_, np_, pres, pres_stride, np_, ncols, stream_);
     copy_cols(uacc_, size_, vel, vel_stride, size_, ncols, stream_);
     remember_pressure(pacc_, ncols, nullptr, bzero_);
+    copy_cols(bzero_, np_, b_, np_, np_, ncols, stream_);  // b_ is the solve's zero-start b again
   }
 
   // MIXED: FP64 defect correction around bf16-transform inner solves.
@@ -670,12 +711,14 @@ class Solver {
     bool bzero_ok = false;  // bzero_ holds this solve's zero-start Schur right-hand side
     if (guess) {
       copy_cols(guess, guess_stride, pacc_, np_, np_, ncols, stream_);
-    } else if (warm_mode_ != MAC_WARM_OFF && warm_mode_ != MAC_WARM_NETWORK &&
+    } else if ((warm_mode_ == MAC_WARM_HISTORY || warm_mode_ == MAC_WARM_PREVIOUS ||
+                (warm_mode_ == MAC_WARM_NETWORK && recorded_steps(ncols) >= net_depth_limit_)) &&
                learned_guess(nullptr, ncols, pacc_)) {
       // With a history, the velocities start from their own extrapolation too: a good
       // pressure with u = 0 leaves an O(1) velocity defect, whose bf16 correction (after
-      // D) would swamp the pressure's accuracy.
-      if (warm_mode_ == MAC_WARM_HISTORY) history_velocity(ncols, uacc_);
+      // D) would swamp the pressure's accuracy.  (A network handle with a deep enough
+      // history takes the extrapolation alone, without forming b for the network.)
+      if (history_period_ > 0) history_velocity(ncols, uacc_);
       out.warm = true;
     } else {
       HIP_CHECK(hipMemsetAsync(pacc_, 0, sizeof(double) * np_ * ncols, stream_));
@@ -694,6 +737,7 @@ class Solver {
       if (learned_guess(b_, ncols, pacc_)) {
         if (history_period_ > 0) history_velocity(ncols, uacc_);
         out.warm = true;
+        out.network = net_base_ != nullptr;
       }
     }
     std::vector<int> inner(ncols);
@@ -823,6 +867,38 @@ class Solver {
       sub_cols(b_, q_, r_, np_, np_, ncols, cg_, stream_);
       if (accumulate_) accum_velocity(du_, wf_, size_, size_, ncols, cg_, nullptr, sums_, stream_);
     }
+    reduce(r_, r_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(rr_));
+  }
+
+  // The network's start in the FP64 solve: the combination a g + beta c of the base g and
+  // the network's correction c = guess - g with the smallest starting residual
+  // ||b - S(a g + beta c)||_2, never larger than the best multiple of the base alone.
+  // S c costs one Schur product (as the one-term start); S g is the base's recorded
+  // right-hand side b - (b - b_base), exact to the recorded solves' tolerance.  On the
+  // columns whose CG runs.
+  void start_two_term(const double* g, int ncols, double mass, double visc) {
+    std::vector<CgState> st(ncols);
+    HIP_CHECK(hipMemcpyAsync(st.data(), cg_, sizeof(CgState) * ncols, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    std::vector<int> live;
+    for (int c = 0; c < ncols; ++c)
+      if (st[c].active) live.push_back(c);
+    if (live.empty()) return;
+    const int64_t n = np_ * ncols;
+    {
+      const double w[2] = {1.0, -1.0};
+      const double* cdiff[2] = {io_g_, g};
+      extrapolate(cdiff, w, 2, z_, n, stream_);  // c = guess - base
+      const double* bdiff[2] = {b_, warm_db_};
+      extrapolate(bdiff, w, 2, p_, n, stream_);  // b_base = b - (b - b_base)
+    }
+    use_state_ = true;
+    schur(z_, q_, live, mass, visc, nullptr);    // q = S c
+    use_state_ = false;
+    const double* pairs[5][2] = {{p_, p_}, {p_, q_}, {q_, q_}, {p_, b_}, {q_, b_}};
+    for (int k = 0; k < 5; ++k)
+      reduce(pairs[k][0], pairs[k][1], np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(gal_ + k * ncols));
+    galerkin2_start(x_, r_, g, z_, b_, p_, q_, gal_, np_, ncols, cg_, stream_);
     reduce(r_, r_, np_, np_, ncols, RED_SUM, partials_, cg_, stream_, fin(rr_));
   }
 
@@ -1056,6 +1132,7 @@ class Solver {
     sums3_ = (double*)take(sizeof(double) * (ncols + 4));
     rr_ = (double*)take(sizeof(double) * (ncols + 4));
     cg_ = (CgState*)take(sizeof(CgState) * ncols);
+    gal_ = (double*)take(sizeof(double) * 8 * ncols);
     flags_ = (int*)take(sizeof(int) * ncols);
     cap_ = ncols;
   }
@@ -1102,14 +1179,16 @@ class Solver {
   double cnn_alpha_ = 1.0;
   int warm_mode_ = 0;
   int history_period_ = 0, hist_ncols_ = 0, hist_slots_ = 0;
-  int64_t hist_count_ = 0;
-  static constexpr int kHistSteps = 3;  // steps kept per stage slot
-  double shift_cur_ = 0.0;                     // viscosity / mass of the solve in progress
-  double hist_shift_[kHistSteps * 16] = {};    // and of each recorded slot (period <= 16)
-  double* hist_ = nullptr;      // ring of 3*period recorded pressures (np x ncols each)
+  int64_t hist_count_ = 0, hist_committed_ = 0;
+  static constexpr int kMaxHistSteps = kMaxExtrapolation;  // deepest extrapolation (degree 5)
+  int hist_steps_ = 6;                         // steps the extrapolation uses
+  int hist_ring_steps() const { return hist_steps_ + 1; }  // + one step for a trial in progress
+  double shift_cur_ = 0.0;                             // viscosity / mass of the solve in progress
+  double hist_shift_[(kMaxHistSteps + 1) * 16] = {};   // and of each recorded slot (period <= 16)
+  double* hist_ = nullptr;      // ring of (steps+1)*period recorded pressures (np x ncols each)
   double* hist_u_ = nullptr;    // and velocities (size x ncols each; mixed solves)
   double* hist_b_ = nullptr;    // and zero-start Schur right-hand sides (np x ncols each)
-  bool hist_b_ok_[kHistSteps * 16] = {};       // slot's right-hand side recorded
+  bool hist_b_ok_[(kMaxHistSteps + 1) * 16] = {};      // slot's right-hand side recorded
   double* bprev_ = nullptr;     // the previous solve's zero-start Schur right-hand side
   bool prev_b_ok_ = false;
   double* bzero_ = nullptr;     // mixed network solves: this solve's, kept from the sweeps
@@ -1123,6 +1202,9 @@ class Solver {
   double *partials_ = nullptr, *sums_ = nullptr, *sums2_ = nullptr, *sums3_ = nullptr, *rr_ = nullptr;
   double* means_ = nullptr;  // mixed CG: column means of z and q (kMaxCols each)
   CgState* cg_ = nullptr;
+  double* gal_ = nullptr;          // two-term Galerkin start: five dot products per column
+  const double* net_base_ = nullptr;  // base guess the network corrected in this solve, if it ran
+  int net_depth_limit_ = 3;        // the network serves solves with fewer recorded steps than this
   int* flags_ = nullptr;
   int* host_flags_ = nullptr;
   Mailbox* mailbox_ = nullptr;      // host view (coherent pinned memory)
@@ -1295,7 +1377,7 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
       report->seconds = secs;
       report->sweeps = out.sweeps;
       report->total_iterations = total;
-      report->warm_started = out.warm ? 1 : 0;
+      report->warm_started = out.warm ? (out.network ? 2 : 1) : 0;
       report->first_residual = out.first_residual;
       report->first_divergence = out.first_divergence;
     }
@@ -1325,6 +1407,27 @@ int32_t mac_stokes_set_warm_history(void* h, int32_t period) {
   Solver* s = static_cast<Solver*>(h);
   if (!s || period < 0 || period > 16) return MAC_INVALID_ARGUMENT;
   return guarded(s, [&] { s->set_history(period); });
+}
+
+int32_t mac_stokes_set_history_steps(void* h, int32_t steps) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s || steps < 1 || steps > macs::kMaxExtrapolation) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] { s->set_history_steps(steps); });
+}
+
+int32_t mac_stokes_history_mark(void* h, int32_t rollback) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] {
+    if (rollback) s->rollback_history();
+    else s->commit_history();
+  });
+}
+
+int32_t mac_stokes_set_network_depth(void* h, int32_t steps) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s || steps < 1 || steps > macs::kMaxExtrapolation + 1) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] { s->set_net_depth_limit(steps); });
 }
 
 int32_t mac_stokes_set_warm_start(void* h, int32_t mode, void* cnn, double alpha) {

@@ -43,6 +43,7 @@ EXPORTED = (
     "mac_cnn_num_params",
     "mac_cnn_set_params",
     "mac_cnn_get_params",
+    "mac_cnn_plan",
     "mac_cnn_forward",
     "mac_cnn_loss_grad",
     "mac_cnn_train_step",
@@ -50,6 +51,9 @@ EXPORTED = (
     "mac_stokes_reset_history",
     "mac_stokes_set_warm_history",
     "mac_stokes_last_schur_rhs",
+    "mac_stokes_set_history_steps",
+    "mac_stokes_history_mark",
+    "mac_stokes_set_network_depth",
     "mac_warm_prepare",
     "mac_apply_divergence",
     "mac_apply_divergence_transpose",
@@ -165,6 +169,8 @@ def load() -> ctypes.CDLL:
     lib.mac_cnn_set_params.restype = _I32
     lib.mac_cnn_get_params.argtypes = [_P, _P]
     lib.mac_cnn_get_params.restype = _I32
+    lib.mac_cnn_plan.argtypes = [_I32, _I32, _I32, _I32, _P]
+    lib.mac_cnn_plan.restype = _I32
     lib.mac_cnn_forward.argtypes = [_P, _P, _P, _I32, _I32, _I32]
     lib.mac_cnn_forward.restype = _I32
     lib.mac_cnn_loss_grad.argtypes = [_P, _P, _P, _I32, _I32, _I32, _P, F]
@@ -177,6 +183,12 @@ def load() -> ctypes.CDLL:
     lib.mac_stokes_reset_history.restype = _I32
     lib.mac_stokes_set_warm_history.argtypes = [_P, _I32]
     lib.mac_stokes_set_warm_history.restype = _I32
+    lib.mac_stokes_set_history_steps.argtypes = [_P, _I32]
+    lib.mac_stokes_set_history_steps.restype = _I32
+    lib.mac_stokes_history_mark.argtypes = [_P, _I32]
+    lib.mac_stokes_history_mark.restype = _I32
+    lib.mac_stokes_set_network_depth.argtypes = [_P, _I32]
+    lib.mac_stokes_set_network_depth.restype = _I32
     lib.mac_stokes_last_schur_rhs.argtypes = [_P, _P, _I64, _I32]
     lib.mac_stokes_last_schur_rhs.restype = _I32
     lib.mac_warm_prepare.argtypes = [_I32, _I32, _P, _I64, _P, _I64, _P, _I64, _I32, _D, _P, _P, _P]

@@ -139,6 +139,14 @@ class _Run:
         return self.result
 
 
+def _mark(model: Any, accepted: bool) -> None:
+    """Tells a model that keeps per-trial state (the backend's warm-start history) how the
+    trial ended."""
+    hook = getattr(model, "commit_trial" if accepted else "discard_trial", None)
+    if hook is not None:
+        hook()
+
+
 def _all_finite(model: Any, field_: Any) -> bool:
     check = getattr(model, "all_finite", None)
     return bool(check(field_)) if check is not None else bool(np.isfinite(field_).all())
@@ -166,6 +174,7 @@ def integrate(model: Any, scheme: Any, initial: State, steps: list[float] | Arra
         try:
             trial = scheme.step(model, history.state, float(h))
             diag = run.diagnose(trial)
+            _mark(model, True)
             run.accept(trial, diag)
             history.state, history.accepted_steps = trial.state, history.accepted_steps + 1
             if k + 1 in wanted:
@@ -454,6 +463,7 @@ def integrate_adaptive(model: Any, scheme: Any, initial: Any, T: float, initial_
             break
 
         if reason:
+            _mark(model, False)
             if partitioned:
                 part.drop()
             rejections += 1
@@ -470,6 +480,7 @@ def integrate_adaptive(model: Any, scheme: Any, initial: Any, T: float, initial_
             continue
 
         # ---- accept
+        _mark(model, True)
         rejections = 0
         prev_e, prev_u = e, e_u
         if scalar_control is not None:

@@ -132,22 +132,32 @@ class Pairs:
     p: list
 
 
-def base_guess(history: list, period: int | None) -> Any:
+def history_weights(steps: int) -> list[float]:
+    """Equal-step extrapolation weights through the last `steps` values:
+    (-1)^(k+1) C(steps, k), k = 1..steps (2, -1; 3, -3, 1; ...)."""
+    from math import comb
+    return [(-1.0) ** (k + 1) * comb(steps, k) for k in range(1, steps + 1)]
+
+
+def base_guess(history: list, period: int | None, depth: int = 3) -> Any:
     """The solver's base guess from the recorded pressures (oldest first) of solves with
-    the same column count: 3 p[-P] - 3 p[-2P] + p[-3P], 2 p[-P] - p[-2P] or p[-P] as
-    steps are recorded, zero before (P = period), or the previous pressure when there is
-    no period."""
+    the same column count: the same stage of the last min(depth, recorded) steps
+    extrapolated with `history_weights`, zero before (P = period), or the previous
+    pressure when there is no period."""
     if period:
-        steps = min(len(history) // period, 3)
-        if steps == 3:
-            return 3.0 * (history[-period] - history[-2 * period]) + history[-3 * period]
-        if steps == 2:
-            return 2.0 * history[-period] - history[-2 * period]
-        return history[-period] if steps == 1 else None
+        steps = min(len(history) // period, depth)
+        if steps == 0:
+            return None
+        w = history_weights(steps)
+        out = w[0] * history[-period]
+        for k in range(1, steps):
+            out = out + w[k] * history[-(k + 1) * period]
+        return out
     return history[-1] if history else None
 
 
-def collect_pairs(model, scheme, initial, steps: int, dt: float, period: int | None = None) -> Pairs:
+def collect_pairs(model, scheme, initial, steps: int, dt: float, period: int | None = None,
+                  depth: int = 3) -> Pairs:
     """Run `scheme` on `model` (whose backend must be an FP64 GpuTensorStokes) and record,
     for every stage solve that has a base guess, (b - b_base, base guess, pressure) per
     column as device tensors of shape (np, ncols); `period` as in
@@ -162,20 +172,20 @@ def collect_pairs(model, scheme, initial, steps: int, dt: float, period: int | N
 
     def recording(rhs, *, mass, viscosity, pressure_guess=None):
         res = original(rhs, mass=mass, viscosity=viscosity, pressure_guess=pressure_guess)
-        k = 1 if np.ndim(rhs) == 1 else np.shape(rhs)[1]
+        k = 1 if len(rhs.shape) == 1 else int(rhs.shape[1])
         b = backend.last_schur_rhs(k)
-        p = torch.as_tensor(np.asarray(res.pressure).reshape(model.grid.np, k), device=b.device)
+        p = torch.as_tensor(res.pressure, device=b.device).reshape(model.grid.np, k)
         past, past_b = history.setdefault(k, []), rhs_history.setdefault(k, [])
-        base = base_guess(past, period)
+        base = base_guess(past, period, depth)
         if base is not None:
             # Solves without a base guess are not learned: the solver then starts from zero.
-            pairs.b.append(b - base_guess(past_b, period))
+            pairs.b.append(b - base_guess(past_b, period, depth))
             pairs.base.append(base)
             pairs.p.append(p.clone())
         past.append(p.clone())
         past_b.append(b.clone())
-        del past[:-3 * max(period or 1, 1)]
-        del past_b[:-3 * max(period or 1, 1)]
+        del past[:-depth * max(period or 1, 1)]
+        del past_b[:-depth * max(period or 1, 1)]
         return res
 
     backend.solve = recording

@@ -96,6 +96,7 @@ class GpuTensorStokes:
         self.sweeps = 0
         self.last_sweeps = 0
         self.warm_starts = 0
+        self.network_starts = 0
         self.last_report: dict[str, float] = {}
         self.warm_start = None
 
@@ -177,26 +178,37 @@ class GpuTensorStokes:
         self.last_iterations = tuple(int(i) for i in iters)
         self.sweeps += int(report.sweeps)
         self.last_sweeps = int(report.sweeps)
-        self.warm_starts += int(report.warm_started)
+        self.warm_starts += int(report.warm_started > 0)
+        self.network_starts += int(report.warm_started == 2)
         self.last_report = {"residual": report.residual, "divergence_inf": report.divergence_inf,
                             "velocity_max": report.velocity_max, "seconds": report.seconds,
                             "sweeps": int(report.sweeps), "total_iterations": int(report.total_iterations),
-                            "warm_started": bool(report.warm_started),
+                            "warm_started": bool(report.warm_started), "network": report.warm_started == 2,
                             "first_residual": report.first_residual,
                             "first_divergence": report.first_divergence}
         _native.raise_for(code, message)
         return self.last_report
 
     # ---------------------------------------------------- learned warm start
-    def set_warm_start(self, net: Any | None, period: int | None = None) -> None:
+    def set_warm_start(self, net: Any | None, period: int | None = None,
+                       history_steps: int | None = None, network_depth: int | None = None) -> None:
         """Starting pressure when no `pressure_guess` is passed: a trained `WarmStartNet`
         (its correction of a base guess), "previous" (the previous solve's pressures),
         "history" (the same stage of the last three steps extrapolated quadratically in
-        time, with `period` stage solves per step), or None (zero, as the reference).  `period` also
+        time, with `period` stage solves per step; `history_steps` sets how many steps
+        the extrapolation reaches back, 1..6, default 6), or None (zero, as the
+        reference).  With a network and a period, the network corrects the extrapolation
+        only while fewer than `network_depth` steps are recorded (default 3).  `period` also
         makes the network's base the history extrapolation.  FP64 starts the Schur CG
         from the Galerkin-scaled guess; mixed takes it as the initial outer iterate.
         Either way the solve runs to the same tolerance, so results are unchanged to
         tolerance."""
+        if network_depth is not None:
+            _native.raise_for(self._lib.mac_stokes_set_network_depth(self._handle, int(network_depth)),
+                              "network_depth must be in 1..7")
+        if history_steps is not None:
+            _native.raise_for(self._lib.mac_stokes_set_history_steps(self._handle, int(history_steps)),
+                              "history_steps must be in 1..6")
         if period is not None:
             _native.raise_for(self._lib.mac_stokes_set_warm_history(self._handle, int(period)),
                               "period must be in 0..16")
@@ -218,6 +230,15 @@ class GpuTensorStokes:
     def reset_history(self) -> None:
         """Forget the previous solve's pressures (they feed the warm start)."""
         self._lib.mac_stokes_reset_history(self._handle)
+
+    def commit_history(self) -> None:
+        """The solves since the last mark belong to an accepted step: keep them."""
+        _native.raise_for(self._lib.mac_stokes_history_mark(self._handle, 0), "history mark failed")
+
+    def discard_history(self) -> None:
+        """The solves since the last mark belong to a rejected or failed trial: forget
+        them, so later guesses extrapolate accepted steps only."""
+        _native.raise_for(self._lib.mac_stokes_history_mark(self._handle, 1), "history mark failed")
 
     def last_schur_rhs(self, ncols: int) -> Any:
         """The last FP64 solve's Schur right-hand sides b, (np, ncols) device tensor."""

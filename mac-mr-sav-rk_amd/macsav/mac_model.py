@@ -195,6 +195,18 @@ class MACNavierStokes:
         return Stage(pressure, residual, divergence_inf, r, list(candidates), list(root_residuals),
                      scalar_residual, continuity_residual)
 
+    # The drivers mark every trial's end: a backend that keeps solves for its warm start
+    # (the same-stage history) keeps an accepted step's and forgets a rejected one's.
+    def commit_trial(self) -> None:
+        mark = getattr(self.backend, "commit_history", None)
+        if mark is not None:
+            mark()
+
+    def discard_trial(self) -> None:
+        mark = getattr(self.backend, "discard_history", None)
+        if mark is not None:
+            mark()
+
     # ------------------------------------------------------- diagnostics
     def angular_momentum(self, velocity: Any) -> float:
         return float(self.fields.dot(angular_weights(self.grid), velocity))

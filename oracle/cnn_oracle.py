@@ -171,3 +171,14 @@ def unflatten(flat: np.ndarray, width: int = 32, cin: int = 2):
         out.append((flat[k:k + nw].reshape(co, ci, 3, 3).copy(), flat[k + nw:k + nw + co].copy()))
         k += nw + co
     return out
+
+
+def adam_step(w, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba, 2015, Algorithm 1) in float64: the moments, their
+    bias corrections 1 - b^step, and w - lr m_hat / (sqrt(v_hat) + eps).  Returns the new
+    (w, m, v)."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    m_hat = m / (1 - b1 ** step)
+    v_hat = v / (1 - b2 ** step)
+    return w - lr * m_hat / (np.sqrt(v_hat) + eps), m, v

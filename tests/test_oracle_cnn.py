@@ -69,3 +69,50 @@ def test_flatten_roundtrip():
     for (a, b), (c, d) in zip(params, back):
         np.testing.assert_array_equal(a, c)
         np.testing.assert_array_equal(b, d)
+
+
+def test_adam_first_step_is_the_signed_learning_rate_and_moments_follow_the_recurrence():
+    # Closed forms: with zero moments, one step gives m_hat = g and v_hat = g^2, so the
+    # update is lr g / (|g| + eps) ~ lr sign(g); a second step with gradient g2 has
+    # m = 0.9 (0.1 g) + 0.1 g2 and v = 0.999 (0.001 g^2) + 0.001 g2^2.
+    g = np.array([0.3, -2.0, 1e-3])
+    w0 = np.array([1.0, 2.0, -1.0])
+    w1, m1, v1 = co.adam_step(w0, g, np.zeros(3), np.zeros(3), 1, lr=0.01)
+    np.testing.assert_allclose(w1, w0 - 0.01 * g / (np.abs(g) + 1e-8), rtol=1e-12)
+    g2 = np.array([-0.1, 0.5, 0.2])
+    w2, m2, v2 = co.adam_step(w1, g2, m1, v1, 2, lr=0.01)
+    np.testing.assert_allclose(m2, 0.9 * 0.1 * g + 0.1 * g2, rtol=1e-12)
+    np.testing.assert_allclose(v2, 0.999 * 0.001 * g * g + 0.001 * g2 * g2, rtol=1e-12)
+    mh, vh = m2 / (1 - 0.81), v2 / (1 - 0.999 ** 2)
+    np.testing.assert_allclose(w2, w1 - 0.01 * mh / (np.sqrt(vh) + 1e-8), rtol=1e-12)
+
+
+def test_network_buffer_plan_covers_every_pass_extent():
+    """The host-side plan the network sizes its buffers by (mac_cnn_plan, no GPU call):
+    each buffer holds the largest extent a pass writes or reads in it, for the shapes the
+    solver and the training use.  The extents are derived here from the topology (4
+    levels, 2x2 pooling, `width` channels; weight-gradient partials per 16x16 tile)."""
+    import ctypes
+    from macsav import _native
+    lib = _native.load()
+    for width, n, H, W in [(32, 2, 256, 256), (32, 16, 128, 128), (8, 3, 8, 8), (32, 1, 512, 256), (32, 8, 32, 64)]:
+        counts = (ctypes.c_int64 * 29)()
+        assert lib.mac_cnn_plan(width, n, H, W, counts) == 0
+        c = list(counts)
+        full = n * width * H * W
+        level = [full, full // 4, full // 16, full // 64]
+        # e0 p1 e1 p2 e2 p3 e3 u2 g2 u1 g1 u0 y dy du0 dg1 du1 dg2 du2 de3 dp3 de2 dp2 de1 dp1 de0
+        want = [level[0], level[1], level[1], level[2], level[2], level[3], level[3], level[2], level[2], level[1],
+                level[1], level[0], n * H * W, n * H * W, level[0], level[1], level[1], level[2], level[2], level[3],
+                level[3], level[2], level[2], level[1], level[1], level[0]]
+        assert c[:26] == want
+        tiles = -(-H // 16) * -(-W // 16) * n
+        layers = [(width, 2)] + [(width, width)] * 5 + [(1, width)]
+        for l, (cout, cin) in enumerate(layers):
+            scale = 2 ** min(l, 6 - l)                  # resolution of layer l: H/1, /2, /4, /8, /4, /2, /1
+            tl = -(-(H // scale) // 16) * -(-(W // scale) // 16) * n
+            assert tl * (cout * cin * 9 + cout) <= c[26]
+        assert c[26] == tiles * (width * width * 9 + width)
+        assert c[27] >= 256 and 2 * c[28] >= width * width * 9
+    bad = (ctypes.c_int64 * 29)()
+    assert lib.mac_cnn_plan(32, 1, 12, 16, bad) == _native.MAC_INVALID_ARGUMENT
