@@ -250,8 +250,9 @@ def bench_learned(args) -> dict:
     pressure, pressure) pairs of the training seeds' FP64 solves (13 steps each, about
     100 pairs per seed), then every held-out seed's run solved with the zero start (the
     reference), the previous-stage pressure (Galerkin-scaled) and the network's
-    two-term Galerkin start over {previous, correction}, all to Schur-CG rtol 1e-12 and
-    the 1e-10 gate; CG iterations and wall time per start."""
+    two-term start over {previous, correction}, all by the FP64 form at the bar (Schur-CG
+    rtol 1e-12 plus FP64 defect correction to 1e-10); CG iterations and wall time per
+    start."""
     import numpy as np
     import torch
     from macsav import GpuTensorStokes, MACGrid
@@ -267,7 +268,9 @@ def bench_learned(args) -> dict:
     steps = 13
 
     def backend(warm=None, net=None):
-        b = GpuTensorStokes(grid, tolerance=BAR, krylov_tolerance=1e-12)
+        # The FP64 form at the bar: Schur-CG rtol 1e-12, then FP64 defect correction to
+        # residual and max|D u| <= 1e-10 (its iterations count with the CG's).
+        b = GpuTensorStokes(grid, tolerance=BAR, krylov_tolerance=1e-12, fp64_refine=3)
         if warm == "previous":
             b.set_warm_start("previous")
         elif warm == "network":
@@ -572,10 +575,13 @@ def bench_stage1024(args) -> dict:
         net, net_info = train_cold_start_network(grid)
         print(f"[stage1024] network {net_info}", file=sys.stderr, flush=True)
     forms, velocities, profs = {}, {}, {}
+    wanted = set(args.forms.split(",")) if args.forms else set(FORMS)
+    wanted |= {"fp64", "fp64+refine"}          # the baselines every line reports
     for form in FORMS:
-        if form == "mixed+network" and net is None:
+        if form not in wanted or (form == "mixed+network" and net is None):
             continue
-        profile = form == "mixed+network" or (net is None and form == "mixed+history")
+        profile = form == next(f for f in ("mixed+network", "mixed+history", "mixed", "fp64+refine")
+                               if f in wanted and (f != "mixed+network" or net is not None))
         forms[form], profs[form], velocities[form] = replay(grid, rec, fill, W, K, form, net, profile)
         print(f"[stage1024] {form}: {forms[form]}", file=sys.stderr, flush=True)
 
@@ -591,10 +597,10 @@ def bench_stage1024(args) -> dict:
         forms[form]["speedup_vs_fp64"] = forms[form]["stage_solves_per_s"] / forms["fp64"]["stage_solves_per_s"]
     runs = {}
     for form in ("fp64+refine", "mixed+history", "mixed+network"):
-        if form in forms:
+        if form in forms and not args.no_whole_run:
             runs[form] = whole_run(grid, rec, form, net)
             print(f"[stage1024] whole run {form}: {runs[form]}", file=sys.stderr, flush=True)
-    head = "mixed+network" if "mixed+network" in forms else "mixed+history"
+    head = next(f for f in ("mixed+network", "mixed+history", "mixed", "fp64+refine") if f in forms)
     main_stats, prof = forms[head], profs[head]
     gemm_s = prof["gemm_ms"] * 1e-3
     launches = max(prof["gemm_launches"], 1)
@@ -657,6 +663,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-network", action="store_true")
+    ap.add_argument("--no-whole-run", action="store_true", help="skip timing every solve of the run")
+    ap.add_argument("--forms", default="", help="comma-separated subset of the forms (the two FP64 baselines "
+                                                "always run)")
     ap.add_argument("--workload", choices=["stage1024", "learned", "cavities", "run", "combined", "sizes"],
                     default="stage1024")
     ap.add_argument("--t-end", type=float, default=0.05, help="combined run: final time")

@@ -702,7 +702,7 @@ class Solver {
       copy_cols(f, f_stride, fcomp_, size_, size_, ncols, stream_);
       fc = fcomp_;
     }
-    HIP_CHECK(hipMemsetAsync(uacc_, 0, sizeof(double) * size_ * ncols, stream_));
+    bool u_set = false;  // uacc_ holds a velocity guess (else it starts from zero)
     // Starting pressure: the caller's guess, else the warm start's (history, previous
     // solve or network), else zero.  It is the outer iterate, so the first FP64 defect
     // measures its true error and every sweep improves on it; started inside a bf16 inner
@@ -718,7 +718,7 @@ class Solver {
       // pressure with u = 0 leaves an O(1) velocity defect, whose bf16 correction (after
       // D) would swamp the pressure's accuracy.  (A network handle with a deep enough
       // history takes the extrapolation alone, without forming b for the network.)
-      if (history_period_ > 0) history_velocity(ncols, uacc_);
+      if (history_period_ > 0) u_set = history_velocity(ncols, uacc_);
       out.warm = true;
     } else {
       HIP_CHECK(hipMemsetAsync(pacc_, 0, sizeof(double) * np_ * ncols, stream_));
@@ -735,11 +735,12 @@ class Solver {
       copy_cols(b_, np_, bzero_, np_, np_, ncols, stream_);  // the sweeps reuse b_
       bzero_ok = true;
       if (learned_guess(b_, ncols, pacc_)) {
-        if (history_period_ > 0) history_velocity(ncols, uacc_);
+        if (history_period_ > 0) u_set = history_velocity(ncols, uacc_);
         out.warm = true;
         out.network = net_base_ != nullptr;
       }
     }
+    if (!u_set) HIP_CHECK(hipMemsetAsync(uacc_, 0, sizeof(double) * size_ * ncols, stream_));
     std::vector<int> inner(ncols);
     // r_u goes straight into the transform's input buffer, in the precision the split
     // (or plain bf16) transform keeps of it: FP32 holds every bit of its bf16 hi + lo.
@@ -757,6 +758,9 @@ class Solver {
       out.divergence = 0.0;
       for (int c = 0; c < ncols; ++c) out.divergence = std::max(out.divergence, (double)mb->vals[3 + c]);
       if (sweep == 0) {
+        // The first defect also measures max|f|: a non-finite right-hand side is an
+        // invalid argument, as the reference rejects it (tensor_stokes.py:246-257).
+        if (!std::isfinite(out.fmax)) throw ArgFailure("Invalid Stokes RHS");
         out.first_residual = out.residual;
         out.first_divergence = out.divergence;
       }
@@ -910,23 +914,34 @@ class Solver {
     std::vector<int> active;
     bool settled = false;  // every column left the loop through the convergence test
     for (int it = 0; it < opts_.max_iterations; ++it) {
-      // From the second iteration on, (r, r) arrives as the update's partials.
-      cg_check_post(cg_, rr_, flags_, ncols, mailbox_dev_, ++seq_, stream_, it == 0 ? nullptr : partials_,
-                    reduce_blocks(np_));
-      const volatile Mailbox* mb = wait_post(seq_);
-      active.clear();
-      for (int c = 0; c < ncols; ++c)
-        if (mb->flags[c]) active.push_back(c);
-      if (active.empty()) {
-        use_state_ = false;
-        for (int c = 0; c < ncols; ++c) {
-          iters[c] = mb->iters[c];
-          if (fail && !mb->converged[c])
-            throw SolveFailure(MAC_NOT_CONVERGED,
-                               "Pressure Schur CG failed with info=" + std::to_string(opts_.max_iterations));
+      if (it == 0 && mixed_) {
+        // A mixed sweep's inner CG starts from x = 0 on b != 0 (a sweep runs only while the
+        // defect is above the bar): every column with b != 0 takes at least one iteration,
+        // so the first check runs on the device alone and the iteration is queued behind
+        // it without a host round trip.  A column with b = 0 is inactive on the device and
+        // every vector kernel skips it; its transform columns compute into scratch only.
+        cg_check(cg_, rr_, flags_, ncols, stream_);
+        active.clear();
+        for (int c = 0; c < ncols; ++c) active.push_back(c);
+      } else {
+        // From the second iteration on, (r, r) arrives as the update's partials.
+        cg_check_post(cg_, rr_, flags_, ncols, mailbox_dev_, ++seq_, stream_, it == 0 ? nullptr : partials_,
+                      reduce_blocks(np_));
+        const volatile Mailbox* mb = wait_post(seq_);
+        active.clear();
+        for (int c = 0; c < ncols; ++c)
+          if (mb->flags[c]) active.push_back(c);
+        if (active.empty()) {
+          use_state_ = false;
+          for (int c = 0; c < ncols; ++c) {
+            iters[c] = mb->iters[c];
+            if (fail && !mb->converged[c])
+              throw SolveFailure(MAC_NOT_CONVERGED,
+                                 "Pressure Schur CG failed with info=" + std::to_string(opts_.max_iterations));
+          }
+          settled = true;
+          break;
         }
-        settled = true;
-        break;
       }
       if (mixed_) {
         // The mixed inner CG folds the mean removals of z and q into their consumers:
@@ -1348,14 +1363,18 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
       pr = s->io_p_;
       lp = np;
     }
-    // Non-finite RHS values are rejected like the reference (ValueError).
-    macs::reduce(f, nullptr, size, lf, ncols, macs::RED_MAXABS, s->scratch_partials(), nullptr, st);
-    macs::finalize(s->scratch_partials(), macs::reduce_blocks(size), ncols, macs::RED_MAXABS, s->scratch_sums(), st);
-    std::vector<double> fm(ncols);
-    HIP_CHECK(hipMemcpyAsync(fm.data(), s->scratch_sums(), sizeof(double) * ncols, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    for (double v : fm)
-      if (!is_finite(v)) throw macs::ArgFailure("Invalid Stokes RHS");
+    // Non-finite RHS values are rejected like the reference (ValueError); the mixed solve
+    // checks max|f| from its first defect instead of a pass of its own.
+    if (s->opts().precision != MAC_PRECISION_MIXED) {
+      macs::reduce(f, nullptr, size, lf, ncols, macs::RED_MAXABS, s->scratch_partials(), nullptr, st);
+      macs::finalize(s->scratch_partials(), macs::reduce_blocks(size), ncols, macs::RED_MAXABS, s->scratch_sums(),
+                     st);
+      std::vector<double> fm(ncols);
+      HIP_CHECK(hipMemcpyAsync(fm.data(), s->scratch_sums(), sizeof(double) * ncols, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      for (double v : fm)
+        if (!is_finite(v)) throw macs::ArgFailure("Invalid Stokes RHS");
+    }
     Solver::Outcome out = s->solve(f, lf, ncols, mass, viscosity, vel, lv, pr, lp, gs, lg);
     HIP_CHECK(hipStreamSynchronize(st));
     if (!device_pointers) {
