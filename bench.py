@@ -115,15 +115,15 @@ def record_run(grid, steps: int, seed: int = 0, modes: int = 32, nu: float = NU,
     return rec
 
 
-def make_solver(grid, form: str, net=None):
+def make_solver(grid, form: str, net=None, history_steps: int = HISTORY_STEPS):
     from macsav import GpuTensorStokes
     f = FORMS[form]
     s = GpuTensorStokes(grid, tolerance=BAR, krylov_tolerance=f.get("krylov", 1e-14), precision=f["precision"],
                         refine_tolerance=BAR, fp64_refine=f.get("refine", 0))
     if f.get("warm") == "history":
-        s.set_warm_start("history", period=STAGES, history_steps=HISTORY_STEPS)
+        s.set_warm_start("history", period=STAGES, history_steps=history_steps)
     elif f.get("warm") == "network":
-        s.set_warm_start(net, period=STAGES, history_steps=HISTORY_STEPS, network_depth=3)
+        s.set_warm_start(net, period=STAGES, history_steps=history_steps, network_depth=3)
     return s
 
 
@@ -144,7 +144,7 @@ def replay(grid, rec, fill: int, warmup: int, steps: int, form: str, net=None, p
     rec[fill + warmup:][:steps] timed; optionally a second pass with HIP events on the
     timed solves' transform GEMMs."""
     import torch
-    solver = make_solver(grid, form, net)
+    solver = make_solver(grid, form, net, HISTORY_STEPS)
     first = fill + warmup
 
     def run_once(keep=None, profiled=False):
@@ -183,11 +183,11 @@ def whole_run(grid, rec, form: str, net=None):
     """Every recorded stage solve timed in order from a fresh solver (cold start
     included), after one untimed solve of the last record on a throwaway solver."""
     import torch
-    scratch = make_solver(grid, form, net)
+    scratch = make_solver(grid, form, net, HISTORY_STEPS)
     f, m, v = rec[-1]
     scratch.solve(f, mass=m, viscosity=v)
     del scratch
-    solver = make_solver(grid, form, net)
+    solver = make_solver(grid, form, net, HISTORY_STEPS)
     torch.cuda.synchronize()
     before = _tally(solver)
     worst = [0.0, 0.0]
@@ -664,6 +664,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-network", action="store_true")
     ap.add_argument("--no-whole-run", action="store_true", help="skip timing every solve of the run")
+    ap.add_argument("--history-steps", type=int, default=HISTORY_STEPS,
+                    help="steps the history extrapolation reaches back (1..6)")
     ap.add_argument("--forms", default="", help="comma-separated subset of the forms (the two FP64 baselines "
                                                 "always run)")
     ap.add_argument("--workload", choices=["stage1024", "learned", "cavities", "run", "combined", "sizes"],
@@ -675,6 +677,8 @@ def main() -> int:
     ap.add_argument("--test-seeds", type=int, default=10)
     ap.add_argument("--epochs", type=int, default=12)
     args = ap.parse_args()
+    global HISTORY_STEPS
+    HISTORY_STEPS = args.history_steps
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
