@@ -36,6 +36,7 @@ combined (adaptive 2048^2 with lid), sizes (solve time against grid size).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -112,7 +113,24 @@ def record_run(grid, steps: int, seed: int = 0, modes: int = 32, nu: float = NU,
     state = model.state(0.0, stream_function_velocity(grid, seed=seed, modes=modes))
     for _ in range(steps):
         state = scheme.step(model, state, dt).state
+    del backend.solve        # the wrapper holds the backend: break the cycle so it is freed now
     return rec
+
+
+def quiesce(hold: bool = True):
+    """Free what earlier work left to the cyclic collector (a solver freed there releases
+    device memory, which synchronises the device) and finish queued device work, then
+    (hold) keep the collector off until `resume`: nothing of that kind lands in a timed
+    region.  Long runs that make garbage as they go pass hold=False."""
+    import torch
+    gc.collect()
+    torch.cuda.synchronize()
+    if hold:
+        gc.disable()
+
+
+def resume():
+    gc.enable()
 
 
 def make_solver(grid, form: str, net=None, history_steps: int = HISTORY_STEPS):
@@ -157,6 +175,7 @@ def replay(grid, rec, fill: int, warmup: int, steps: int, form: str, net=None, p
             solver.set_profiling(True)
         before = _tally(solver)
         worst = [0.0, 0.0]
+        quiesce()
         t0 = time.perf_counter()
         for f, m, v in rec[first:first + steps]:
             res = solver.solve(f, mass=m, viscosity=v)
@@ -166,6 +185,7 @@ def replay(grid, rec, fill: int, warmup: int, steps: int, form: str, net=None, p
                 keep.append(res.velocity)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        resume()
         return _stats(solver, before, steps, elapsed, worst), elapsed
 
     velocities = []
@@ -188,16 +208,18 @@ def whole_run(grid, rec, form: str, net=None):
     scratch.solve(f, mass=m, viscosity=v)
     del scratch
     solver = make_solver(grid, form, net, HISTORY_STEPS)
-    torch.cuda.synchronize()
     before = _tally(solver)
     worst = [0.0, 0.0]
+    quiesce()
     t0 = time.perf_counter()
     for f, m, v in rec:
         res = solver.solve(f, mass=m, viscosity=v)
         solver.commit_history()
         worst = [max(worst[0], res.residual), max(worst[1], res.divergence_inf)]
     torch.cuda.synchronize()
-    return _stats(solver, before, len(rec), time.perf_counter() - t0, worst)
+    elapsed = time.perf_counter() - t0
+    resume()
+    return _stats(solver, before, len(rec), elapsed, worst)
 
 
 def train_cold_start_network(grid, seeds=(11, 12, 13, 14, 15, 16), steps: int = 3):
@@ -361,7 +383,7 @@ def bench_cavities(args) -> dict:
     flat = torch.as_tensor(batch.pack(rhs), device="cuda:0")
     batch.solve_packed(flat, mass=mass, viscosity=visc)
     timings = batch.autotune(flat, mass=mass, viscosity=visc, repeats=1)
-    torch.cuda.synchronize()
+    quiesce()
     t0 = time.perf_counter()
     its = 0
     for _ in range(args.steps):
@@ -369,13 +391,14 @@ def bench_cavities(args) -> dict:
         its += sum(sum(c) for c in out["iterations"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    resume()
     value = n * args.steps / elapsed
     # Baseline: the same cavities one at a time through the single-cavity library path.
     singles = [GpuTensorStokes(g, tolerance=1e-9) for g in grids]
     dev = [torch.as_tensor(r, device="cuda:0") for r in rhs]
     for s, r, m, v in zip(singles, dev, mass, visc):
         s.solve(r, mass=m, viscosity=v)
-    torch.cuda.synchronize()
+    quiesce()
     t0 = time.perf_counter()
     worst = 0.0
     for s, r, m, v, k in zip(singles, dev, mass, visc, range(n)):
@@ -385,6 +408,7 @@ def bench_cavities(args) -> dict:
              for c in (0, 1)], 1)))))
     torch.cuda.synchronize()
     loop = n / (time.perf_counter() - t0)
+    resume()
     return {"metric": "cavity stage solves/s, batched uneven cavities", "value": value,
             "unit": "cavity stage solves/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
             "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -426,13 +450,14 @@ def bench_run(args) -> dict:
         state = model.state(0.0, x0)
         for _ in range(args.warmup):
             state = scheme.step(model, state, DT).state
-        torch.cuda.synchronize()
         solves0 = backend.solves
+        quiesce()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             state = scheme.step(model, state, DT).state
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        resume()
         u = model.vector(state)
         finals[form] = u.cpu().numpy() if hasattr(u, "cpu") else np.asarray(u)
         out[form] = {"steps_per_s": args.steps / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
@@ -490,6 +515,7 @@ def bench_combined(args) -> dict:
         def progress(k, t, secs, form=form):
             if k % 10 == 0:
                 print(f"[{form}] accepted {k} t={t:.5f} {secs:.1f}s", file=sys.stderr, flush=True)
+        quiesce(hold=False)
         t0 = time.perf_counter()
         res = integrate_adaptive(model, make_scheme("sdirk3_mrsav"), model.state(0.0, x0), t_end, first,
                                  controller=controller, progress=progress, retain_history=False)
@@ -540,12 +566,13 @@ def bench_sizes(args) -> dict:
                                 refine_tolerance=1e-10)
             for f, m, v in rec[:4]:
                 s.solve(f, mass=m, viscosity=v)
-            torch.cuda.synchronize()
+            quiesce()
             t0 = time.perf_counter()
             for f, m, v in rec[4:]:
                 s.solve(f, mass=m, viscosity=v)
             torch.cuda.synchronize()
             row[precision] = 1e3 * (time.perf_counter() - t0) / (len(rec) - 4)
+            resume()
         row["mixed_speedup"] = row["fp64"] / row["mixed"]
         rows[str(n)] = row
         print(f"[sizes] {n}: {row}", file=sys.stderr, flush=True)
@@ -652,6 +679,7 @@ def bench_stage1024(args) -> dict:
 
 
 def main() -> int:
+    global HISTORY_STEPS
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed stage solves")
@@ -677,7 +705,6 @@ def main() -> int:
     ap.add_argument("--test-seeds", type=int, default=10)
     ap.add_argument("--epochs", type=int, default=12)
     args = ap.parse_args()
-    global HISTORY_STEPS
     HISTORY_STEPS = args.history_steps
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
