@@ -246,7 +246,13 @@ const char* mac_noalloc_last_error(void);
  * cavity keeps its own grid, (mass, viscosity) and columns, and its results are those of
  * the cavity solved alone.  Segments (cavity i, column c) are laid out cavity-major,
  * column-minor; mac_batch_layout gives each segment's first element in the velocity and
- * pressure buffers (plus the totals at index count*ncols).  FP64 only. */
+ * pressure buffers (plus the totals at index count*ncols).
+ * precision MAC_PRECISION_FP64: every cavity's result is bitwise that of the cavity solved
+ * alone.  MAC_PRECISION_MIXED: per cavity the mixed form's outer FP64 defect correction
+ * (refine_tolerance, inner_tolerance, max_sweeps) around inner Schur-CG solves whose
+ * transforms are split bf16 (hi/lo parts of data and matrix, FP32 accumulation) and whose
+ * vectors, dot products, defects and updates are FP64; finished cavities drop out of the
+ * sweeps on the device. */
 void* mac_batch_create(int32_t count, const int32_t* nx, const int32_t* ny, const double* lx, const double* ly,
                        int32_t ncols, const mac_stokes_opts* opts);
 const char* mac_batch_create_error(void);
@@ -257,8 +263,12 @@ int32_t mac_batch_layout(void* handle, int64_t* velocity_offsets, int64_t* press
  * iters_per_check iterations between host reads of the convergence flags (1..16).  The
  * results do not depend on either. */
 int32_t mac_batch_set_launch(void* handle, int32_t use_graph, int32_t iters_per_check);
+/* Outer sweeps of the last batched solve (mixed form; 1 for FP64), -1 for a null handle. */
+int32_t mac_batch_last_sweeps(void* handle);
 /* mass, viscosity: per cavity.  iterations: per segment.  residual, divergence_inf: per
- * cavity (max over its columns, the reference's definitions).  Errors as mac_stokes_solve;
+ * cavity (max over its columns, the reference's definitions; in the mixed form those of the
+ * last FP64 defect; iterations then sum the inner CG iterations over the sweeps).  Errors
+ * as mac_stokes_solve;
  * the message names the first failing cavity. */
 int32_t mac_stokes_solve_batched(void* handle, const double* rhs, const double* mass, const double* viscosity,
                                  double* velocity, double* pressure, int32_t* iterations, double* residual,

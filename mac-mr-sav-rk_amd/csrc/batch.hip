@@ -42,7 +42,7 @@ enum Len : int { LEN_FACES = 0, LEN_CELLS = 1 };
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_bdiv(const double* __restrict__ vel, double* __restrict__ out,
                                                    const Seg* segs, double sign, double* partials, int pstride,
-                                                   const CgState* st) {
+                                                   const CgState* st, const double* __restrict__ add) {
   const int s = blockIdx.y;
   if (st != nullptr && !st[s].active) return;
   const Seg S = segs[s];
@@ -62,6 +62,7 @@ __global__ __launch_bounds__(kThreads) void k_bdiv(const double* __restrict__ ve
     if (j < ny - 1) value += v[j * nx + i] / S.hy;
     if (j > 0) value -= v[(j - 1) * nx + i] / S.hy;
     value = sign * value;
+    if (add != nullptr) value = add[S.poff + cell] + value;
     o[cell] = value;
     if (MODE == RED_SUM) accum += value;
     if (MODE == RED_MAXABS) accum = fmax(accum, absn(value));
@@ -196,11 +197,13 @@ __global__ __launch_bounds__(kThreads) void k_bupdate_xr(double* __restrict__ x,
   if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
 }
 
-// Per segment: max|mass u + visc K u - D^T p - f|, max|f|, max|u| (partials at q * nseg * pstride).
+// Per segment: max|mass u + visc K u - D^T p - f|, max|f|, max|u| (partials at q * nseg * pstride);
+// WRITE: the momentum residual r_u = f - (mass u + visc K u - D^T p) into ru as well.
+template <bool WRITE>
 __global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__ vel, const double* __restrict__ f,
                                                       const double* __restrict__ p, const Seg* segs,
                                                       const double* __restrict__ mass_visc, double* partials,
-                                                      int pstride) {
+                                                      int pstride, double* __restrict__ ru) {
   const int s = blockIdx.y;
   const Seg S = segs[s];
   if ((int)blockIdx.x >= S.nbv) return;
@@ -245,6 +248,7 @@ __global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__
     const double fv = fc[fidx];
     const double vv = vc[fidx];
     const double d = face_defect(mass, vv, visc, k.kv, lift, fv);
+    if (WRITE) ru[S.voff + fidx] = -d;
     md = fmax(md, absn(d));
     mf = fmax(mf, absn(fv));
     mv = fmax(mv, absn(vv));
@@ -272,12 +276,67 @@ __global__ void k_bfill(double* __restrict__ dst, int64_t n, double v) {
     dst[i] = v;
 }
 
+// ---- mixed form: FP64 defect correction around bf16x3-transform inner solves
+// Per segment, its cavity's outer state from the maxima of the cavity's columns (chk:
+// four sets of nseg values, max|defect|, max|f|, max|u|, max|D u|): finished when
+// max|defect| / (1 + max|f|) <= tol and max|D u| <= tol, else active.
+__global__ void k_bouter(const double* __restrict__ chk, int nseg, int ncols, double tol, int* __restrict__ active) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const int first = (s / ncols) * ncols;
+  double d = 0.0, f = 0.0, dv = 0.0;
+  for (int c = first; c < first + ncols; ++c) {
+    d = fmax(d, chk[c]);
+    f = fmax(f, chk[nseg + c]);
+    dv = fmax(dv, chk[3 * nseg + c]);
+  }
+  active[s] = (d / (1.0 + f) <= tol && dv <= tol) ? 0 : 1;
+}
+
+// Segment CG states against the outer flags.  GATE_SET: the state's active flag becomes
+// the cavity's (an inner solve's first launches skip finished cavities); GATE_MASK: after
+// the CG set-up, finished cavities stay inactive; GATE_POST: after the inner CG, its
+// iterations are added to the segment's total and the flag returns to the cavity's.
+enum Gate : int { GATE_SET = 0, GATE_MASK = 1, GATE_POST = 2 };
+__global__ void k_bgate(CgState* st, const int* __restrict__ active, int nseg, int mode, int* iters_total) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  if (mode == GATE_MASK) {
+    if (!active[s]) {
+      st[s].active = 0;
+      st[s].converged = 1;
+    }
+    return;
+  }
+  if (mode == GATE_POST) iters_total[s] += st[s].iters;
+  st[s].active = active[s];
+}
+
+// u += du, p += dp on the segments whose state is active.
+__global__ __launch_bounds__(kThreads) void k_baccum(double* __restrict__ u, const double* __restrict__ du,
+                                                     double* __restrict__ p, const double* __restrict__ dp,
+                                                     const Seg* segs, const CgState* st) {
+  const int s = blockIdx.y;
+  if (!st[s].active) return;
+  const Seg S = segs[s];
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.size; i += gridDim.x * kThreads)
+    u[S.voff + i] += du[S.voff + i];
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.np; i += gridDim.x * kThreads)
+    p[S.poff + i] += dp[S.poff + i];
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096); }
 
 struct DevMatrix64 {
   int n = 0, kind = -1;
   double* q = nullptr;
   double* qt = nullptr;
+};
+
+// bf16 hi / lo parts of Q and Q^T (n x ld8, zero padded) for the mixed form.
+struct DevMatrixB {
+  int n = 0, kind = -1, ld8 = 0;
+  __bf16 *qb = nullptr, *qb_lo = nullptr, *qtb = nullptr, *qtb_lo = nullptr;
 };
 
 struct Cavity {
@@ -384,6 +443,8 @@ class BatchSolver {
     flags_ = (int*)take(sizeof(int) * nseg_);
     HIP_CHECK(hipHostMalloc((void**)&host_flags_, sizeof(int) * nseg_, hipHostMallocDefault));
     build_tables();
+    mixed_ = o.precision == MAC_PRECISION_MIXED;
+    if (mixed_) setup_mixed();
   }
 
   ~BatchSolver() {
@@ -410,11 +471,15 @@ class BatchSolver {
   std::string error;
   int use_graph = 1;        // replay one captured CG iteration instead of launching it
   int iters_per_check = 1;  // CG iterations queued between host reads of the flags
+  int last_sweeps = 0;      // outer sweeps of the last solve (1 for the FP64 form)
+
+  bool mixed() const { return mixed_; }
 
   struct Outcome {
     std::vector<int> iterations;              // per segment
     std::vector<double> residual, divergence, vmax;  // per cavity
     std::vector<int> failed_cg;               // cavities whose CG did not converge
+    int sweeps = 1;                           // mixed form: outer sweeps of the batch
   };
 
   // f, vel, pres: device buffers in the batch layout.  mass, visc: per cavity.
@@ -447,26 +512,7 @@ class BatchSolver {
     k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(x_, P_, 0.0);
     copy(b_, r_, P_);
     copy(sums2_, rr_, nseg_);
-    // Schur PCG, every segment at once; converged segments drop out on the device.  As in
-    // scipy's loop, the convergence test runs only at the top of an iteration, so none
-    // follows the last of max_iterations.
-    int done_iters = 0;
-    while (done_iters < opts_.max_iterations) {
-      cg_check(cg_, rr_, flags_, nseg_, stream_);
-      HIP_CHECK(hipMemcpyAsync(host_flags_, flags_, sizeof(int) * nseg_, hipMemcpyDeviceToHost, stream_));
-      HIP_CHECK(hipStreamSynchronize(stream_));
-      int active = 0;
-      for (int s = 0; s < nseg_; ++s) active += host_flags_[s] != 0;
-      if (active == 0) break;
-      // Queue up to iters_per_check iterations (never past max_iterations); the state
-      // test at the top of each queued iteration runs on the device.
-      const int burst = std::min(iters_per_check, opts_.max_iterations - done_iters);
-      for (int k = 0; k < burst; ++k) {
-        if (k > 0) cg_check(cg_, rr_, flags_, nseg_, stream_);
-        iteration();
-      }
-      done_iters += burst;
-    }
+    run_cg();
     std::vector<CgState> st(nseg_);
     HIP_CHECK(hipMemcpyAsync(st.data(), cg_, sizeof(CgState) * nseg_, hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -489,7 +535,8 @@ class BatchSolver {
     copy(vout_, vel, V_);
     copy(x_, pres, P_);
     // Checks: residual and divergence per cavity (max over its columns).
-    k_bdefect<<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(vel, f, pres, segs_, mv_, partials_, pstride_);
+    k_bdefect<false><<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(vel, f, pres, segs_, mv_, partials_, pstride_,
+                                                                   nullptr);
     for (int q = 0; q < 3; ++q)
       k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_ + (int64_t)q * nseg_ * pstride_, pstride_,
                                                               segs_, LEN_FACES, chk_ + q * nseg_);
@@ -513,7 +560,138 @@ class BatchSolver {
     return out;
   }
 
+  // Mixed form, per cavity as the single-cavity mixed solve: from u = 0, p = 0, sweeps of
+  //   FP64 defect r_u = f - (mass u + visc K u + G p), r_c = -D u, and their maxima;
+  //   stop when max|r_u| / (1 + max|f|) <= refine_tolerance and max|D u| <= refine_tolerance;
+  //   inner solve with bf16x3 transforms: W = L~^-1 r_u, b = r_c - D W (mean removed),
+  //   Schur PCG to inner_tolerance, dp = x - mean(x), du = L~^-1 (r_u + D^T dp);
+  //   u += du, p += dp.
+  // Finished cavities drop out on the device (their segments' states stay inactive); the
+  // host reads the outer flags and maxima once per sweep.  The CG vectors, dot products,
+  // defects and updates are FP64; only the transforms are split bf16 with FP32
+  // accumulation.  Returns per-cavity residual and max|D u| of the last defect, and per
+  // segment the inner CG iterations summed over sweeps.
+  Outcome solve_mixed(const double* f, const double* mass, const double* visc, double* vel, double* pres) {
+    const int nc = count();
+    set_shifts(mass, visc);
+    k_bfill<<<blocks_for(V_), kThreads, 0, stream_>>>(uacc_, V_, 0.0);
+    k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(pacc_, P_, 0.0);
+    HIP_CHECK(hipMemsetAsync(iters_total_, 0, sizeof(int) * nseg_, stream_));
+    const unsigned gb = (unsigned)((nseg_ + 63) / 64);
+    std::vector<double> h(4 * (size_t)nseg_);
+    std::vector<int> act(nseg_);
+    Outcome out;
+    int sweep = 0;
+    for (;; ++sweep) {
+      k_bdefect<true><<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(uacc_, f, pacc_, segs_, mv_, partials_, pstride_,
+                                                                     ru_);
+      for (int q = 0; q < 3; ++q)
+        k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_ + (int64_t)q * nseg_ * pstride_, pstride_,
+                                                                segs_, LEN_FACES, chk_ + q * nseg_);
+      bdiv(uacc_, rc_, -1.0, RED_MAXABS, nullptr);
+      bfinalize(RED_MAXABS, LEN_CELLS, chk_ + 3 * nseg_);
+      k_bouter<<<gb, 64, 0, stream_>>>(chk_, nseg_, ncols_, opts_.refine_tolerance, outer_);
+      HIP_CHECK(hipMemcpyAsync(h.data(), chk_, sizeof(double) * 4 * nseg_, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipMemcpyAsync(act.data(), outer_, sizeof(int) * nseg_, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      // A non-finite right-hand side is an invalid argument (tensor_stokes.py:246-257).
+      if (sweep == 0)
+        for (int q = 0; q < nseg_; ++q)
+          if (!std::isfinite(h[nseg_ + q])) throw ArgFailure("Invalid Stokes RHS");
+      int active = 0;
+      bool changed = false;
+      for (int q = 0; q < nseg_; ++q) {
+        // A cavity whose defect is no longer finite stops here; the gate reports it.
+        const bool finite = std::isfinite(h[q]) && std::isfinite(h[3 * nseg_ + q]);
+        if (act[q] && !finite) {
+          act[q] = 0;
+          changed = true;
+        }
+        active += act[q] != 0;
+      }
+      if (active == 0 || sweep >= opts_.max_sweeps) break;
+      if (changed) HIP_CHECK(hipMemcpy(outer_, act.data(), sizeof(int) * nseg_, hipMemcpyHostToDevice));
+      // Inner solve for (du, dp) on the unfinished cavities.
+      k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_SET, nullptr);
+      copy(ru_, vin_, V_);
+      velocity_inverse(true);
+      bdiv(vout_, b_, -1.0, RED_SUM, cg_, rc_);
+      bfinalize(RED_SUM, LEN_CELLS, sums_);
+      k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, sums_, b_, segs_, partials_, pstride_,
+                                                                           cg_);
+      bfinalize(RED_SUM, LEN_CELLS, sums2_);
+      k_breduce<RED_MAXABS, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, nullptr, segs_, LEN_CELLS,
+                                                                                  partials_, pstride_, cg_);
+      bfinalize(RED_MAXABS, LEN_CELLS, sums3_);
+      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, nseg_, stream_);
+      k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_MASK, nullptr);
+      k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(x_, P_, 0.0);
+      copy(b_, r_, P_);
+      copy(sums2_, rr_, nseg_);
+      run_cg();
+      k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_POST, iters_total_);
+      k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, nullptr, segs_, LEN_CELLS,
+                                                                               partials_, pstride_, cg_);
+      bfinalize(RED_SUM, LEN_CELLS, sums_);
+      k_bsub_mean_dot<false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, sums_, nullptr, segs_, nullptr,
+                                                                            pstride_, cg_);
+      k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(x_, ru_, vin_, segs_, cg_);
+      velocity_inverse(true);
+      k_baccum<<<dim3((unsigned)std::min<int64_t>((maxfaces_ + kThreads - 1) / kThreads, 64), (unsigned)nseg_),
+                 kThreads, 0, stream_>>>(uacc_, vout_, pacc_, x_, segs_, cg_);
+    }
+    out.sweeps = sweep;
+    // p mean-zero; hand out u and p.
+    k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(pacc_, nullptr, segs_, LEN_CELLS,
+                                                                             partials_, pstride_, nullptr);
+    bfinalize(RED_SUM, LEN_CELLS, sums_);
+    k_bsub_mean_dot<false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(pacc_, sums_, nullptr, segs_, nullptr,
+                                                                          pstride_, nullptr);
+    copy(uacc_, vel, V_);
+    copy(pacc_, pres, P_);
+    out.iterations.resize(nseg_);
+    HIP_CHECK(hipMemcpyAsync(out.iterations.data(), iters_total_, sizeof(int) * nseg_, hipMemcpyDeviceToHost,
+                             stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    out.residual.assign(nc, 0.0);
+    out.divergence.assign(nc, 0.0);
+    out.vmax.assign(nc, 0.0);
+    std::vector<double> dmax(nc, 0.0), fmax_(nc, 0.0);
+    for (int q = 0; q < nseg_; ++q) {
+      const int c = segs_host_[q].cav;
+      dmax[c] = std::max(dmax[c], h[q]);
+      fmax_[c] = std::max(fmax_[c], h[nseg_ + q]);
+      out.vmax[c] = std::max(out.vmax[c], h[2 * nseg_ + q]);
+      out.divergence[c] = std::max(out.divergence[c], h[3 * nseg_ + q]);
+    }
+    for (int c = 0; c < nc; ++c) out.residual[c] = dmax[c] / (1.0 + fmax_[c]);
+    return out;
+  }
+
  private:
+  // Schur PCG, every segment at once; converged segments drop out on the device.  As in
+  // scipy's loop, the convergence test runs only at the top of an iteration, so none
+  // follows the last of max_iterations.
+  void run_cg() {
+    int done_iters = 0;
+    while (done_iters < opts_.max_iterations) {
+      cg_check(cg_, rr_, flags_, nseg_, stream_);
+      HIP_CHECK(hipMemcpyAsync(host_flags_, flags_, sizeof(int) * nseg_, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      int active = 0;
+      for (int q = 0; q < nseg_; ++q) active += host_flags_[q] != 0;
+      if (active == 0) break;
+      // Queue up to iters_per_check iterations (never past max_iterations); the state
+      // test at the top of each queued iteration runs on the device.
+      const int burst = std::min(iters_per_check, opts_.max_iterations - done_iters);
+      for (int k = 0; k < burst; ++k) {
+        if (k > 0) cg_check(cg_, rr_, flags_, nseg_, stream_);
+        iteration();
+      }
+      done_iters += burst;
+    }
+  }
+
   // ------------------------------------------------------------ one CG iteration
   void iteration() {
     if (use_graph) {
@@ -554,17 +732,29 @@ class BatchSolver {
   }
 
   // ------------------------------------------------------------ transforms
-  // Velocity: vout = L^-1 vin for every segment (per-cavity shifts in the tables).
+  // Velocity: vout = L^-1 vin for every segment (per-cavity shifts in the tables); the
+  // mixed form's tables run the split bf16 GEMM.
   void velocity_inverse(bool use_state) {
     for (int k = 0; k < 4; ++k) {
-      GemmTable t = vel_tab_[k];
+      GemmTable t = mixed_ ? velb_tab_[k] : vel_tab_[k];
       t.use_state = use_state ? 1 : 0;
-      gemm_f64_table(t, stream_);
+      if (mixed_) run_table(t, kPassTypes[k][0], kPassTypes[k][1], k == 0 ? velb_avec_ : true);
+      else gemm_f64_table(t, stream_);
     }
   }
   // Pressure: z = mass (D D^T)^+ r + visc r.
   void pressure_precondition() {
-    for (int k = 0; k < 4; ++k) gemm_f64_table(pre_tab_[k], stream_);
+    for (int k = 0; k < 4; ++k) {
+      if (mixed_) run_table(preb_tab_[k], kPassTypes[k][0], kPassTypes[k][1], k == 0 ? preb_avec_ : true);
+      else gemm_f64_table(pre_tab_[k], stream_);
+    }
+  }
+  // A-operand / output types of the four passes in the mixed form: the FP64 input, two
+  // FP32 intermediates, the FP64 output.
+  static constexpr int kPassTypes[4][2] = {{DT_F64, DT_F32}, {DT_F32, DT_F32}, {DT_F32, DT_F32}, {DT_F32, DT_F64}};
+  void run_table(const GemmTable& t, int a_type, int c_type, bool a_vec) {
+    if (gemm_bf16_table(t, a_type, c_type, a_vec, stream_) != 0)
+      throw HipFailure("unsupported split-bf16 table pass");
   }
 
   static void pass(std::vector<GemmEntry>& v, const double* A, int lda, const DevMatrix64* Q, double* C, int ldc,
@@ -630,6 +820,138 @@ class BatchSolver {
     }
   }
 
+  // ------------------------------------------------------------ mixed form set-up
+  // FP32 transform intermediates: per segment one region for each block (u, v, p), large
+  // enough for both transposed layouts with rows padded to 8 floats, 16-byte aligned; the
+  // split-bf16 tables over them; the outer iterate and residual buffers.
+  void setup_mixed() {
+    int64_t t = 0;
+    toff_.assign(3 * (size_t)nseg_, 0);
+    for (int q = 0; q < nseg_; ++q) {
+      const Seg& S = segs_host_[q];
+      const int dims[3][2] = {{S.ny, S.nx - 1}, {S.ny - 1, S.nx}, {S.ny, S.nx}};
+      for (int b = 0; b < 3; ++b) {
+        const int by = dims[b][0], bx = dims[b][1];
+        const int64_t need = std::max<int64_t>((int64_t)bx * round8(by), (int64_t)by * round8(bx));
+        toff_[3 * q + b] = t;
+        t += (need + 3) & ~int64_t(3);
+      }
+    }
+    tf1_ = static_cast<float*>(dev_alloc(sizeof(float) * t));
+    tf2_ = static_cast<float*>(dev_alloc(sizeof(float) * t));
+    uacc_ = static_cast<double*>(dev_alloc(sizeof(double) * V_));
+    pacc_ = static_cast<double*>(dev_alloc(sizeof(double) * P_));
+    ru_ = static_cast<double*>(dev_alloc(sizeof(double) * V_));
+    rc_ = static_cast<double*>(dev_alloc(sizeof(double) * P_));
+    outer_ = static_cast<int*>(dev_alloc(sizeof(int) * nseg_));
+    iters_total_ = static_cast<int*>(dev_alloc(sizeof(int) * nseg_));
+    build_tables_bf16();
+  }
+
+  static void passb(std::vector<GemmEntry>& v, const void* A, int lda, const DevMatrixB* Q, bool transposed, void* C,
+                    int ldc, int M, int N, int K, const double* ea, const double* eb, const CgState* st) {
+    GemmEntry e{};
+    e.A = A;
+    e.Bt = transposed ? Q->qtb : Q->qb;
+    e.Bt_lo = transposed ? Q->qtb_lo : Q->qb_lo;
+    e.C = C;
+    e.ea = ea;
+    e.eb = eb;
+    e.M = M;
+    e.N = N;
+    e.K = K;
+    e.lda = lda;
+    e.ldb = Q->ld8;
+    e.ldc = ldc;
+    e.st = st;
+    v.push_back(e);
+  }
+
+  // The same four passes per block as build_tables, FP32 intermediates in between.
+  void build_tables_bf16() {
+    std::vector<GemmEntry> v[4], pr[4];
+    for (int q = 0; q < nseg_; ++q) {
+      const Seg& S = segs_host_[q];
+      const Cavity& C = cav_[S.cav];
+      const CgState* st = cg_ + q;
+      struct Blk { int ny, nx; const DevMatrixB *my, *mx; double *ly, *lx; int64_t off; int reg; };
+      const Blk blocks[2] = {
+          {C.ny, C.nx - 1, matrix_b(C.ny, KIND_DST2), matrix_b(C.nx - 1, KIND_DST1), C.luy, C.lux, 0, 0},
+          {C.ny - 1, C.nx, matrix_b(C.ny - 1, KIND_DST1), matrix_b(C.nx, KIND_DST2), C.lvy, C.lvx, (int64_t)S.nu, 1}};
+      for (const Blk& B : blocks) {
+        const int64_t o = S.voff + B.off;
+        float* t1 = tf1_ + toff_[3 * q + B.reg];
+        float* t2 = tf2_ + toff_[3 * q + B.reg];
+        const int ry = round8(B.ny), rx = round8(B.nx);
+        passb(v[0], vin_ + o, B.nx, B.mx, false, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr, st);
+        passb(v[1], t1, ry, B.my, false, t2, ry, B.nx, B.ny, B.ny, B.lx, B.ly, st);
+        passb(v[2], t2, ry, B.my, true, t1, rx, B.nx, B.ny, B.ny, nullptr, nullptr, st);
+        passb(v[3], t1, rx, B.mx, true, vout_ + o, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr, st);
+      }
+      const DevMatrixB* px = matrix_b(C.nx, KIND_DCT2);
+      const DevMatrixB* py = matrix_b(C.ny, KIND_DCT2);
+      float* t1 = tf1_ + toff_[3 * q + 2];
+      float* t2 = tf2_ + toff_[3 * q + 2];
+      const int ry = round8(C.ny), rx = round8(C.nx);
+      passb(pr[0], r_ + S.poff, C.nx, px, false, t1, ry, C.ny, C.nx, C.nx, nullptr, nullptr, st);
+      passb(pr[1], t1, ry, py, false, t2, ry, C.nx, C.ny, C.ny, C.lpx, C.lpy, st);
+      pr[1].back().thresh = C.p_thresh;
+      passb(pr[2], t2, ry, py, true, t1, rx, C.nx, C.ny, C.ny, nullptr, nullptr, st);
+      passb(pr[3], t1, rx, px, true, z_ + S.poff, C.nx, C.ny, C.nx, C.nx, nullptr, nullptr, st);
+      pr[3].back().aux = r_ + S.poff;
+    }
+    // Vector loads of the FP64 first pass need even rows and 16-byte aligned bases; the
+    // FP32 intermediates have them by construction.
+    auto vec64 = [](const std::vector<GemmEntry>& t) {
+      for (const GemmEntry& e : t)
+        if (e.lda % 2 != 0 || reinterpret_cast<uintptr_t>(e.A) % 16 != 0) return false;
+      return true;
+    };
+    velb_avec_ = vec64(v[0]);
+    preb_avec_ = vec64(pr[0]);
+    for (int k = 1; k < 4; ++k)
+      for (const std::vector<GemmEntry>* t : {&v[k], &pr[k]})
+        for (const GemmEntry& e : *t)
+          if (e.lda % 4 != 0 || reinterpret_cast<uintptr_t>(e.A) % 16 != 0)
+            throw HipFailure("misaligned FP32 transform intermediate");
+    const int vepi[4] = {EPI_STORE, EPI_SHIFT, EPI_STORE, EPI_STORE};
+    const int pepi[4] = {EPI_STORE, EPI_LAMBDA, EPI_STORE, EPI_COMBINE};
+    const int trans[4] = {1, 0, 1, 0};
+    for (int k = 0; k < 4; ++k) {
+      velb_host_[k] = v[k];
+      preb_host_[k] = pr[k];
+      velb_tab_[k] = make_table(velb_host_[k], velb_dev_[k], velb_beg_[k], vepi[k], trans[k]);
+      preb_tab_[k] = make_table(preb_host_[k], preb_dev_[k], preb_beg_[k], pepi[k], trans[k]);
+    }
+  }
+
+  const DevMatrixB* matrix_b(int n, int kind) {
+    for (auto& m : matsb_)
+      if (m->n == n && m->kind == kind) return m.get();
+    auto m = std::make_unique<DevMatrixB>();
+    m->n = n;
+    m->kind = kind;
+    m->ld8 = round8(n);
+    const std::vector<double> q = transform_matrix(n, kind);
+    const size_t len = (size_t)n * m->ld8;
+    std::vector<__bf16> qb(len, (__bf16)0.0f), qbl(len, (__bf16)0.0f), qtb(len, (__bf16)0.0f), qtbl(len, (__bf16)0.0f);
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        const double v = q[(size_t)a * n + b], vt = q[(size_t)b * n + a];
+        const __bf16 hv = (__bf16)(float)v, ht = (__bf16)(float)vt;
+        qb[(size_t)a * m->ld8 + b] = hv;
+        qtb[(size_t)a * m->ld8 + b] = ht;
+        qbl[(size_t)a * m->ld8 + b] = (__bf16)(float)(v - (double)(float)hv);
+        qtbl[(size_t)a * m->ld8 + b] = (__bf16)(float)(vt - (double)(float)ht);
+      }
+    m->qb = static_cast<__bf16*>(upload_raw(qb.data(), len * sizeof(__bf16)));
+    m->qb_lo = static_cast<__bf16*>(upload_raw(qbl.data(), len * sizeof(__bf16)));
+    m->qtb = static_cast<__bf16*>(upload_raw(qtb.data(), len * sizeof(__bf16)));
+    m->qtb_lo = static_cast<__bf16*>(upload_raw(qtbl.data(), len * sizeof(__bf16)));
+    matsb_.push_back(std::move(m));
+    return matsb_.back().get();
+  }
+
   GemmTable make_table(std::vector<GemmEntry>& host, GemmEntry*& dev, int*& beg, int epi, int trans) {
     std::vector<int> begins(host.size());
     GemmTable t{};
@@ -669,6 +991,21 @@ class BatchSolver {
                              hipMemcpyHostToDevice, stream_));
     HIP_CHECK(hipMemcpyAsync(pre_dev_[3], pre_host_[3].data(), sizeof(GemmEntry) * pre_host_[3].size(),
                              hipMemcpyHostToDevice, stream_));
+    if (mixed_) {
+      for (size_t i = 0; i < velb_host_[1].size(); ++i) {
+        const int q = (int)(i / 2);
+        velb_host_[1][i].s0 = mv[2 * q];
+        velb_host_[1][i].s1 = mv[2 * q + 1];
+      }
+      for (size_t q = 0; q < preb_host_[3].size(); ++q) {
+        preb_host_[3][q].s0 = mv[2 * q];
+        preb_host_[3][q].s1 = mv[2 * q + 1];
+      }
+      HIP_CHECK(hipMemcpyAsync(velb_dev_[1], velb_host_[1].data(), sizeof(GemmEntry) * velb_host_[1].size(),
+                               hipMemcpyHostToDevice, stream_));
+      HIP_CHECK(hipMemcpyAsync(preb_dev_[3], preb_host_[3].data(), sizeof(GemmEntry) * preb_host_[3].size(),
+                               hipMemcpyHostToDevice, stream_));
+    }
     HIP_CHECK(hipMemcpyAsync(mv_, mv.data(), sizeof(double) * mv.size(), hipMemcpyHostToDevice, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
@@ -676,10 +1013,12 @@ class BatchSolver {
   // ------------------------------------------------------------ helpers
   dim3 seg_grid(int len) const { return dim3((unsigned)(len == LEN_FACES ? nbv_max_ : nbp_max_), (unsigned)nseg_); }
   dim3 dt_grid() const { return dim3((unsigned)((maxfaces_ + kThreads - 1) / kThreads), (unsigned)nseg_); }
-  void bdiv(const double* vel, double* out, double sign, int mode, const CgState* st) {
+  void bdiv(const double* vel, double* out, double sign, int mode, const CgState* st, const double* add = nullptr) {
     const dim3 g = seg_grid(LEN_CELLS);
-    if (mode == RED_SUM) k_bdiv<RED_SUM><<<g, kThreads, 0, stream_>>>(vel, out, segs_, sign, partials_, pstride_, st);
-    else k_bdiv<RED_MAXABS><<<g, kThreads, 0, stream_>>>(vel, out, segs_, sign, partials_, pstride_, st);
+    if (mode == RED_SUM)
+      k_bdiv<RED_SUM><<<g, kThreads, 0, stream_>>>(vel, out, segs_, sign, partials_, pstride_, st, add);
+    else
+      k_bdiv<RED_MAXABS><<<g, kThreads, 0, stream_>>>(vel, out, segs_, sign, partials_, pstride_, st, add);
   }
   void bfinalize(int mode, int len, double* out) {
     if (mode == RED_SUM) k_bfinalize<RED_SUM><<<nseg_, kThreads, 0, stream_>>>(partials_, pstride_, segs_, len, out);
@@ -741,6 +1080,18 @@ class BatchSolver {
   int *vel_beg_[4] = {}, *pre_beg_[4] = {};
   GemmTable vel_tab_[4] = {}, pre_tab_[4] = {};
   std::vector<double> shifts_;
+  // mixed form
+  bool mixed_ = false;
+  std::vector<std::unique_ptr<DevMatrixB>> matsb_;
+  std::vector<int64_t> toff_;
+  float *tf1_ = nullptr, *tf2_ = nullptr;
+  double *uacc_ = nullptr, *pacc_ = nullptr, *ru_ = nullptr, *rc_ = nullptr;
+  int *outer_ = nullptr, *iters_total_ = nullptr;
+  std::vector<GemmEntry> velb_host_[4], preb_host_[4];
+  GemmEntry *velb_dev_[4] = {}, *preb_dev_[4] = {};
+  int *velb_beg_[4] = {}, *preb_beg_[4] = {};
+  GemmTable velb_tab_[4] = {}, preb_tab_[4] = {};
+  bool velb_avec_ = false, preb_avec_ = false;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
 };
@@ -791,8 +1142,12 @@ void* mac_batch_create(int32_t count, const int32_t* nx, const int32_t* ny, cons
                                    std::to_string(i) + ")";
       return nullptr;
     }
+  const bool mixed_ok = o.precision == MAC_PRECISION_MIXED && o.refine_tolerance > 0 &&
+                        std::isfinite(o.refine_tolerance) && o.inner_tolerance > 0 && o.inner_tolerance < 1 &&
+                        o.max_sweeps >= 1;
   if (!(o.tolerance > 0) || !(o.krylov_tolerance > 0) || !std::isfinite(o.tolerance) ||
-      !std::isfinite(o.krylov_tolerance) || o.max_iterations < 1 || o.precision != MAC_PRECISION_FP64) {
+      !std::isfinite(o.krylov_tolerance) || o.max_iterations < 1 ||
+      (o.precision != MAC_PRECISION_FP64 && !mixed_ok)) {
     macs::g_batch_create_error = "Invalid solver controls";
     return nullptr;
   }
@@ -824,6 +1179,11 @@ int32_t mac_batch_layout(void* h, int64_t* velocity_offsets, int64_t* pressure_o
   velocity_offsets[n] = s->total_faces();
   pressure_offsets[n] = s->total_cells();
   return MAC_OK;
+}
+
+int32_t mac_batch_last_sweeps(void* h) {
+  BatchSolver* s = static_cast<BatchSolver*>(h);
+  return s ? s->last_sweeps : -1;
 }
 
 int32_t mac_batch_set_launch(void* h, int32_t use_graph, int32_t iters_per_check) {
@@ -860,10 +1220,14 @@ int32_t mac_stokes_solve_batched(void* h, const double* rhs, const double* mass,
       vel = s->io_v();
       pr = s->io_p();
     }
-    // Non-finite right-hand sides are rejected like the reference (ValueError).
-    for (double m : s->rhs_max(f))
-      if (!std::isfinite(m)) throw macs::ArgFailure("Invalid Stokes RHS");
-    BatchSolver::Outcome out = s->solve(f, mass, viscosity, vel, pr);
+    // Non-finite right-hand sides are rejected like the reference (ValueError); the mixed
+    // form finds them in its first defect.
+    if (!s->mixed())
+      for (double m : s->rhs_max(f))
+        if (!std::isfinite(m)) throw macs::ArgFailure("Invalid Stokes RHS");
+    BatchSolver::Outcome out = s->mixed() ? s->solve_mixed(f, mass, viscosity, vel, pr)
+                                          : s->solve(f, mass, viscosity, vel, pr);
+    s->last_sweeps = out.sweeps;
     if (iterations)
       for (size_t i = 0; i < out.iterations.size(); ++i) iterations[i] = out.iterations[i];
     if (!out.failed_cg.empty())

@@ -237,8 +237,9 @@ __device__ inline void store4<double>(double* p, const f4& v) {
 // AMODE: how A reaches LDS.  0 = bf16 rows (16-byte loads of 8), 1 = fp32 rows with
 // lda % 4 == 0 and 16-byte aligned bases (float4 loads), 2 = one element per load,
 // 3 = fp64 rows with even lda and 16-byte aligned bases (two double2 loads per 4).
+// One BM x BN output tile (index lt of entry E, tile-rows visited in groups of GROUP).
 template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
-__global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
+__device__ __forceinline__ void bf16_tile(const GemmEntry& E, const int lt, const int GROUP, const int trans_c) {
   using C = Cfg<BM, BN, SPLIT, NW>;
   constexpr int BK = C::BK, LDS = C::LDS, FM = C::FM, FN = C::FN, AE = C::AE, BE = C::BE, NT = C::NT;
   static_assert(AE >= (AMODE == 0 ? 8 : (AMODE == 1 || AMODE == 3) ? 4 : 1) && BE >= 8,
@@ -250,10 +251,6 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
   __shared__ __attribute__((aligned(16))) __bf16 As[2][C::PARTS * BM * LDS];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[2][C::PARTS * BN * LDS];
 
-  const int t = xcd_tile(blockIdx.x, batch.total_tiles);
-  const GemmEntry& E = batch.e[find_entry(batch, t)];
-  const int lt = t - E.tile_begin;
-  const int GROUP = batch.group;
   const int per_group = GROUP * E.tiles_n;
   const int first_m = (lt / per_group) * GROUP;
   const int gsize = min(E.tiles_m - first_m, GROUP);
@@ -458,7 +455,7 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
   if (kt < nk) compute(0);
 
   // f32-result MFMA layout: register r of lane l holds row 4*(l/16) + r, column l%16.
-  const bool vec_t = EPI == EPI_STORE && batch.trans_c && (E.ldc & 3) == 0;
+  const bool vec_t = EPI == EPI_STORE && trans_c && (E.ldc & 3) == 0;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -469,9 +466,34 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
         store4<TC>(static_cast<TC*>(E.C) + (int64_t)n * E.ldc + mb, acc[i][j]);
       } else {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) epilogue<EPI, TC>(E, batch.trans_c, mb + r, n, (double)acc[i][j][r]);
+        for (int r = 0; r < 4; ++r) epilogue<EPI, TC>(E, trans_c, mb + r, n, (double)acc[i][j][r]);
       }
     }
+}
+
+
+template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
+__global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
+  const int t = xcd_tile(blockIdx.x, batch.total_tiles);
+  const GemmEntry& E = batch.e[find_entry(batch, t)];
+  bf16_tile<EPI, TA, TC, SPLIT, BM, BN, AMODE, NW>(E, t - E.tile_begin, batch.group, batch.trans_c);
+}
+
+// Grouped split-bf16 launch over a device table (64x64 tiles, 4 waves): each block finds
+// its entry by binary search over the ascending tile_begin array; entries whose state is
+// inactive (a converged segment) return at once.
+template <int EPI, typename TA, typename TC, int AMODE>
+__global__ __launch_bounds__(256) void k_gemm_bf16_table(const GemmTable tb) {
+  const int t = xcd_tile(blockIdx.x, tb.total_tiles);
+  int lo = 0, hi = tb.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tb.begin[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmEntry E = tb.e[lo];
+  if (tb.use_state && E.st != nullptr && !E.st->active) return;
+  bf16_tile<EPI, TA, TC, true, 64, 64, AMODE, 4>(E, t - E.tile_begin, 1, tb.trans_c);
 }
 
 int tile_batch(GemmBatch& b, int bm, int bn) {
@@ -625,6 +647,39 @@ void gemm_f64_table(const GemmTable& t, hipStream_t s) {
     case EPI_LAMBDA: k_gemm_f64_table<EPI_LAMBDA><<<t.total_tiles, 256, 0, s>>>(t); break;
     default: k_gemm_f64_table<EPI_COMBINE><<<t.total_tiles, 256, 0, s>>>(t); break;
   }
+}
+
+namespace {
+template <int EPI, typename TA, typename TC, int AMODE>
+void launch_table(const GemmTable& t, hipStream_t s) {
+  k_gemm_bf16_table<EPI, TA, TC, AMODE><<<t.total_tiles, 256, 0, s>>>(t);
+}
+}  // namespace
+
+int gemm_bf16_table(const GemmTable& t, int a_type, int c_type, bool a_vec, hipStream_t s) {
+  if (t.count <= 0 || t.total_tiles <= 0) return 0;
+  if (a_type == DT_F64 && c_type == DT_F32 && t.epi == EPI_STORE) {
+    if (a_vec) launch_table<EPI_STORE, double, float, 3>(t, s);
+    else launch_table<EPI_STORE, double, float, 2>(t, s);
+    return 0;
+  }
+  if (!a_vec || a_type != DT_F32) return -1;
+  if (c_type == DT_F32) {
+    switch (t.epi) {
+      case EPI_STORE: launch_table<EPI_STORE, float, float, 1>(t, s); return 0;
+      case EPI_SHIFT: launch_table<EPI_SHIFT, float, float, 1>(t, s); return 0;
+      case EPI_LAMBDA: launch_table<EPI_LAMBDA, float, float, 1>(t, s); return 0;
+      default: return -1;
+    }
+  }
+  if (c_type == DT_F64) {
+    switch (t.epi) {
+      case EPI_STORE: launch_table<EPI_STORE, float, double, 1>(t, s); return 0;
+      case EPI_COMBINE: launch_table<EPI_COMBINE, float, double, 1>(t, s); return 0;
+      default: return -1;
+    }
+  }
+  return -1;
 }
 
 void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s) {

@@ -67,6 +67,12 @@ int tile_table_f64(GemmEntry* entries, int count, int* begins);
 // a_type: DT_F64 / DT_F32 / DT_BF16 for A and aux/C per c_type; Bt is always bf16.
 // split: three-product hi/lo form (A must be DT_F32 or DT_F64, Bt_lo set).
 void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s);
+// Split-bf16 (bf16x3, FP32 accumulation) grouped launch over a device table with 64x64
+// tiles, the tiling tile_table_f64 gives.  Supported: FP64 -> FP32 store (a_vec: even
+// lda and 16-byte aligned rows, else element loads), FP32 -> FP32 store / shift / lambda
+// and FP32 -> FP64 store / combine (a_vec required: lda % 4 == 0 and 16-byte aligned
+// rows).  Every entry carries Bt_lo.  Returns 0, or -1 for an unsupported combination.
+int gemm_bf16_table(const GemmTable& t, int a_type, int c_type, bool a_vec, hipStream_t s);
 // Events that the next gemm_f64 / gemm_bf16 launch records at its kernel's own start and
 // end (hipExtLaunchKernelGGL), consumed by that launch; null for plain launches.
 void set_gemm_events(hipEvent_t start, hipEvent_t stop);

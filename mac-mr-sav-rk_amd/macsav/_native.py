@@ -76,6 +76,7 @@ EXPORTED = (
     "mac_batch_last_error",
     "mac_batch_layout",
     "mac_batch_set_launch",
+    "mac_batch_last_sweeps",
     "mac_stokes_solve_batched",
 )
 
@@ -232,6 +233,8 @@ def load() -> ctypes.CDLL:
     lib.mac_batch_layout.restype = _I32
     lib.mac_batch_set_launch.argtypes = [_P, _I32, _I32]
     lib.mac_batch_set_launch.restype = _I32
+    lib.mac_batch_last_sweeps.argtypes = [_P]
+    lib.mac_batch_last_sweeps.restype = _I32
     lib.mac_stokes_solve_batched.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I32]
     lib.mac_stokes_solve_batched.restype = _I32
     _lib = lib

@@ -4,8 +4,13 @@ The reference advances independent cavities by calling `TensorStokes.solve`
 (solver/mac/tensor_stokes.py:245-364) once per cavity.  Here every cavity of a batch
 (its own grid, lengths, (mass, viscosity) and right-hand-side columns) is solved by one
 library call, `mac_stokes_solve_batched`, whose transform passes and vector steps each
-cover every cavity in one grouped launch.  Each cavity's results, iteration counts and
-errors are those of the cavity solved alone by `GpuTensorStokes`.
+cover every cavity in one grouped launch.
+
+`precision="fp64"` (default) gives every cavity the results of `GpuTensorStokes` on that
+cavity alone, bitwise.  `precision="mixed"` runs the mixed form per cavity: FP64 defect
+correction to `refine_tolerance` on the residual and max|D u|, around inner Schur-CG solves
+(to `inner_tolerance`) whose transforms are split bf16 with FP32 accumulation; cavities
+that have met the bar drop out of the remaining sweeps on the device.
 
 `solve(rhs_list, mass=..., viscosity=...)` returns one `StokesResult` per cavity; the
 zero-copy form `solve_packed` works on one flat device buffer in the batch layout given
@@ -38,14 +43,16 @@ def _as_list(v: Any, n: int, name: str) -> np.ndarray:
 
 
 class GpuBatchedStokes:
-    """Batched FP64 stage solve over cavities of uneven sizes on one GPU."""
+    """Batched stage solve over cavities of uneven sizes on one GPU (FP64 or mixed form)."""
 
     LAUNCH_SHAPES = ((1, 1), (1, 2), (1, 3), (0, 1))
 
     def __init__(self, grids: Sequence[Any], ncols: int = 2, tolerance: float = 1e-9,
-                 krylov_tolerance: float = 1e-12, max_iterations: int = 100, device: int = 0) -> None:
+                 krylov_tolerance: float = 1e-12, max_iterations: int = 100, device: int = 0,
+                 precision: str = "fp64", refine_tolerance: float = 1e-10, inner_tolerance: float = 1e-3,
+                 max_sweeps: int = 30) -> None:
         if (tolerance <= 0 or krylov_tolerance <= 0 or not np.isfinite([tolerance, krylov_tolerance]).all()
-                or max_iterations < 1):
+                or max_iterations < 1 or precision not in ("fp64", "mixed")):
             raise ValueError("Invalid solver controls")
         grids = [g if hasattr(g, "nx") else g.grid for g in grids]
         if not grids:
@@ -56,12 +63,18 @@ class GpuBatchedStokes:
         self.tolerance = tolerance
         self.krylov_tolerance = krylov_tolerance
         self.max_iterations = max_iterations
+        self.precision = precision
         lib = _native.load()
         opts = _native.default_opts()
         opts.tolerance = tolerance
         opts.krylov_tolerance = krylov_tolerance
         opts.max_iterations = max_iterations
         opts.device = device
+        if precision == "mixed":
+            opts.precision = _native.MAC_PRECISION_MIXED
+            opts.refine_tolerance = refine_tolerance
+            opts.inner_tolerance = inner_tolerance
+            opts.max_sweeps = max_sweeps
         n = len(grids)
         nx = (ctypes.c_int32 * n)(*[g.nx for g in grids])
         ny = (ctypes.c_int32 * n)(*[g.ny for g in grids])
@@ -86,6 +99,7 @@ class GpuBatchedStokes:
         self.right_hand_sides = 0
         self.total_iterations = 0
         self.last_iterations: list[list[int]] = []
+        self.last_sweeps = 0
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
@@ -175,6 +189,7 @@ class GpuBatchedStokes:
             self.right_hand_sides += nseg
             self.total_iterations += int(sum(its))
             self.last_iterations = iterations
+            self.last_sweeps = int(self._lib.mac_batch_last_sweeps(self._h))
             self.solve_seconds += elapsed
         _native.raise_for(code, self._lib.mac_batch_last_error(self._h).decode())
         return {"velocity": velocity, "pressure": pressure, "iterations": iterations,

@@ -55,15 +55,18 @@ def test_null_handle_is_an_invalid_argument():
     assert lib.mac_apply_divergence(None, None, 0, 1, None, 0) == _native.MAC_INVALID_ARGUMENT
 
 
-@pytest.mark.parametrize("count,nx,ncols,field,value", [
-    (0, 8, 2, None, None), (2, 2, 2, None, None), (2, 8, 0, None, None), (2, 8, 9, None, None),
-    (2, 8, 2, "tolerance", -1.0), (2, 8, 2, "precision", _native.MAC_PRECISION_MIXED),
+MIXED = _native.MAC_PRECISION_MIXED
+
+
+@pytest.mark.parametrize("count,nx,ncols,fields", [
+    (0, 8, 2, {}), (2, 2, 2, {}), (2, 8, 0, {}), (2, 8, 9, {}), (2, 8, 2, {"tolerance": -1.0}),
+    (2, 8, 2, {"precision": 7}), (2, 8, 2, {"precision": MIXED, "inner_tolerance": 1.5}),
+    (2, 8, 2, {"precision": MIXED, "refine_tolerance": 0.0}), (2, 8, 2, {"precision": MIXED, "max_sweeps": 0}),
 ])
-def test_batch_create_rejects_invalid_cavities_and_controls_before_touching_the_gpu(count, nx, ncols, field,
-                                                                                   value):
+def test_batch_create_rejects_invalid_cavities_and_controls_before_touching_the_gpu(count, nx, ncols, fields):
     lib = _native.load()
     opts = _native.default_opts()
-    if field:
+    for field, value in fields.items():
         setattr(opts, field, value)
     n = max(count, 1)
     arr_i = (ctypes.c_int32 * n)(*([nx] * n))
@@ -76,6 +79,7 @@ def test_batch_create_rejects_invalid_cavities_and_controls_before_touching_the_
 def test_batch_null_handle_is_an_invalid_argument():
     lib = _native.load()
     assert lib.mac_batch_last_error(None) == b"null handle"
+    assert lib.mac_batch_last_sweeps(None) == -1
     assert lib.mac_stokes_solve_batched(None, None, None, None, None, None, None, None, None, 0) == \
         _native.MAC_INVALID_ARGUMENT
     assert lib.mac_batch_set_launch(None, 1, 1) == _native.MAC_INVALID_ARGUMENT

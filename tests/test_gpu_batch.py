@@ -124,3 +124,66 @@ def test_batch_failures_raise_like_the_reference():
     # Still usable after the errors.
     ok = b.solve(rhs, mass=1.0, viscosity=0.1)
     assert all(r.residual < 1e-9 for r in ok)
+
+
+# ---------------------------------------------------------------- mixed form
+def test_mixed_batch_meets_the_bar_and_matches_the_oracle_cavity_by_cavity():
+    """Mixed form (bf16x3 transforms, FP64 defect correction) on the uneven batch: every
+    cavity within rel L2 1e-9 of the oracle's FP64 solve, residual and max|D u| <= 1e-10
+    absolute, and as close to the oracle as the single-cavity mixed solve."""
+    from macsav import GpuBatchedStokes, GpuTensorStokes
+    grids = _grids()
+    rhs = [_rhs(g, 50 + i) for i, g in enumerate(grids)]
+    mass, visc = _shifts()
+    batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9, precision="mixed")
+    print(f"mixed batch of {len(grids)} cavities", flush=True)
+    out = batch.solve(rhs, mass=mass, viscosity=visc)
+    print(f"sweeps {batch.last_sweeps}, inner iterations {batch.last_iterations}", flush=True)
+    assert 1 <= batch.last_sweeps <= 30
+    for i, g in enumerate(grids):
+        orc = mo.TensorStokes(mo.Grid(g.nx, g.ny, g.lx, g.ly), tolerance=1e-9, krylov_tolerance=1e-14)
+        want = orc.solve(rhs[i], mass=mass[i], viscosity=visc[i])
+        assert out[i].residual <= 1e-10 and out[i].divergence_inf <= 1e-10, i
+        assert _rel(out[i].velocity, want.velocity) <= 1e-9, i
+        assert _rel(out[i].pressure, want.pressure) <= 1e-9, i
+        single = GpuTensorStokes(g, tolerance=1e-9, precision="mixed").solve(rhs[i], mass=mass[i],
+                                                                             viscosity=visc[i])
+        assert _rel(out[i].velocity, single.velocity) <= 1e-9, i
+        # the defect measured on the returned fields, in FP64 on the host
+        d = mo.packed_divergence(np.asarray(out[i].velocity)[:, 0], mo.Grid(g.nx, g.ny, g.lx, g.ly))
+        assert np.abs(d).max() <= 1e-10, i
+
+
+def test_mixed_batch_launch_shapes_give_identical_results_and_finished_cavities_drop_out():
+    """Graph replay or direct launches, any CG burst length: bitwise the same fields.  A
+    cavity with a zero right-hand side is finished at the first defect: zero fields and no
+    inner iterations, while the others sweep on."""
+    from macsav import GpuBatchedStokes
+    grids = _grids()
+    batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9, precision="mixed")
+    rhs = [_rhs(g, 70 + i) for i, g in enumerate(grids)]
+    rhs[3] = np.zeros_like(rhs[3])
+    flat = torch.as_tensor(batch.pack(rhs), device="cuda:0")
+    mass, visc = _shifts()
+    first = batch.solve_packed(flat, mass=mass, viscosity=visc)
+    base_v, base_p = first["velocity"].clone(), first["pressure"].clone()
+    vo, po = batch.layout()
+    assert first["iterations"][3] == [0, 0]
+    assert all(sum(its) > 0 for k, its in enumerate(first["iterations"]) if k != 3)
+    assert float(base_v[vo[6]:vo[8]].abs().max()) == 0.0 and float(base_p[po[6]:po[8]].abs().max()) == 0.0
+    for shape in [(1, 1), (1, 3), (0, 1), (0, 2)]:
+        batch.set_launch(*shape)
+        again = batch.solve_packed(flat, mass=mass, viscosity=visc)
+        assert torch.equal(again["velocity"], base_v) and torch.equal(again["pressure"], base_p), shape
+        assert again["iterations"] == first["iterations"]
+
+
+def test_mixed_batch_rejects_a_non_finite_right_hand_side():
+    from macsav import GpuBatchedStokes
+    grids = _grids()
+    batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9, precision="mixed")
+    rhs = [_rhs(g, 90 + i) for i, g in enumerate(grids)]
+    rhs[2][5, 1] = np.nan
+    mass, visc = _shifts()
+    with pytest.raises(ValueError):
+        batch.solve(rhs, mass=mass, viscosity=visc)
