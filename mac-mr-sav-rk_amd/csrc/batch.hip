@@ -34,7 +34,19 @@ struct Seg {
   int nbv, nbp;       // reduction blocks over faces / cells
   int cav;
   int64_t voff, poff; // first element of the segment's velocity / pressure column
+  int64_t t32u, t32v; // mixed form: the u / v blocks' FP32 transform input, rows padded to 8
 };
+
+// Index of packed face f of segment S in the FP32 transform-input layout (the u block's
+// rows padded to round8(nx - 1), the v block's to round8(nx)).
+__device__ inline int64_t pad32_index(const Seg& S, int f) {
+  if (f < S.nu) {
+    const int w = S.nx - 1, j = f / w;
+    return S.t32u + (int64_t)j * ((w + 7) & ~7) + (f - j * w);
+  }
+  const int e = f - S.nu, j = e / S.nx;
+  return S.t32v + (int64_t)j * ((S.nx + 7) & ~7) + (e - j * S.nx);
+}
 
 enum Len : int { LEN_FACES = 0, LEN_CELLS = 1 };
 
@@ -73,9 +85,11 @@ __global__ __launch_bounds__(kThreads) void k_bdiv(const double* __restrict__ ve
   }
 }
 
-// out = add + D^T p per segment (packed velocity layout; add optional).
+// out = add + D^T p per segment (packed velocity layout; add optional), or (out32) D^T p
+// in the FP32 transform-input layout.
 __global__ __launch_bounds__(kThreads) void k_bdt(const double* __restrict__ p, const double* __restrict__ add,
-                                                  double* __restrict__ out, const Seg* segs, const CgState* st) {
+                                                  double* __restrict__ out, const Seg* segs, const CgState* st,
+                                                  float* __restrict__ out32) {
   const int s = blockIdx.y;
   if (st != nullptr && !st[s].active) return;
   const Seg S = segs[s];
@@ -95,7 +109,8 @@ __global__ __launch_bounds__(kThreads) void k_bdt(const double* __restrict__ p, 
     value = (pc[lower] - pc[lower + nx]) / S.hy;
   }
   if (add != nullptr) value = add[S.voff + f] + value;
-  out[S.voff + f] = value;
+  if (out32 != nullptr) out32[pad32_index(S, f)] = (float)value;
+  else out[S.voff + f] = value;
 }
 
 template <int MODE, bool DOT>
@@ -198,12 +213,13 @@ __global__ __launch_bounds__(kThreads) void k_bupdate_xr(double* __restrict__ x,
 }
 
 // Per segment: max|mass u + visc K u - D^T p - f|, max|f|, max|u| (partials at q * nseg * pstride);
-// WRITE: the momentum residual r_u = f - (mass u + visc K u - D^T p) into ru as well.
+// WRITE: the momentum residual r_u = f - (mass u + visc K u - D^T p) as well, in FP32 in the
+// transform-input layout (the split transform keeps 16 of its bits).
 template <bool WRITE>
 __global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__ vel, const double* __restrict__ f,
                                                       const double* __restrict__ p, const Seg* segs,
                                                       const double* __restrict__ mass_visc, double* partials,
-                                                      int pstride, double* __restrict__ ru) {
+                                                      int pstride, float* __restrict__ ru32) {
   const int s = blockIdx.y;
   const Seg S = segs[s];
   if ((int)blockIdx.x >= S.nbv) return;
@@ -248,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__
     const double fv = fc[fidx];
     const double vv = vc[fidx];
     const double d = face_defect(mass, vv, visc, k.kv, lift, fv);
-    if (WRITE) ru[S.voff + fidx] = -d;
+    if (WRITE) ru32[pad32_index(S, fidx)] = (float)(-d);
     md = fmax(md, absn(d));
     mf = fmax(mf, absn(fv));
     mv = fmax(mv, absn(vv));
@@ -310,6 +326,18 @@ __global__ void k_bgate(CgState* st, const int* __restrict__ active, int nseg, i
   }
   if (mode == GATE_POST) iters_total[s] += st[s].iters;
   st[s].active = active[s];
+}
+
+// du += alpha L~^-1 D^T p (vout) on the active segments: the inner CG builds the velocity
+// correction du = W + sum_k alpha_k L~^-1 D^T p_k = L~^-1 (r_u + D^T x) as it goes.
+__global__ __launch_bounds__(kThreads) void k_bacc_du(double* __restrict__ du, const double* __restrict__ vout,
+                                                      const Seg* segs, const CgState* st) {
+  const int s = blockIdx.y;
+  if (!st[s].active) return;
+  const Seg S = segs[s];
+  const double alpha = st[s].alpha;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.size; i += gridDim.x * kThreads)
+    du[S.voff + i] = fma(alpha, vout[S.voff + i], du[S.voff + i]);
 }
 
 // u += du, p += dp on the segments whose state is active.
@@ -530,7 +558,7 @@ class BatchSolver {
     bfinalize(RED_SUM, LEN_CELLS, sums_);
     k_bsub_mean_dot<false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, sums_, nullptr, segs_, nullptr,
                                                                           pstride_, nullptr);
-    k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(x_, f, vin_, segs_, nullptr);
+    k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(x_, f, vin_, segs_, nullptr, nullptr);
     velocity_inverse(false);
     copy(vout_, vel, V_);
     copy(x_, pres, P_);
@@ -564,7 +592,8 @@ class BatchSolver {
   //   FP64 defect r_u = f - (mass u + visc K u + G p), r_c = -D u, and their maxima;
   //   stop when max|r_u| / (1 + max|f|) <= refine_tolerance and max|D u| <= refine_tolerance;
   //   inner solve with bf16x3 transforms: W = L~^-1 r_u, b = r_c - D W (mean removed),
-  //   Schur PCG to inner_tolerance, dp = x - mean(x), du = L~^-1 (r_u + D^T dp);
+  //   Schur PCG to inner_tolerance, dp = x - mean(x), du = L~^-1 (r_u + D^T dp), summed
+  //   during the CG as W + sum_k alpha_k L~^-1 D^T p_k from the Schur products;
   //   u += du, p += dp.
   // Finished cavities drop out on the device (their segments' states stay inactive); the
   // host reads the outer flags and maxima once per sweep.  The CG vectors, dot products,
@@ -584,7 +613,7 @@ class BatchSolver {
     int sweep = 0;
     for (;; ++sweep) {
       k_bdefect<true><<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(uacc_, f, pacc_, segs_, mv_, partials_, pstride_,
-                                                                     ru_);
+                                                                     tf0_);
       for (int q = 0; q < 3; ++q)
         k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_ + (int64_t)q * nseg_ * pstride_, pstride_,
                                                                 segs_, LEN_FACES, chk_ + q * nseg_);
@@ -613,8 +642,8 @@ class BatchSolver {
       if (changed) HIP_CHECK(hipMemcpy(outer_, act.data(), sizeof(int) * nseg_, hipMemcpyHostToDevice));
       // Inner solve for (du, dp) on the unfinished cavities.
       k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_SET, nullptr);
-      copy(ru_, vin_, V_);
-      velocity_inverse(true);
+      velocity_inverse(true);  // W = L~^-1 r_u (r_u in the FP32 transform input)
+      copy(vout_, du_, V_);    // du starts at W; the CG adds alpha_k L~^-1 D^T p_k
       bdiv(vout_, b_, -1.0, RED_SUM, cg_, rc_);
       bfinalize(RED_SUM, LEN_CELLS, sums_);
       k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, sums_, b_, segs_, partials_, pstride_,
@@ -630,15 +659,13 @@ class BatchSolver {
       copy(sums2_, rr_, nseg_);
       run_cg();
       k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_POST, iters_total_);
+      // dp = x - mean(x) (D^T annihilates the mean, so du needs no correction).
       k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, nullptr, segs_, LEN_CELLS,
                                                                                partials_, pstride_, cg_);
       bfinalize(RED_SUM, LEN_CELLS, sums_);
       k_bsub_mean_dot<false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, sums_, nullptr, segs_, nullptr,
                                                                             pstride_, cg_);
-      k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(x_, ru_, vin_, segs_, cg_);
-      velocity_inverse(true);
-      k_baccum<<<dim3((unsigned)std::min<int64_t>((maxfaces_ + kThreads - 1) / kThreads, 64), (unsigned)nseg_),
-                 kThreads, 0, stream_>>>(uacc_, vout_, pacc_, x_, segs_, cg_);
+      k_baccum<<<acc_grid(), kThreads, 0, stream_>>>(uacc_, du_, pacc_, x_, segs_, cg_);
     }
     out.sweeps = sweep;
     // p mean-zero; hand out u and p.
@@ -719,7 +746,7 @@ class BatchSolver {
     cg_rho(cg_, sums2_, nseg_, stream_);
     k_bupdate_p<<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(p_, z_, segs_, cg_);
     // q = S p = D L^-1 D^T p, mean removed; alpha = rho / (p, q)
-    k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(p_, nullptr, vin_, segs_, cg_);
+    k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(p_, nullptr, vin_, segs_, cg_, mixed_ ? tf0_ : nullptr);
     velocity_inverse(true);
     bdiv(vout_, q_, 1.0, RED_SUM, cg_);
     bfinalize(RED_SUM, LEN_CELLS, sums_);
@@ -727,6 +754,7 @@ class BatchSolver {
                                                                          cg_);
     bfinalize(RED_SUM, LEN_CELLS, sums2_);
     cg_alpha(cg_, sums2_, nseg_, stream_);
+    if (mixed_) k_bacc_du<<<acc_grid(), kThreads, 0, stream_>>>(du_, vout_, segs_, cg_);
     k_bupdate_xr<<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, r_, p_, q_, segs_, cg_, partials_, pstride_);
     bfinalize(RED_SUM, LEN_CELLS, rr_);
   }
@@ -738,7 +766,7 @@ class BatchSolver {
     for (int k = 0; k < 4; ++k) {
       GemmTable t = mixed_ ? velb_tab_[k] : vel_tab_[k];
       t.use_state = use_state ? 1 : 0;
-      if (mixed_) run_table(t, kPassTypes[k][0], kPassTypes[k][1], k == 0 ? velb_avec_ : true);
+      if (mixed_) run_table(t, k == 0 ? DT_F32 : kPassTypes[k][0], kPassTypes[k][1], true);
       else gemm_f64_table(t, stream_);
     }
   }
@@ -749,8 +777,9 @@ class BatchSolver {
       else gemm_f64_table(pre_tab_[k], stream_);
     }
   }
-  // A-operand / output types of the four passes in the mixed form: the FP64 input, two
-  // FP32 intermediates, the FP64 output.
+  // A-operand / output types of the four passes in the mixed form: the input (FP64 for the
+  // pressure, the FP32 transform input for the velocity), two FP32 intermediates, the FP64
+  // output.
   static constexpr int kPassTypes[4][2] = {{DT_F64, DT_F32}, {DT_F32, DT_F32}, {DT_F32, DT_F32}, {DT_F32, DT_F64}};
   void run_table(const GemmTable& t, int a_type, int c_type, bool a_vec) {
     if (gemm_bf16_table(t, a_type, c_type, a_vec, stream_) != 0)
@@ -837,11 +866,17 @@ class BatchSolver {
         t += (need + 3) & ~int64_t(3);
       }
     }
+    tf0_ = static_cast<float*>(dev_alloc(sizeof(float) * t));
     tf1_ = static_cast<float*>(dev_alloc(sizeof(float) * t));
     tf2_ = static_cast<float*>(dev_alloc(sizeof(float) * t));
+    for (int q = 0; q < nseg_; ++q) {
+      segs_host_[q].t32u = toff_[3 * q];
+      segs_host_[q].t32v = toff_[3 * q + 1];
+    }
+    HIP_CHECK(hipMemcpy(segs_, segs_host_.data(), sizeof(Seg) * nseg_, hipMemcpyHostToDevice));
     uacc_ = static_cast<double*>(dev_alloc(sizeof(double) * V_));
     pacc_ = static_cast<double*>(dev_alloc(sizeof(double) * P_));
-    ru_ = static_cast<double*>(dev_alloc(sizeof(double) * V_));
+    du_ = static_cast<double*>(dev_alloc(sizeof(double) * V_));
     rc_ = static_cast<double*>(dev_alloc(sizeof(double) * P_));
     outer_ = static_cast<int*>(dev_alloc(sizeof(int) * nseg_));
     iters_total_ = static_cast<int*>(dev_alloc(sizeof(int) * nseg_));
@@ -883,7 +918,7 @@ class BatchSolver {
         float* t1 = tf1_ + toff_[3 * q + B.reg];
         float* t2 = tf2_ + toff_[3 * q + B.reg];
         const int ry = round8(B.ny), rx = round8(B.nx);
-        passb(v[0], vin_ + o, B.nx, B.mx, false, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr, st);
+        passb(v[0], tf0_ + toff_[3 * q + B.reg], rx, B.mx, false, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr, st);
         passb(v[1], t1, ry, B.my, false, t2, ry, B.nx, B.ny, B.ny, B.lx, B.ly, st);
         passb(v[2], t2, ry, B.my, true, t1, rx, B.nx, B.ny, B.ny, nullptr, nullptr, st);
         passb(v[3], t1, rx, B.mx, true, vout_ + o, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr, st);
@@ -907,8 +942,10 @@ class BatchSolver {
         if (e.lda % 2 != 0 || reinterpret_cast<uintptr_t>(e.A) % 16 != 0) return false;
       return true;
     };
-    velb_avec_ = vec64(v[0]);
     preb_avec_ = vec64(pr[0]);
+    for (const GemmEntry& e : v[0])
+      if (e.lda % 4 != 0 || reinterpret_cast<uintptr_t>(e.A) % 16 != 0)
+        throw HipFailure("misaligned FP32 transform input");
     for (int k = 1; k < 4; ++k)
       for (const std::vector<GemmEntry>* t : {&v[k], &pr[k]})
         for (const GemmEntry& e : *t)
@@ -1013,6 +1050,9 @@ class BatchSolver {
   // ------------------------------------------------------------ helpers
   dim3 seg_grid(int len) const { return dim3((unsigned)(len == LEN_FACES ? nbv_max_ : nbp_max_), (unsigned)nseg_); }
   dim3 dt_grid() const { return dim3((unsigned)((maxfaces_ + kThreads - 1) / kThreads), (unsigned)nseg_); }
+  dim3 acc_grid() const {
+    return dim3((unsigned)std::min<int64_t>((maxfaces_ + kThreads - 1) / kThreads, 64), (unsigned)nseg_);
+  }
   void bdiv(const double* vel, double* out, double sign, int mode, const CgState* st, const double* add = nullptr) {
     const dim3 g = seg_grid(LEN_CELLS);
     if (mode == RED_SUM)
@@ -1084,14 +1124,14 @@ class BatchSolver {
   bool mixed_ = false;
   std::vector<std::unique_ptr<DevMatrixB>> matsb_;
   std::vector<int64_t> toff_;
-  float *tf1_ = nullptr, *tf2_ = nullptr;
-  double *uacc_ = nullptr, *pacc_ = nullptr, *ru_ = nullptr, *rc_ = nullptr;
+  float *tf0_ = nullptr, *tf1_ = nullptr, *tf2_ = nullptr;  // transform input, intermediates
+  double *uacc_ = nullptr, *pacc_ = nullptr, *rc_ = nullptr, *du_ = nullptr;
   int *outer_ = nullptr, *iters_total_ = nullptr;
   std::vector<GemmEntry> velb_host_[4], preb_host_[4];
   GemmEntry *velb_dev_[4] = {}, *preb_dev_[4] = {};
   int *velb_beg_[4] = {}, *preb_beg_[4] = {};
   GemmTable velb_tab_[4] = {}, preb_tab_[4] = {};
-  bool velb_avec_ = false, preb_avec_ = false;
+  bool preb_avec_ = false;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
 };
