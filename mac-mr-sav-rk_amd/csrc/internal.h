@@ -96,7 +96,8 @@ struct GridDims {
 // Per-column Schur-CG scalars, resident in device memory so the whole recurrence
 // stays on the GPU; the host only reads the `active` flags once per iteration.
 struct CgState {
-  double bnorm, tol, rr, rho, rho_prev, alpha, beta, pad0;
+  double bnorm, tol, rr, rho, rho_prev, alpha, beta;
+  double mtol;  // > 0: the column also stops once max|r| <= mtol (posted checks only)
   int active, iters, converged, pad1;
 };
 
@@ -215,9 +216,11 @@ void cg_check_post(CgState* st, double* rr, int* flags, int ncols, Mailbox* mb, 
 void post_values(const double* a, int na, const double* b, int nb, Mailbox* mb, int seq, hipStream_t s);
 
 // CG scalar steps (one thread per column).
-// rr_out (optional) also receives bb, the first recurrence residual (r, r).
+// rr_out (optional) also receives bb, the first recurrence residual (r, r).  mtol > 0: a
+// column also stops, at a posted check whose partials carry max|r| (the mixed update's),
+// once max|r| <= mtol.
 void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s,
-             double* rr_out = nullptr);
+             double* rr_out = nullptr, double mtol = 0.0);
 // A mixed sweep's inner-CG start on b (n per column, contiguous): b -= sums[c] / n,
 // r = b, x = 0; (b, b) into bb[c] and max|b| into bmax[c] (partials: 2 * ncols *
 // reduce_blocks(n) entries); r32 (optional) receives r rounded to FP32.

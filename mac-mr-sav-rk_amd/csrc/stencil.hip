@@ -368,13 +368,14 @@ __global__ void k_copy_cols(const double* src, int64_t ss, double* dst, int64_t 
 // ---- CG scalars (scipy 1.15.3 cg loop: stop when ||r|| < rtol*||b||, tested at the
 // top of each iteration; rho = (r,z); beta = rho/rho_prev; alpha = rho/(p,q)).
 __global__ void k_cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols,
-                          double* rr_out) {
+                          double* rr_out, double mtol) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
   if (rr_out != nullptr) rr_out[c] = bb[c];
   CgState s{};
   s.bnorm = sqrt(bb[c]);
   s.tol = rtol * s.bnorm;
+  s.mtol = mtol;
   s.rr = bb[c];
   s.active = (bmax[c] != 0.0 && s.bnorm != 0.0) ? 1 : 0;
   s.converged = s.active ? 0 : 1;
@@ -416,18 +417,25 @@ __global__ __launch_bounds__(kThreads) void k_cg_check_post(CgState* st, double*
                                                             int nblk, int* flags, int ncols, Mailbox* mb,
                                                             int seq) {
   __shared__ double sh[4];
+  __shared__ double rmax[64];
   if (partials != nullptr) {
     for (int k = 0; k < ncols; ++k) {
       const double v = combine_partials<RED_SUM>(partials + (int64_t)k * nblk, nblk, sh);
       if (threadIdx.x == 0) rr[k] = v;
       __syncthreads();  // sh is reused; rr[k] is read by thread k below
+      // max|r| from the update's second set of partials, for a column with a max-norm stop
+      double m = INFINITY;
+      if (st[k].mtol > 0.0) m = combine_partials<RED_MAXABS>(partials + (int64_t)(ncols + k) * nblk, nblk, sh);
+      if (threadIdx.x == 0) rmax[k] = m;
+      __syncthreads();
     }
   }
   const int c = threadIdx.x;
   if (c < ncols) {
     if (st[c].active) {
       st[c].rr = rr[c];
-      if (sqrt(rr[c]) < st[c].tol) {
+      const bool small = partials != nullptr && st[c].mtol > 0.0 && rmax[c] <= st[c].mtol;
+      if (sqrt(rr[c]) < st[c].tol || small) {
         st[c].active = 0;
         st[c].converged = 1;
       }
@@ -546,16 +554,23 @@ __global__ __launch_bounds__(kThreads) void k_update_xr_mean(double* __restrict_
   __shared__ double sh[4];
   const double alpha = st[c].alpha, qm = qmean[c];
   const int64_t off = (int64_t)c * stride;
-  double accum = 0.0;
+  double accum = 0.0, amax = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
     x[off + i] += alpha * p[off + i];
     const double rv = r[off + i] - alpha * (q[off + i] - qm);
     r[off + i] = rv;
     if (r32 != nullptr) r32[off + i] = (float)rv;
     accum += rv * rv;
+    amax = fmax(amax, absn(rv));
   }
   accum = block_reduce<RED_SUM>(accum, sh);
-  if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
+  __syncthreads();
+  amax = block_reduce<RED_MAXABS>(amax, sh);
+  // (r, r) per block, then max|r| per block in a second set after every column's sums.
+  if (threadIdx.x == 0) {
+    partials[(int64_t)c * gridDim.x + blockIdx.x] = accum;
+    partials[(int64_t)(gridDim.y + c) * gridDim.x + blockIdx.x] = amax;
+  }
 }
 
 __global__ void k_cg_rho(CgState* st, const double* rz, int ncols) {
@@ -806,8 +821,8 @@ void copy_cols(const double* src, int64_t ss, double* dst, int64_t ds, int64_t n
 }
 
 void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int ncols, hipStream_t s,
-             double* rr_out) {
-  k_cg_init<<<(ncols + 63) / 64, 64, 0, s>>>(st, bb, bmax, rtol, ncols, rr_out);
+             double* rr_out, double mtol) {
+  k_cg_init<<<(ncols + 63) / 64, 64, 0, s>>>(st, bb, bmax, rtol, ncols, rr_out, mtol);
 }
 void sweep_start(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,
                  double* bb, double* bmax, hipStream_t s, float* r32) {

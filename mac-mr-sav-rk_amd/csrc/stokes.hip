@@ -775,7 +775,11 @@ class Solver {
       // b -= mean, r = b, x = 0, with (b, b) and max|b| for the start test.
       sweep_start(b_, r_, x_, np_, ncols, sums_, partials_, sums2_, sums3_, stream_, tfb_);
       r32_ = tfb_;
-      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_, rr_);
+      // The inner CG's recurrence residual is r = b - S~ x = -D (u + du): the sweep's new
+      // divergence.  Besides the relative test, a column stops once max|r| is half the bar,
+      // where the sweep is certain to meet it (the momentum defect it leaves is the split
+      // operator's error on an already small correction).
+      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_, rr_, 0.5 * opts_.refine_tolerance);
       accumulate_ = true;
       pcg(ncols, mass, visc, inner, false);
       accumulate_ = false;
