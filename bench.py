@@ -355,10 +355,11 @@ def bench_learned(args) -> dict:
 ETA2 = 1.0 - 0.5 ** 0.5                          # sdirk2 diagonal
 
 
-def record_cavity_stages(grids, nu, steps: int, dt: float = 5e-4, lid: float = 1.0):
+def record_cavity_stages(grids, nu, steps: int, dt: float = 5e-4, lid: float = 1.0, spinup: int = 0):
     """Per cavity, the stage solves ((size, 2) device right-hand sides, mass, viscosity) of
-    its first `steps` sdirk2_mrsav steps from rest with the top wall moving at `lid`, each
-    cavity on its own device-resident model with the single-cavity FP64 backend."""
+    `steps` sdirk2_mrsav steps after `spinup` unrecorded ones, from rest with the top wall
+    moving at `lid`, each cavity on its own device-resident model with the single-cavity
+    FP64 backend."""
     import numpy as np
     from macsav import GpuTensorStokes
     from macsav.mac_ns_device import DeviceMACNavierStokes
@@ -375,12 +376,16 @@ def record_cavity_stages(grids, nu, steps: int, dt: float = 5e-4, lid: float = 1
             rec.append((rhs.clone(), mass, viscosity))
             return inner(rhs, mass=mass, viscosity=viscosity, pressure_guess=pressure_guess)
 
-        backend.solve = recording
         state = model.state(0.0, np.zeros(g.size))
+        for _ in range(spinup):
+            state = scheme.step(model, state, dt).state
+        backend.solve = recording
         for _ in range(steps):
             state = scheme.step(model, state, dt).state
         del backend.solve
         stages.append(rec)
+        if len(stages) % 32 == 0:
+            print(f"[cavities] recorded {len(stages)} of {len(grids)} cavities", file=sys.stderr, flush=True)
     return stages
 
 
@@ -389,11 +394,13 @@ def bench_cavities(args) -> dict:
     {64, 96, ..., 512}, nu log-uniform on [1e-4, 1e-2], the top wall moving at 1,
     sdirk2_mrsav with dt = 5e-4.  The first `--record-steps` steps of every cavity are
     run from rest on its own device model and their stage solves recorded (two
-    right-hand sides each); each recorded stage is then one batched solve of every
-    cavity, timed over all stages, in the mixed form (`value`: FP64 CG with split-bf16
-    transforms under FP64 defect correction to residual and max|D u| <= 1e-10) and the
-    FP64 form.  The baseline is a loop of single-cavity plain FP64 GPU solves
-    (GpuTensorStokes, one cavity at a time) of the same right-hand sides."""
+    right-hand sides each).  Each recorded stage is then one batched solve of every
+    cavity: the first `--fill-steps` steps untimed (they fill the history), the rest
+    timed, in the mixed form with the same-stage history start (`value`), the mixed form
+    from zero (FP64 CG with split-bf16 transforms under FP64 defect correction to
+    residual and max|D u| <= 1e-10) and the FP64 form.  The baseline is a loop of
+    single-cavity plain FP64 GPU solves (GpuTensorStokes, one cavity at a time, zero
+    start) of the same timed right-hand sides."""
     import numpy as np
     import torch
     from macsav import GpuBatchedStokes, GpuTensorStokes, MACGrid
@@ -403,25 +410,39 @@ def bench_cavities(args) -> dict:
     nu = 10.0 ** rng.uniform(-4, -2, size=n)
     grids = [MACGrid(int(a), int(b)) for a, b in sizes]
     t0 = time.perf_counter()
-    stages = record_cavity_stages(grids, nu, args.record_steps)
+    stages = record_cavity_stages(grids, nu, args.record_steps, spinup=args.spinup_steps)
     record_s = time.perf_counter() - t0
     n_stages = len(stages[0])
+    first = min(2 * args.fill_steps, n_stages - 1)
+    timed = n_stages - first
     print(f"[cavities] recorded {n_stages} stages of {n} cavities in {record_s:.1f}s", file=sys.stderr, flush=True)
     masses = [np.array([stages[i][k][1] for i in range(n)]) for k in range(n_stages)]
     viscs = [np.array([stages[i][k][2] for i in range(n)]) for k in range(n_stages)]
     forms, outs = {}, {}
-    for form in ("mixed", "fp64"):
+    flats = None
+    for form in ("mixed+history", "mixed", "fp64"):
+        precision = "fp64" if form == "fp64" else "mixed"
         t0 = time.perf_counter()
-        batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9, precision=form, inner_tolerance=args.inner_tolerance)
+        batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9, precision=precision,
+                                 inner_tolerance=args.inner_tolerance)
         setup = time.perf_counter() - t0
-        flats = [torch.as_tensor(batch.pack([stages[i][k][0] for i in range(n)])) for k in range(n_stages)]
+        if flats is None:
+            flats = [torch.as_tensor(batch.pack([stages[i][k][0] for i in range(n)])) for k in range(n_stages)]
         batch.solve_packed(flats[-1], mass=masses[-1], viscosity=viscs[-1])
         timings = batch.autotune(flats[-1], mass=masses[-1], viscosity=viscs[-1], repeats=1)
+        if form == "mixed+history":
+            batch.set_history(2, args.history_steps)
+        for k in range(first):
+            batch.solve_packed(flats[k], mass=masses[k], viscosity=viscs[k])
+            if k % 2 == 1 and form == "mixed+history":
+                batch.commit_history()
         quiesce()
         t0 = time.perf_counter()
         res, its, sweeps = [], 0, []
-        for k in range(n_stages):
+        for k in range(first, n_stages):
             out = batch.solve_packed(flats[k], mass=masses[k], viscosity=viscs[k])
+            if k % 2 == 1 and form == "mixed+history":
+                batch.commit_history()
             res.append(out)
             its += sum(sum(c) for c in out["iterations"])
             sweeps.append(batch.last_sweeps)
@@ -430,48 +451,57 @@ def bench_cavities(args) -> dict:
         resume()
         worst_r = max(max(o["residual"]) for o in res)
         worst_d = max(max(o["divergence_inf"]) for o in res)
-        forms[form] = {"cavity_stage_solves_per_s": n * n_stages / elapsed, "ms_per_stage": 1e3 * elapsed / n_stages,
-                       "cg_iterations_per_rhs": its / (2 * n * n_stages), "sweeps_per_stage": sweeps,
+        forms[form] = {"cavity_stage_solves_per_s": n * timed / elapsed, "ms_per_stage": 1e3 * elapsed / timed,
+                       "cg_iterations_per_rhs": its / (2 * n * timed), "sweeps_per_stage": sweeps,
                        "worst_residual": worst_r, "worst_divergence_inf": worst_d,
                        "meets_bar": worst_r <= BAR and worst_d <= BAR, "launch_shape": list(batch.launch),
                        "autotune_seconds": {f"{a},{b}": v for (a, b), v in timings.items()}, "setup_seconds": setup}
         outs[form] = (batch, res[-1])
         print(f"[cavities] {form}: {forms[form]}", file=sys.stderr, flush=True)
-        del flats
-    # Mixed against FP64 on the last stage, cavity by cavity (relative L2 of the velocity).
-    (bm, om), (bf, of) = outs["mixed"], outs["fp64"]
-    vo = bm.velocity_offsets
-    rel = 0.0
-    for i in range(n):
-        a, b = om["velocity"][vo[2 * i]:vo[2 * i + 2]], of["velocity"][vo[2 * i]:vo[2 * i + 2]]
-        rel = max(rel, float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
-    # Baseline: the same stage solves, one cavity at a time through the single-cavity path.
+    del flats
+    # The mixed forms against FP64 on the last stage, cavity by cavity (relative L2 of u).
+    rel = {}
+    bf, of = outs["fp64"]
+    vo = bf.velocity_offsets
+    for form in ("mixed+history", "mixed"):
+        om = outs[form][1]
+        worst = 0.0
+        for i in range(n):
+            a, b = om["velocity"][vo[2 * i]:vo[2 * i + 2]], of["velocity"][vo[2 * i]:vo[2 * i + 2]]
+            worst = max(worst, float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
+        rel[form] = worst
+    # Baseline: the same timed stage solves, one cavity at a time through the single path.
     singles = [GpuTensorStokes(g, tolerance=1e-9) for g in grids]
     for i in range(n):
         singles[i].solve(stages[i][0][0], mass=stages[i][0][1], viscosity=stages[i][0][2])
     quiesce()
     t0 = time.perf_counter()
-    for k in range(n_stages):
+    for k in range(first, n_stages):
         for i in range(n):
             singles[i].solve(stages[i][k][0], mass=stages[i][k][1], viscosity=stages[i][k][2])
     torch.cuda.synchronize()
     loop_s = time.perf_counter() - t0
     resume()
-    loop = n * n_stages / loop_s
-    value = forms["mixed"]["cavity_stage_solves_per_s"]
+    loop = n * timed / loop_s
+    head = forms["mixed+history"]
+    value = head["cavity_stage_solves_per_s"]
     return {"metric": "cavity stage solves/s, batched uneven lid-driven cavities", "value": value,
-            "unit": "cavity stage solves/s", "n_gpus": 1, "steps": n_stages, "warmup": 1,
-            "ms_per_step": forms["mixed"]["ms_per_stage"], "higher_is_better": True, "scaling": "weak",
+            "unit": "cavity stage solves/s", "n_gpus": 1, "steps": timed, "warmup": first,
+            "ms_per_step": head["ms_per_stage"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "bf16-gemm/f64-cg",
-            "data": f"recorded: the first {args.record_steps} sdirk2_mrsav steps ({n_stages} stage solves, 2 RHS "
-                    f"each) of every cavity from rest with the lid moving at 1",
+            "data": f"recorded: sdirk2_mrsav steps {args.spinup_steps + 1}..{args.spinup_steps + args.record_steps} "
+                    f"({n_stages} stage solves, 2 RHS each) of every cavity from rest with the lid moving at 1; "
+                    f"stages {first}..{n_stages - 1} timed",
             "config": {"workload": f"{n} cavities, nx,ny uniform on {{64..512 step 32}}, nu log-uniform "
-                                   f"[1e-4,1e-2], lid 1, sdirk2_mrsav dt=5e-4", "total_faces": bm.total_faces,
-                       "form": "mixed", "bar": "residual<=1e-10 and divergence_inf<=1e-10 (absolute)"},
-            "forms": forms, "mixed_vs_fp64_max_rel_l2_velocity": rel,
+                                   f"[1e-4,1e-2], lid 1, sdirk2_mrsav dt=5e-4", "total_faces": bf.total_faces,
+                       "form": "mixed+history", "history_steps": args.history_steps,
+                       "inner_tolerance": args.inner_tolerance,
+                       "bar": "residual<=1e-10 and divergence_inf<=1e-10 (absolute)"},
+            "forms": forms, "max_rel_l2_velocity_vs_fp64": rel,
             "per_cavity_loop_fp64": {"value": loop, "unit": "cavity stage solves/s"},
             "speedup_vs_loop_fp64": value / loop,
             "speedup_vs_batched_fp64": value / forms["fp64"]["cavity_stage_solves_per_s"],
+            "speedup_vs_batched_mixed_zero_start": value / forms["mixed"]["cavity_stage_solves_per_s"],
             "record_seconds": record_s}
 
 
@@ -753,7 +783,9 @@ def main() -> int:
     ap.add_argument("--t-end", type=float, default=0.05, help="combined run: final time")
     ap.add_argument("--max-steps", type=int, default=2000, help="combined run: accepted-step cap")
     ap.add_argument("--cavities", type=int, default=256)
-    ap.add_argument("--record-steps", type=int, default=4, help="cavities: sdirk2_mrsav steps recorded per cavity")
+    ap.add_argument("--record-steps", type=int, default=8, help="cavities: sdirk2_mrsav steps recorded per cavity")
+    ap.add_argument("--fill-steps", type=int, default=3, help="cavities: recorded steps solved untimed first")
+    ap.add_argument("--spinup-steps", type=int, default=72, help="cavities: steps run before the recorded ones")
     ap.add_argument("--inner-tolerance", type=float, default=1e-4, help="cavities: inner CG tolerance, mixed form")
     ap.add_argument("--train-seeds", type=int, default=40)
     ap.add_argument("--test-seeds", type=int, default=10)

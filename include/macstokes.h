@@ -265,6 +265,16 @@ int32_t mac_batch_layout(void* handle, int64_t* velocity_offsets, int64_t* press
 int32_t mac_batch_set_launch(void* handle, int32_t use_graph, int32_t iters_per_check);
 /* Outer sweeps of the last batched solve (mixed form; 1 for FP64), -1 for a null handle. */
 int32_t mac_batch_last_sweeps(void* handle);
+/* No counterpart in the reference (its solves start from zero).  Mixed form only: each
+ * batched solve's outer iterate (u, p) starts from the Lagrange extrapolation in time of
+ * the solves `period`, 2 period, ... back (the same stage of the last `steps` steps,
+ * 1..6), fewer while fewer are recorded; period 0 switches it off and every call
+ * restarts the record.  The step sizes come from the solves' viscosity / mass (cavity 0;
+ * the cavities step together).  MAC_INVALID_ARGUMENT for an FP64 handle. */
+int32_t mac_batch_set_history(void* handle, int32_t period, int32_t steps);
+/* rollback 0 keeps the solves since the last mark in the history (an accepted step);
+ * 1 forgets them (a rejected trial). */
+int32_t mac_batch_history_mark(void* handle, int32_t rollback);
 /* mass, viscosity: per cavity.  iterations: per segment.  residual, divergence_inf: per
  * cavity (max over its columns, the reference's definitions; in the mixed form those of the
  * last FP64 defect; iterations then sum the inner CG iterations over the sweeps).  Errors

@@ -481,6 +481,8 @@ class BatchSolver {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : allocs_) (void)hipFree(p);
+    if (hist_u_) (void)hipFree(hist_u_);
+    if (hist_p_) (void)hipFree(hist_p_);
     if (host_flags_) (void)hipHostFree(host_flags_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -508,7 +510,36 @@ class BatchSolver {
     std::vector<double> residual, divergence, vmax;  // per cavity
     std::vector<int> failed_cg;               // cavities whose CG did not converge
     int sweeps = 1;                           // mixed form: outer sweeps of the batch
+    bool warm = false;                        // mixed form: started from the history
   };
+
+  // Same-stage history for the mixed form (period: solves per step; steps: how many steps
+  // the extrapolation reaches back, 1..kMaxExtrapolation; period 0 switches it off).  All
+  // cavities step together, so one step-size sequence (cavity 0's viscosity / mass) gives
+  // every cavity's weights.  The ring keeps steps + 1 steps, so the solves of an unmarked
+  // trial never overwrite the committed ones the extrapolation reads.
+  void set_history(int period, int steps) {
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    const int slots = period > 0 ? (steps + 1) * period : 0;
+    if (slots > hist_slots_) {
+      if (hist_u_) HIP_CHECK(hipFree(hist_u_));
+      if (hist_p_) HIP_CHECK(hipFree(hist_p_));
+      hist_u_ = hist_p_ = nullptr;
+      hist_slots_ = 0;
+      HIP_CHECK(hipMalloc(&hist_u_, sizeof(double) * V_ * slots));
+      HIP_CHECK(hipMalloc(&hist_p_, sizeof(double) * P_ * slots));
+      hist_slots_ = slots;
+    }
+    hist_period_ = period;
+    hist_steps_ = steps;
+    hist_count_ = hist_committed_ = 0;
+    hist_shift_.assign((size_t)std::max(slots, 1), 0.0);
+  }
+  // rollback: forget the solves since the last mark (a rejected trial), else keep them.
+  void history_mark(bool rollback) {
+    if (rollback) hist_count_ = hist_committed_;
+    else hist_committed_ = hist_count_;
+  }
 
   // f, vel, pres: device buffers in the batch layout.  mass, visc: per cavity.
   // Per-segment max|f| (non-finite values reported as inf).
@@ -603,8 +634,27 @@ class BatchSolver {
   Outcome solve_mixed(const double* f, const double* mass, const double* visc, double* vel, double* pres) {
     const int nc = count();
     set_shifts(mass, visc);
-    k_bfill<<<blocks_for(V_), kThreads, 0, stream_>>>(uacc_, V_, 0.0);
-    k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(pacc_, P_, 0.0);
+    // Outer start: the same-stage history's extrapolation when one is recorded, else zero.
+    const double shift = mass[0] > 0.0 ? visc[0] / mass[0] : 0.0;
+    const int slots = hist_period_ > 0 ? (hist_steps_ + 1) * hist_period_ : 0;
+    const int steps = hist_period_ > 0 ? (int)std::min<int64_t>(hist_count_ / hist_period_, hist_steps_) : 0;
+    if (steps > 0) {
+      double h[kMaxExtrapolation + 1] = {shift}, w[kMaxExtrapolation];
+      const double* su[kMaxExtrapolation];
+      const double* sp[kMaxExtrapolation];
+      for (int k = 1; k <= steps; ++k) {
+        const int slot = (int)((hist_count_ - (int64_t)k * hist_period_) % slots);
+        h[k] = hist_shift_[slot];
+        su[k - 1] = hist_u_ + (int64_t)slot * V_;
+        sp[k - 1] = hist_p_ + (int64_t)slot * P_;
+      }
+      history_weights(h, steps, w);
+      extrapolate(su, w, steps, uacc_, V_, stream_);
+      extrapolate(sp, w, steps, pacc_, P_, stream_);
+    } else {
+      k_bfill<<<blocks_for(V_), kThreads, 0, stream_>>>(uacc_, V_, 0.0);
+      k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(pacc_, P_, 0.0);
+    }
     HIP_CHECK(hipMemsetAsync(iters_total_, 0, sizeof(int) * nseg_, stream_));
     const unsigned gb = (unsigned)((nseg_ + 63) / 64);
     std::vector<double> h(4 * (size_t)nseg_);
@@ -676,6 +726,14 @@ class BatchSolver {
                                                                           pstride_, nullptr);
     copy(uacc_, vel, V_);
     copy(pacc_, pres, P_);
+    if (slots > 0) {  // record this solve for the later same-stage starts
+      const int slot = (int)(hist_count_ % slots);
+      copy(uacc_, hist_u_ + (int64_t)slot * V_, V_);
+      copy(pacc_, hist_p_ + (int64_t)slot * P_, P_);
+      hist_shift_[slot] = shift;
+      ++hist_count_;
+    }
+    out.warm = steps > 0;
     out.iterations.resize(nseg_);
     HIP_CHECK(hipMemcpyAsync(out.iterations.data(), iters_total_, sizeof(int) * nseg_, hipMemcpyDeviceToHost,
                              stream_));
@@ -1132,6 +1190,10 @@ class BatchSolver {
   int *velb_beg_[4] = {}, *preb_beg_[4] = {};
   GemmTable velb_tab_[4] = {}, preb_tab_[4] = {};
   bool preb_avec_ = false;
+  int hist_period_ = 0, hist_steps_ = 0, hist_slots_ = 0;
+  int64_t hist_count_ = 0, hist_committed_ = 0;
+  double *hist_u_ = nullptr, *hist_p_ = nullptr;
+  std::vector<double> hist_shift_;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
 };
@@ -1218,6 +1280,24 @@ int32_t mac_batch_layout(void* h, int64_t* velocity_offsets, int64_t* pressure_o
   }
   velocity_offsets[n] = s->total_faces();
   pressure_offsets[n] = s->total_cells();
+  return MAC_OK;
+}
+
+int32_t mac_batch_set_history(void* h, int32_t period, int32_t steps) {
+  BatchSolver* s = static_cast<BatchSolver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  return macs::batch_guarded(s, [&] {
+    if (!s->mixed() || period < 0 || period > 16 || (period > 0 && (steps < 1 || steps > macs::kMaxExtrapolation)))
+      throw macs::ArgFailure("Require the mixed form, 0 <= period <= 16 and 1 <= steps <= 6");
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    s->set_history(period, period > 0 ? steps : 0);
+  });
+}
+
+int32_t mac_batch_history_mark(void* h, int32_t rollback) {
+  BatchSolver* s = static_cast<BatchSolver*>(h);
+  if (!s) return MAC_INVALID_ARGUMENT;
+  s->history_mark(rollback != 0);
   return MAC_OK;
 }
 

@@ -76,4 +76,36 @@ inline std::vector<double> eigenvalues_1d(int n, double h, int kind) {
 
 inline int round8(int n) { return (n + 7) & ~7; }
 
+// Weights w[0..steps-1] of the last `steps` same-stage solves (newest first) in the
+// Lagrange extrapolation to the new solve's time.  h[0] is the new solve's step size and
+// h[k] that of the solve k steps back (each known up to a common factor: a stage's
+// viscosity / mass is nu a_ii dt); consecutive same-stage solves are taken (h[k-1] +
+// h[k]) / 2 apart.  Equal or unknown steps give (-1)^(k+1) C(steps, k).
+inline void history_weights(const double* h, int steps, double* w) {
+  bool equal = true, known = std::isfinite(h[0]) && h[0] > 0.0;
+  for (int k = 1; k <= steps; ++k) {
+    equal = equal && h[k] == h[0];
+    known = known && std::isfinite(h[k]) && h[k] > 0.0;
+  }
+  if (equal || !known) {
+    double c = 1.0;  // C(steps, k), built up k by k
+    for (int k = 1; k <= steps; ++k) {
+      c = c * (steps - k + 1) / k;
+      w[k - 1] = (k % 2 == 1) ? c : -c;
+    }
+    return;
+  }
+  double x[16] = {0.0};  // node times relative to the new solve (steps < 16)
+  for (int k = 1; k <= steps; ++k) x[k] = x[k - 1] - 0.5 * (h[k - 1] + h[k]);
+  for (int k = 1; k <= steps; ++k) {
+    double num = 1.0, den = 1.0;
+    for (int m = 1; m <= steps; ++m) {
+      if (m == k) continue;
+      num *= -x[m];
+      den *= x[k] - x[m];
+    }
+    w[k - 1] = num / den;
+  }
+}
+
 }  // namespace macs

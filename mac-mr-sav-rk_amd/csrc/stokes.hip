@@ -494,30 +494,7 @@ class Solver {
   void history_weights(int steps, Index index, double* w) const {
     double h[kMaxHistSteps + 1] = {shift_cur_};
     for (int k = 1; k <= steps; ++k) h[k] = hist_shift_[index(k)];
-    bool equal = true, known = std::isfinite(h[0]) && h[0] > 0.0;
-    for (int k = 1; k <= steps; ++k) {
-      equal = equal && h[k] == h[0];
-      known = known && std::isfinite(h[k]) && h[k] > 0.0;
-    }
-    if (equal || !known) {  // equal steps (or no step information): binomial weights
-      double c = 1.0;       // C(steps, k), built up k by k
-      for (int k = 1; k <= steps; ++k) {
-        c = c * (steps - k + 1) / k;
-        w[k - 1] = (k % 2 == 1) ? c : -c;
-      }
-      return;
-    }
-    double x[kMaxHistSteps + 1] = {0.0};  // node times relative to the new solve
-    for (int k = 1; k <= steps; ++k) x[k] = x[k - 1] - 0.5 * (h[k - 1] + h[k]);
-    for (int k = 1; k <= steps; ++k) {
-      double num = 1.0, den = 1.0;
-      for (int m = 1; m <= steps; ++m) {
-        if (m == k) continue;
-        num *= -x[m];
-        den *= x[k] - x[m];
-      }
-      w[k - 1] = num / den;
-    }
+    macs::history_weights(h, steps, w);
   }
   // Records the solve's pressures (and, for the mixed solve, velocities: its outer
   // iterate starts from both) for the warm starts, with the zero start's Schur

@@ -77,6 +77,8 @@ EXPORTED = (
     "mac_batch_layout",
     "mac_batch_set_launch",
     "mac_batch_last_sweeps",
+    "mac_batch_set_history",
+    "mac_batch_history_mark",
     "mac_stokes_solve_batched",
 )
 
@@ -235,6 +237,10 @@ def load() -> ctypes.CDLL:
     lib.mac_batch_set_launch.restype = _I32
     lib.mac_batch_last_sweeps.argtypes = [_P]
     lib.mac_batch_last_sweeps.restype = _I32
+    lib.mac_batch_set_history.argtypes = [_P, _I32, _I32]
+    lib.mac_batch_set_history.restype = _I32
+    lib.mac_batch_history_mark.argtypes = [_P, _I32]
+    lib.mac_batch_history_mark.restype = _I32
     lib.mac_stokes_solve_batched.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I32]
     lib.mac_stokes_solve_batched.restype = _I32
     _lib = lib

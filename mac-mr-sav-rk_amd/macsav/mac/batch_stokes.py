@@ -143,6 +143,21 @@ class GpuBatchedStokes:
                           "Invalid launch shape")
         self.launch = (int(use_graph), int(iters_per_check))
 
+    def set_history(self, period: int, steps: int = 3) -> None:
+        """Mixed form: start every batched solve from the extrapolation in time of the
+        same stage of the last `steps` steps (`period` solves per step; 0 switches it off).
+        Results are unchanged to the bar; only the outer sweeps' starting point moves."""
+        code = self._lib.mac_batch_set_history(self._h, int(period), int(steps))
+        _native.raise_for(code, self._lib.mac_batch_last_error(self._h).decode())
+
+    def commit_history(self) -> None:
+        """Keep the solves since the last mark (an accepted step)."""
+        _native.raise_for(self._lib.mac_batch_history_mark(self._h, 0), "history mark failed")
+
+    def discard_history(self) -> None:
+        """Forget the solves since the last mark (a rejected trial)."""
+        _native.raise_for(self._lib.mac_batch_history_mark(self._h, 1), "history mark failed")
+
     def solve_packed(self, rhs: Any, *, mass: Any, viscosity: Any, velocity: Any = None,
                      pressure: Any = None) -> dict:
         """Solve on flat buffers in the batch layout (torch device tensors are used in

@@ -80,6 +80,8 @@ def test_batch_null_handle_is_an_invalid_argument():
     lib = _native.load()
     assert lib.mac_batch_last_error(None) == b"null handle"
     assert lib.mac_batch_last_sweeps(None) == -1
+    assert lib.mac_batch_set_history(None, 2, 3) == _native.MAC_INVALID_ARGUMENT
+    assert lib.mac_batch_history_mark(None, 0) == _native.MAC_INVALID_ARGUMENT
     assert lib.mac_stokes_solve_batched(None, None, None, None, None, None, None, None, None, 0) == \
         _native.MAC_INVALID_ARGUMENT
     assert lib.mac_batch_set_launch(None, 1, 1) == _native.MAC_INVALID_ARGUMENT

@@ -187,3 +187,79 @@ def test_mixed_batch_rejects_a_non_finite_right_hand_side():
     mass, visc = _shifts()
     with pytest.raises(ValueError):
         batch.solve(rhs, mass=mass, viscosity=visc)
+
+
+def _step_sequence(grids, steps, seed):
+    """Right-hand sides of `steps` consecutive steps with two stages each: a fixed field
+    that drifts slowly from step to step, as a run's stage right-hand sides do."""
+    rng = np.random.default_rng(seed)
+    base = [_rhs(g, seed + i) for i, g in enumerate(grids)]
+    drift = [rng.normal(size=b.shape) for b in base]
+    return [[[b * (1.0 + 0.02 * k + 0.01 * stage) + 1e-3 * k * d for b, d in zip(base, drift)]
+             for stage in range(2)] for k in range(steps)]
+
+
+def test_mixed_batch_history_start_cuts_sweeps_and_keeps_results():
+    """Same-stage history (period 2, three steps): every solve meets the bar and matches
+    the oracle to 1e-9, and the warm-started solves need fewer outer sweeps than the same
+    solves from zero."""
+    from macsav import GpuBatchedStokes
+    grids = _grids()
+    mass, visc = _shifts()
+    seq = _step_sequence(grids, 5, 200)
+    cold = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9, precision="mixed")
+    warm = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9, precision="mixed")
+    warm.set_history(2, 3)
+    sweeps_cold, sweeps_warm = [], []
+    for k, step in enumerate(seq):
+        for rhs in step:
+            a = cold.solve(rhs, mass=mass, viscosity=visc)
+            sweeps_cold.append(cold.last_sweeps)
+            b = warm.solve(rhs, mass=mass, viscosity=visc)
+            sweeps_warm.append(warm.last_sweeps)
+            for i in range(len(grids)):
+                assert b[i].residual <= 1e-10 and b[i].divergence_inf <= 1e-10, (k, i)
+                assert _rel(b[i].velocity, a[i].velocity) <= 1e-9, (k, i)
+        warm.commit_history()
+    print(f"sweeps cold {sweeps_cold}, warm {sweeps_warm}", flush=True)
+    assert sweeps_warm[:2] == sweeps_cold[:2]          # nothing recorded yet
+    assert sum(sweeps_warm[4:]) < sum(sweeps_cold[4:])  # two steps or more recorded
+    g = grids[2]
+    orc = mo.TensorStokes(mo.Grid(g.nx, g.ny, g.lx, g.ly), tolerance=1e-9, krylov_tolerance=1e-14)
+    want = orc.solve(seq[-1][-1][2], mass=mass[2], viscosity=visc[2])
+    assert _rel(b[2].velocity, want.velocity) <= 1e-9 and _rel(b[2].pressure, want.pressure) <= 1e-9
+
+
+def test_mixed_batch_history_forgets_a_rejected_trial():
+    """Steps A (accepted), B (rejected) and C on one batch give C bitwise the fields of A
+    then C on another: the history discards B."""
+    from macsav import GpuBatchedStokes
+    grids = _grids()
+    mass, visc = _shifts()
+    seq = _step_sequence(grids, 4, 300)
+    rejected = _step_sequence(grids, 1, 301)[0]
+
+    def run(with_b):
+        batch = GpuBatchedStokes(grids, ncols=2, tolerance=1e-9, precision="mixed")
+        batch.set_history(2, 3)
+        for step in seq[:3]:
+            for rhs in step:
+                batch.solve(rhs, mass=mass, viscosity=visc)
+            batch.commit_history()
+        if with_b:
+            for rhs in rejected:
+                batch.solve(rhs, mass=mass, viscosity=visc)
+            batch.discard_history()
+        return [batch.solve(rhs, mass=mass, viscosity=visc) for rhs in seq[3]]
+    first, second = run(True), run(False)
+    for x, y in zip(first, second):
+        for i in range(len(grids)):
+            np.testing.assert_array_equal(x[i].velocity, y[i].velocity)
+            np.testing.assert_array_equal(x[i].pressure, y[i].pressure)
+
+
+def test_history_is_refused_for_the_fp64_batch():
+    from macsav import GpuBatchedStokes
+    batch = GpuBatchedStokes(_grids(), ncols=2, tolerance=1e-9)
+    with pytest.raises(ValueError):
+        batch.set_history(2, 3)
