@@ -483,6 +483,25 @@ def bench_cavities(args) -> dict:
     loop_s = time.perf_counter() - t0
     resume()
     loop = n * timed / loop_s
+    # CPU baseline: the oracle (scipy.fft + the scipy-cg loop, one FFT worker as the
+    # reference's default) on the cavities of the first timed stage in order, within a
+    # bounded budget.
+    import scipy.fft
+    from oracle import mac_oracle as mo
+    done, t0 = 0, time.perf_counter()
+    with scipy.fft.set_workers(1):
+        for i in range(n):
+            g = grids[i]
+            orc = mo.TensorStokes(mo.Grid(g.nx, g.ny, g.lx, g.ly), tolerance=1e-9)
+            f, m, v = stages[i][first]
+            orc.solve(f.cpu().numpy(), mass=m, viscosity=v)
+            done += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+    cpu_s = time.perf_counter() - t0
+    cpu = {"value": done / cpu_s, "unit": "cavity stage solves/s", "cores": 1, "kind": "port",
+           "sample": f"cavities 0..{done - 1} of stage {first} (oracle TensorStokes, scipy.fft workers=1, host "
+                     f"nproc={os.cpu_count()}), {cpu_s:.1f} s"}
     head = forms["mixed+history"]
     value = head["cavity_stage_solves_per_s"]
     return {"metric": "cavity stage solves/s, batched uneven lid-driven cavities", "value": value,
@@ -502,7 +521,7 @@ def bench_cavities(args) -> dict:
             "speedup_vs_loop_fp64": value / loop,
             "speedup_vs_batched_fp64": value / forms["fp64"]["cavity_stage_solves_per_s"],
             "speedup_vs_batched_mixed_zero_start": value / forms["mixed"]["cavity_stage_solves_per_s"],
-            "record_seconds": record_s}
+            "cpu_baseline": cpu, "record_seconds": record_s}
 
 
 def bench_run(args) -> dict:
