@@ -259,9 +259,13 @@ const char* mac_batch_create_error(void);
 void mac_batch_destroy(void* handle);
 const char* mac_batch_last_error(void* handle);
 int32_t mac_batch_layout(void* handle, int64_t* velocity_offsets, int64_t* pressure_offsets);
-/* Launch shape: replay one captured CG iteration as a hipGraph (use_graph), and queue
- * iters_per_check iterations between host reads of the convergence flags (1..16).  The
- * results do not depend on either. */
+/* Launch shape.  use_graph 0: direct launches; 1: one captured CG iteration replayed as a
+ * hipGraph; with either, iters_per_check (1..16) iterations are queued between host reads
+ * of the convergence flags.  use_graph 2 (mixed form only): the stage graph, one captured
+ * hipGraph launch per stage whose outer tests, inner set-ups, CG iterations and updates are
+ * switched by device flags, iters_per_check CG iterations per round (4 rounds per launch;
+ * a stage that needs more is launched again), one host read per launch.  The results do
+ * not depend on the shape. */
 int32_t mac_batch_set_launch(void* handle, int32_t use_graph, int32_t iters_per_check);
 /* Outer sweeps of the last batched solve (mixed form; 1 for FP64), -1 for a null handle. */
 int32_t mac_batch_last_sweeps(void* handle);

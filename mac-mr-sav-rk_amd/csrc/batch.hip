@@ -7,7 +7,10 @@
 // y dimension walks the segments.  A segment whose Schur-CG has converged is skipped on the
 // device (its GEMM tiles and vector blocks return at once), so the host only reads the
 // per-segment active flags once per iteration.  One CG iteration is a fixed launch sequence,
-// captured once into a hipGraph and replayed.
+// captured once into a hipGraph and replayed.  In the mixed form a whole stage can run as one
+// captured graph instead (the stage launch shape): the outer test, the inner solve's set-up,
+// the CG iterations and the update are switched on and off by device flags, and the host
+// reads one control block per graph launch.
 //
 // Per-segment reductions use the same block partition and order as the single-cavity
 // solver (reduce_blocks of the segment's own length), so every segment's iterates, CG
@@ -48,96 +51,118 @@ __device__ inline int64_t pad32_index(const Seg& S, int f) {
   return S.t32v + (int64_t)j * ((S.nx + 7) & ~7) + (e - j * S.nx);
 }
 
-enum Len : int { LEN_FACES = 0, LEN_CELLS = 1 };
+enum Len : int { LEN_FACES = 0, LEN_CELLS = 1, ITEMS_DT = 2 };
+
+// A launch whose `run` flag (device memory, may be null) reads 0 does nothing: the stage
+// graph switches its outer-loop steps with it.
+__device__ inline bool halted(const int* run) { return run != nullptr && *run == 0; }
+
+// Work items of a launch over the segments: item w is block `y` of segment `x` (the block
+// index within the segment's own partition, so reductions keep the single-cavity order).
+// A launch walks its items in a grid-stride loop, so its grid stays bounded whatever the
+// batch, and a launch whose segments are inactive costs one flag load per item.
+struct Items {
+  const int2* it;
+  int n;
+};
 
 // out = add + sign * D vel per segment (add optional); partial sums / max of the result.
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void k_bdiv(const double* __restrict__ vel, double* __restrict__ out,
-                                                   const Seg* segs, double sign, double* partials, int pstride,
-                                                   const CgState* st, const double* __restrict__ add) {
-  const int s = blockIdx.y;
-  if (st != nullptr && !st[s].active) return;
-  const Seg S = segs[s];
-  if ((int)blockIdx.x >= S.nbp) return;
+__global__ __launch_bounds__(kThreads) void k_bdiv(const Items W, const double* __restrict__ vel,
+                                                   double* __restrict__ out, const Seg* segs, double sign,
+                                                   double* partials, int pstride, const CgState* st,
+                                                   const double* __restrict__ add, const int* run) {
+  if (halted(run)) return;
   __shared__ double sh[4];
-  const double* u = vel + S.voff;
-  const double* v = u + S.nu;
-  double* o = out + S.poff;
-  const int nx = S.nx, ny = S.ny;
-  const SmallDiv dnx(nx);
-  double accum = 0.0;
-  for (int cell = blockIdx.x * kThreads + threadIdx.x; cell < S.np; cell += S.nbp * kThreads) {
-    const int j = dnx.quot(cell), i = cell - j * nx;
-    double value = 0.0;
-    if (i < nx - 1) value += u[j * (nx - 1) + i] / S.hx;
-    if (i > 0) value -= u[j * (nx - 1) + i - 1] / S.hx;
-    if (j < ny - 1) value += v[j * nx + i] / S.hy;
-    if (j > 0) value -= v[(j - 1) * nx + i] / S.hy;
-    value = sign * value;
-    if (add != nullptr) value = add[S.poff + cell] + value;
-    o[cell] = value;
-    if (MODE == RED_SUM) accum += value;
-    if (MODE == RED_MAXABS) accum = fmax(accum, absn(value));
-  }
-  if (MODE != RED_NONE) {
-    accum = block_reduce<MODE>(accum, sh);
-    if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
+  for (int w = blockIdx.x; w < W.n; w += gridDim.x) {
+    const int s = W.it[w].x, blk = W.it[w].y;
+    if (st != nullptr && !st[s].active) continue;
+    const Seg S = segs[s];
+    const double* u = vel + S.voff;
+    const double* v = u + S.nu;
+    double* o = out + S.poff;
+    const int nx = S.nx, ny = S.ny;
+    const SmallDiv dnx(nx);
+    double accum = 0.0;
+    for (int cell = blk * kThreads + threadIdx.x; cell < S.np; cell += S.nbp * kThreads) {
+      const int j = dnx.quot(cell), i = cell - j * nx;
+      double value = 0.0;
+      if (i < nx - 1) value += u[j * (nx - 1) + i] / S.hx;
+      if (i > 0) value -= u[j * (nx - 1) + i - 1] / S.hx;
+      if (j < ny - 1) value += v[j * nx + i] / S.hy;
+      if (j > 0) value -= v[(j - 1) * nx + i] / S.hy;
+      value = sign * value;
+      if (add != nullptr) value = add[S.poff + cell] + value;
+      o[cell] = value;
+      if (MODE == RED_SUM) accum += value;
+      if (MODE == RED_MAXABS) accum = fmax(accum, absn(value));
+    }
+    if (MODE != RED_NONE) {
+      accum = block_reduce<MODE>(accum, sh);
+      if (threadIdx.x == 0) partials[(int64_t)s * pstride + blk] = accum;
+      __syncthreads();  // sh is reused by the next item
+    }
   }
 }
 
 // out = add + D^T p per segment (packed velocity layout; add optional), or (out32) D^T p
-// in the FP32 transform-input layout.
-__global__ __launch_bounds__(kThreads) void k_bdt(const double* __restrict__ p, const double* __restrict__ add,
-                                                  double* __restrict__ out, const Seg* segs, const CgState* st,
-                                                  float* __restrict__ out32) {
-  const int s = blockIdx.y;
-  if (st != nullptr && !st[s].active) return;
-  const Seg S = segs[s];
-  const int f = blockIdx.x * kThreads + threadIdx.x;
-  if (f >= S.size) return;
-  const double* pc = p + S.poff;
-  const int nx = S.nx;
-  double value;
-  if (f < S.nu) {
-    const int j = SmallDiv(nx - 1).quot(f), i = f - j * (nx - 1);
-    const int left = j * nx + i;
-    value = (pc[left] - pc[left + 1]) / S.hx;
-  } else {
-    const int e = f - S.nu;
-    const int j = SmallDiv(nx).quot(e), i = e - j * nx;
-    const int lower = j * nx + i;
-    value = (pc[lower] - pc[lower + nx]) / S.hy;
+// in the FP32 transform-input layout.  One thread per face.
+__global__ __launch_bounds__(kThreads) void k_bdt(const Items W, const double* __restrict__ p,
+                                                  const double* __restrict__ add, double* __restrict__ out,
+                                                  const Seg* segs, const CgState* st, float* __restrict__ out32) {
+  for (int w = blockIdx.x; w < W.n; w += gridDim.x) {
+    const int s = W.it[w].x, blk = W.it[w].y;
+    if (st != nullptr && !st[s].active) continue;
+    const Seg S = segs[s];
+    const int f = blk * kThreads + threadIdx.x;
+    if (f >= S.size) continue;
+    const double* pc = p + S.poff;
+    const int nx = S.nx;
+    double value;
+    if (f < S.nu) {
+      const int j = SmallDiv(nx - 1).quot(f), i = f - j * (nx - 1);
+      const int left = j * nx + i;
+      value = (pc[left] - pc[left + 1]) / S.hx;
+    } else {
+      const int e = f - S.nu;
+      const int j = SmallDiv(nx).quot(e), i = e - j * nx;
+      const int lower = j * nx + i;
+      value = (pc[lower] - pc[lower + nx]) / S.hy;
+    }
+    if (add != nullptr) value = add[S.voff + f] + value;
+    if (out32 != nullptr) out32[pad32_index(S, f)] = (float)value;
+    else out[S.voff + f] = value;
   }
-  if (add != nullptr) value = add[S.voff + f] + value;
-  if (out32 != nullptr) out32[pad32_index(S, f)] = (float)value;
-  else out[S.voff + f] = value;
 }
 
 template <int MODE, bool DOT>
-__global__ __launch_bounds__(kThreads) void k_breduce(const double* __restrict__ x, const double* __restrict__ y,
-                                                      const Seg* segs, int len, double* partials, int pstride,
-                                                      const CgState* st) {
-  const int s = blockIdx.y;
-  if (st != nullptr && !st[s].active) return;
-  const Seg S = segs[s];
-  const int nb = len == LEN_FACES ? S.nbv : S.nbp;
-  if ((int)blockIdx.x >= nb) return;
+__global__ __launch_bounds__(kThreads) void k_breduce(const Items W, const double* __restrict__ x,
+                                                      const double* __restrict__ y, const Seg* segs, int len,
+                                                      double* partials, int pstride, const CgState* st) {
   __shared__ double sh[4];
-  const int64_t off = len == LEN_FACES ? S.voff : S.poff;
-  const int n = len == LEN_FACES ? S.size : S.np;
-  double accum = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += nb * kThreads) {
-    if (MODE == RED_MAXABS) accum = fmax(accum, absn(x[off + i]));
-    else accum += DOT ? x[off + i] * y[off + i] : x[off + i];
+  for (int w = blockIdx.x; w < W.n; w += gridDim.x) {
+    const int s = W.it[w].x, blk = W.it[w].y;
+    if (st != nullptr && !st[s].active) continue;
+    const Seg S = segs[s];
+    const int nb = len == LEN_FACES ? S.nbv : S.nbp;
+    const int64_t off = len == LEN_FACES ? S.voff : S.poff;
+    const int n = len == LEN_FACES ? S.size : S.np;
+    double accum = 0.0;
+    for (int i = blk * kThreads + threadIdx.x; i < n; i += nb * kThreads) {
+      if (MODE == RED_MAXABS) accum = fmax(accum, absn(x[off + i]));
+      else accum += DOT ? x[off + i] * y[off + i] : x[off + i];
+    }
+    accum = block_reduce<MODE>(accum, sh);
+    if (threadIdx.x == 0) partials[(int64_t)s * pstride + blk] = accum;
+    __syncthreads();
   }
-  accum = block_reduce<MODE>(accum, sh);
-  if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
 }
 
 // out[s] = the segment's partials reduced in the order of the single-cavity finalize.
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_bfinalize(const double* partials, int pstride, const Seg* segs,
-                                                        int len, double* out) {
+                                                        int len, double* out, const int* run) {
+  if (halted(run)) return;
   const int s = blockIdx.x;
   const Seg S = segs[s];
   const int nb = len == LEN_FACES ? S.nbv : S.nbp;
@@ -153,77 +178,85 @@ __global__ __launch_bounds__(kThreads) void k_bfinalize(const double* partials, 
 
 // x -= sums[s]/np on the cells; optional partial sums of y*x afterwards.
 template <bool DOT>
-__global__ __launch_bounds__(kThreads) void k_bsub_mean_dot(double* x, const double* sums, const double* y,
-                                                            const Seg* segs, double* partials, int pstride,
-                                                            const CgState* st) {
-  const int s = blockIdx.y;
-  if (st != nullptr && !st[s].active) return;
-  const Seg S = segs[s];
-  if ((int)blockIdx.x >= S.nbp) return;
+__global__ __launch_bounds__(kThreads) void k_bsub_mean_dot(const Items W, double* x, const double* sums,
+                                                            const double* y, const Seg* segs, double* partials,
+                                                            int pstride, const CgState* st) {
   __shared__ double sh[4];
-  double* xc = x + S.poff;
-  const double* yc = DOT ? y + S.poff : nullptr;
-  const double mean = sums[s] / (double)S.np;
-  double accum = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads) {
-    const double v = xc[i] - mean;
-    xc[i] = v;
-    if (DOT) accum += yc[i] * v;
+  for (int w = blockIdx.x; w < W.n; w += gridDim.x) {
+    const int s = W.it[w].x, blk = W.it[w].y;
+    if (st != nullptr && !st[s].active) continue;
+    const Seg S = segs[s];
+    double* xc = x + S.poff;
+    const double* yc = DOT ? y + S.poff : nullptr;
+    const double mean = sums[s] / (double)S.np;
+    double accum = 0.0;
+    for (int i = blk * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads) {
+      const double v = xc[i] - mean;
+      xc[i] = v;
+      if (DOT) accum += yc[i] * v;
+    }
+    if (DOT) {
+      accum = block_reduce<RED_SUM>(accum, sh);
+      if (threadIdx.x == 0) partials[(int64_t)s * pstride + blk] = accum;
+      __syncthreads();
+    }
   }
-  if (DOT) {
+}
+
+__global__ __launch_bounds__(kThreads) void k_bupdate_p(const Items W, double* __restrict__ p,
+                                                        const double* __restrict__ z, const Seg* segs,
+                                                        const CgState* st) {
+  for (int w = blockIdx.x; w < W.n; w += gridDim.x) {
+    const int s = W.it[w].x, blk = W.it[w].y;
+    if (!st[s].active) continue;
+    const Seg S = segs[s];
+    const bool first = st[s].iters == 0;
+    const double beta = st[s].beta;
+    double* pc = p + S.poff;
+    const double* zc = z + S.poff;
+    for (int i = blk * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads)
+      pc[i] = first ? zc[i] : __dadd_rn(__dmul_rn(pc[i], beta), zc[i]);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_bupdate_xr(const Items W, double* __restrict__ x,
+                                                         double* __restrict__ r, const double* __restrict__ p,
+                                                         const double* __restrict__ q, const Seg* segs,
+                                                         const CgState* st, double* partials, int pstride) {
+  __shared__ double sh[4];
+  for (int w = blockIdx.x; w < W.n; w += gridDim.x) {
+    const int s = W.it[w].x, blk = W.it[w].y;
+    if (!st[s].active) continue;
+    const Seg S = segs[s];
+    const double alpha = st[s].alpha;
+    const int64_t off = S.poff;
+    double accum = 0.0;
+    for (int i = blk * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads) {
+      x[off + i] = __dadd_rn(x[off + i], __dmul_rn(alpha, p[off + i]));
+      const double rv = __dsub_rn(r[off + i], __dmul_rn(alpha, q[off + i]));
+      r[off + i] = rv;
+      accum += rv * rv;
+    }
     accum = block_reduce<RED_SUM>(accum, sh);
-    if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
+    if (threadIdx.x == 0) partials[(int64_t)s * pstride + blk] = accum;
+    __syncthreads();
   }
-}
-
-__global__ __launch_bounds__(kThreads) void k_bupdate_p(double* __restrict__ p, const double* __restrict__ z,
-                                                        const Seg* segs, const CgState* st) {
-  const int s = blockIdx.y;
-  if (!st[s].active) return;
-  const Seg S = segs[s];
-  if ((int)blockIdx.x >= S.nbp) return;
-  const bool first = st[s].iters == 0;
-  const double beta = st[s].beta;
-  double* pc = p + S.poff;
-  const double* zc = z + S.poff;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads)
-    pc[i] = first ? zc[i] : __dadd_rn(__dmul_rn(pc[i], beta), zc[i]);
-}
-
-__global__ __launch_bounds__(kThreads) void k_bupdate_xr(double* __restrict__ x, double* __restrict__ r,
-                                                         const double* __restrict__ p, const double* __restrict__ q,
-                                                         const Seg* segs, const CgState* st, double* partials,
-                                                         int pstride) {
-  const int s = blockIdx.y;
-  if (!st[s].active) return;
-  const Seg S = segs[s];
-  if ((int)blockIdx.x >= S.nbp) return;
-  __shared__ double sh[4];
-  const double alpha = st[s].alpha;
-  const int64_t off = S.poff;
-  double accum = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.np; i += S.nbp * kThreads) {
-    x[off + i] = __dadd_rn(x[off + i], __dmul_rn(alpha, p[off + i]));
-    const double rv = __dsub_rn(r[off + i], __dmul_rn(alpha, q[off + i]));
-    r[off + i] = rv;
-    accum += rv * rv;
-  }
-  accum = block_reduce<RED_SUM>(accum, sh);
-  if (threadIdx.x == 0) partials[(int64_t)s * pstride + blockIdx.x] = accum;
 }
 
 // Per segment: max|mass u + visc K u - D^T p - f|, max|f|, max|u| (partials at q * nseg * pstride);
 // WRITE: the momentum residual r_u = f - (mass u + visc K u - D^T p) as well, in FP32 in the
 // transform-input layout (the split transform keeps 16 of its bits).
 template <bool WRITE>
-__global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__ vel, const double* __restrict__ f,
-                                                      const double* __restrict__ p, const Seg* segs,
-                                                      const double* __restrict__ mass_visc, double* partials,
-                                                      int pstride, float* __restrict__ ru32) {
-  const int s = blockIdx.y;
-  const Seg S = segs[s];
-  if ((int)blockIdx.x >= S.nbv) return;
+__global__ __launch_bounds__(kThreads) void k_bdefect(const Items W, const double* __restrict__ vel,
+                                                      const double* __restrict__ f, const double* __restrict__ p,
+                                                      const Seg* segs, const double* __restrict__ mass_visc,
+                                                      double* partials, int pstride, int nseg,
+                                                      float* __restrict__ ru32, const int* run) {
+  if (halted(run)) return;
   __shared__ double sh[4];
+  for (int item = blockIdx.x; item < W.n; item += gridDim.x) {
+  const int s = W.it[item].x, blk = W.it[item].y;
+  const Seg S = segs[s];
   const int nx = S.nx, ny = S.ny, nu = S.nu;
   const double hx2 = S.hx * S.hx, hy2 = S.hy * S.hy;
   const double ox = -1.0 / hx2, oy = -1.0 / hy2;
@@ -233,7 +266,7 @@ __global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__
   const double* pc = p + S.poff;
   const SmallDiv dw(nx - 1), dnx(nx);
   double md = 0.0, mf = 0.0, mv = 0.0;
-  for (int fidx = blockIdx.x * kThreads + threadIdx.x; fidx < S.size; fidx += S.nbv * kThreads) {
+  for (int fidx = blk * kThreads + threadIdx.x; fidx < S.size; fidx += S.nbv * kThreads) {
     FaceRow k;
     double lift;
     if (fidx < nu) {
@@ -275,19 +308,23 @@ __global__ __launch_bounds__(kThreads) void k_bdefect(const double* __restrict__
   __syncthreads();
   mv = block_reduce<RED_MAXABS>(mv, sh);
   if (threadIdx.x == 0) {
-    const int64_t set = (int64_t)gridDim.y * pstride, b = (int64_t)s * pstride + blockIdx.x;
+    const int64_t set = (int64_t)nseg * pstride, b = (int64_t)s * pstride + blk;
     partials[b] = md;
     partials[set + b] = mf;
     partials[2 * set + b] = mv;
   }
+  __syncthreads();
+  }
 }
 
 // dst = src over n elements (kernel form, so it can sit inside a captured graph).
-__global__ void k_bcopy(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+__global__ void k_bcopy(const double* __restrict__ src, double* __restrict__ dst, int64_t n, const int* run) {
+  if (halted(run)) return;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
     dst[i] = src[i];
 }
-__global__ void k_bfill(double* __restrict__ dst, int64_t n, double v) {
+__global__ void k_bfill(double* __restrict__ dst, int64_t n, double v, const int* run) {
+  if (halted(run)) return;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
     dst[i] = v;
 }
@@ -314,9 +351,10 @@ __global__ void k_bouter(const double* __restrict__ chk, int nseg, int ncols, do
 // the CG set-up, finished cavities stay inactive; GATE_POST: after the inner CG, its
 // iterations are added to the segment's total and the flag returns to the cavity's.
 enum Gate : int { GATE_SET = 0, GATE_MASK = 1, GATE_POST = 2 };
-__global__ void k_bgate(CgState* st, const int* __restrict__ active, int nseg, int mode, int* iters_total) {
+__global__ void k_bgate(CgState* st, const int* __restrict__ active, int nseg, int mode, int* iters_total,
+                        const int* run) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nseg) return;
+  if (s >= nseg || halted(run)) return;
   if (mode == GATE_MASK) {
     if (!active[s]) {
       st[s].active = 0;
@@ -330,27 +368,183 @@ __global__ void k_bgate(CgState* st, const int* __restrict__ active, int nseg, i
 
 // du += alpha L~^-1 D^T p (vout) on the active segments: the inner CG builds the velocity
 // correction du = W + sum_k alpha_k L~^-1 D^T p_k = L~^-1 (r_u + D^T x) as it goes.
-__global__ __launch_bounds__(kThreads) void k_bacc_du(double* __restrict__ du, const double* __restrict__ vout,
-                                                      const Seg* segs, const CgState* st) {
-  const int s = blockIdx.y;
-  if (!st[s].active) return;
-  const Seg S = segs[s];
-  const double alpha = st[s].alpha;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.size; i += gridDim.x * kThreads)
-    du[S.voff + i] = fma(alpha, vout[S.voff + i], du[S.voff + i]);
+__global__ __launch_bounds__(kThreads) void k_bacc_du(const Items W, double* __restrict__ du,
+                                                      const double* __restrict__ vout, const Seg* segs,
+                                                      const CgState* st) {
+  for (int w = blockIdx.x; w < W.n; w += gridDim.x) {
+    const int s = W.it[w].x, blk = W.it[w].y;
+    if (!st[s].active) continue;
+    const Seg S = segs[s];
+    const double alpha = st[s].alpha;
+    for (int i = blk * kThreads + threadIdx.x; i < S.size; i += S.nbv * kThreads)
+      du[S.voff + i] = fma(alpha, vout[S.voff + i], du[S.voff + i]);
+  }
 }
 
 // u += du, p += dp on the segments whose state is active.
-__global__ __launch_bounds__(kThreads) void k_baccum(double* __restrict__ u, const double* __restrict__ du,
-                                                     double* __restrict__ p, const double* __restrict__ dp,
-                                                     const Seg* segs, const CgState* st) {
-  const int s = blockIdx.y;
-  if (!st[s].active) return;
-  const Seg S = segs[s];
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.size; i += gridDim.x * kThreads)
-    u[S.voff + i] += du[S.voff + i];
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < S.np; i += gridDim.x * kThreads)
-    p[S.poff + i] += dp[S.poff + i];
+__global__ __launch_bounds__(kThreads) void k_baccum(const Items W, double* __restrict__ u,
+                                                     const double* __restrict__ du, double* __restrict__ p,
+                                                     const double* __restrict__ dp, const Seg* segs,
+                                                     const CgState* st) {
+  for (int w = blockIdx.x; w < W.n; w += gridDim.x) {
+    const int s = W.it[w].x, blk = W.it[w].y;
+    if (!st[s].active) continue;
+    const Seg S = segs[s];
+    for (int i = blk * kThreads + threadIdx.x; i < S.size; i += S.nbv * kThreads) u[S.voff + i] += du[S.voff + i];
+    for (int i = blk * kThreads + threadIdx.x; i < S.np; i += S.nbv * kThreads) p[S.poff + i] += dp[S.poff + i];
+  }
+}
+
+// ---- stage launch shape (mixed form): a whole stage as one captured graph
+// The host-driven sweep loop's decisions, made on the device.  A graph round runs the outer
+// test, the inner solve's set-up, a fixed number of CG iterations and the fold; the flags
+// below switch each part on for the segments and rounds where the host-driven loop would
+// run it, so every segment sees the same operations in the same order.
+struct StageCtl {
+  int halt;          // the stage is finished: no cavity above the bar, or the sweep cap
+  int in_cg;         // the running inner solve's CG is iterating
+  int fold_pending;  // the running inner solve still has to be folded into u and p
+  int run_outer;     // this round makes the outer test
+  int go_setup;      // this round starts an inner solve
+  int go_fold;       // this round folds the finished inner solve in
+  int sweeps;        // inner solves started
+  int cg_iters;      // CG iterations of the running inner solve
+  int max_cg;        // the most CG iterations any inner solve of the stage has taken
+};
+
+__global__ void k_stage_init(StageCtl* c) { *c = StageCtl{}; }
+
+__global__ void k_stage_round_begin(StageCtl* c) {
+  c->run_outer = (!c->halt && !c->in_cg && !c->fold_pending) ? 1 : 0;
+  c->go_setup = 0;
+  c->go_fold = 0;
+}
+
+// k_bouter, and a segment whose own max|defect| or max|D u| is not finite stops (the
+// host-driven loop does the same once it has read the maxima).
+__global__ void k_bouter_stage(const double* __restrict__ chk, int nseg, int ncols, double tol,
+                               int* __restrict__ active, const int* run) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg || halted(run)) return;
+  const int first = (s / ncols) * ncols;
+  double d = 0.0, f = 0.0, dv = 0.0;
+  for (int c = first; c < first + ncols; ++c) {
+    d = fmax(d, chk[c]);
+    f = fmax(f, chk[nseg + c]);
+    dv = fmax(dv, chk[3 * nseg + c]);
+  }
+  const bool finite = isfinite(chk[s]) && isfinite(chk[3 * nseg + s]);
+  active[s] = (d / (1.0 + f) <= tol && dv <= tol) ? 0 : (finite ? 1 : 0);
+}
+
+// After the outer test: the stage is finished when no segment is active or max_sweeps
+// inner solves have run; otherwise an inner solve starts.
+__global__ void k_stage_after_outer(StageCtl* c, const int* __restrict__ active, int nseg, int max_sweeps) {
+  const int run = c->run_outer;
+  int any = 0;
+  if (run)
+    for (int s = threadIdx.x; s < nseg; s += blockDim.x) any |= active[s];
+  any = __syncthreads_or(any);
+  if (threadIdx.x != 0 || !run) return;
+  if (!any || c->sweeps >= max_sweeps) {
+    c->halt = 1;
+  } else {
+    c->go_setup = 1;
+    c->sweeps += 1;
+  }
+}
+
+// Segment CG states for the next part of a round.
+//   STAGE_SETUP: active = the cavity's outer flag when an inner solve starts, else 0 (the
+//     set-up skips every segment).
+//   STAGE_CG_START: after a set-up, run = the states' active flags (from the CG set-up
+//     and the mask of finished cavities), and the inner CG starts.
+//   STAGE_FOLD: once the CG has stopped, the iterations are added to the totals and
+//     active = the outer flag (the host loop's GATE_POST); otherwise active = 0 (the fold
+//     skips every segment).
+enum StageGate : int { STAGE_SETUP = 0, STAGE_CG_START = 1, STAGE_FOLD = 2 };
+__global__ void k_stage_gate(StageCtl* c, CgState* st, const int* __restrict__ outer, int* __restrict__ run,
+                             int* __restrict__ iters_total, int nseg, int mode) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode == STAGE_SETUP) {
+    if (s < nseg) st[s].active = c->go_setup ? outer[s] : 0;
+    return;
+  }
+  if (mode == STAGE_CG_START) {
+    if (!c->go_setup) return;
+    if (s < nseg) run[s] = st[s].active;
+    if (s == 0) {
+      c->in_cg = 1;
+      c->fold_pending = 1;
+      c->cg_iters = 0;
+    }
+    return;
+  }
+  const bool fold = c->fold_pending && !c->in_cg;
+  if (s < nseg) {
+    if (fold) {
+      iters_total[s] += st[s].iters;
+      st[s].active = outer[s];
+    } else {
+      st[s].active = 0;
+    }
+  }
+  if (s == 0) c->go_fold = fold ? 1 : 0;
+}
+
+// cg_init's states where `run` is set.
+__global__ void k_stage_cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int n,
+                                const int* run) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || halted(run)) return;
+  CgState s{};
+  s.bnorm = sqrt(bb[c]);
+  s.tol = rtol * s.bnorm;
+  s.rr = bb[c];
+  s.active = (bmax[c] != 0.0 && s.bnorm != 0.0) ? 1 : 0;
+  s.converged = s.active ? 0 : 1;
+  s.iters = 0;
+  st[c] = s;
+}
+
+// The inner CG's check at the top of an iteration (k_cg_check on the segments whose CG
+// iterates), with the host loop's cap: after max_iterations iterations the CG stops without
+// a check.  When no segment iterates any more, the inner CG has finished.  Outside an inner
+// CG every state stays inactive, so the iteration's launches skip every segment.  A round
+// ends with one more check (`counting` 0) and no iteration after it, so a CG that fits the
+// round finishes there; when it does not, the next round's first check repeats it on the
+// same residuals and counts the iteration that follows.
+__global__ void k_stage_cg_check(StageCtl* c, CgState* st, const double* __restrict__ rr, int* __restrict__ run,
+                                 int nseg, int max_iters, int counting) {
+  const int in_cg = c->in_cg;
+  const bool going = in_cg && c->cg_iters < max_iters;
+  int any = 0;
+  for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+    int a = going ? run[s] : 0;
+    if (a) {
+      st[s].rr = rr[s];
+      if (sqrt(rr[s]) < st[s].tol) {
+        a = 0;
+        st[s].converged = 1;
+      }
+    }
+    st[s].active = a;
+    if (in_cg) run[s] = a;
+    any |= a;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0 && in_cg) {
+    if (!any) {
+      c->in_cg = 0;
+      c->max_cg = max(c->max_cg, c->cg_iters);
+    } else if (counting) {
+      c->cg_iters += 1;
+    }
+  }
+}
+
+__global__ void k_stage_round_end(StageCtl* c) {
+  if (c->go_fold) c->fold_pending = 0;
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096); }
@@ -445,6 +639,16 @@ class BatchSolver {
     V_ = voff;
     P_ = poff;
     segs_ = static_cast<Seg*>(upload_raw(segs_host_.data(), sizeof(Seg) * nseg_));
+    for (int kind = 0; kind < 3; ++kind) {
+      std::vector<int2> it;
+      for (int q = 0; q < nseg_; ++q) {
+        const Seg& S = segs_host_[q];
+        const int nb = kind == LEN_FACES ? S.nbv : kind == LEN_CELLS ? S.nbp : (S.size + kThreads - 1) / kThreads;
+        for (int b = 0; b < nb; ++b) it.push_back(make_int2(q, b));
+      }
+      items_n_[kind] = (int)it.size();
+      items_[kind] = static_cast<const int2*>(upload_raw(it.data(), sizeof(int2) * it.size()));
+    }
     auto take = [&](size_t bytes) { return dev_alloc(bytes); };
     const size_t vb = sizeof(double) * V_, pb = sizeof(double) * P_;
     io_f_ = (double*)take(vb);
@@ -480,6 +684,11 @@ class BatchSolver {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
+    for (StageGraph& g : stage_graphs_) {
+      (void)hipGraphExecDestroy(g.exec);
+      (void)hipGraphDestroy(g.graph);
+    }
+    if (ctl_host_) (void)hipHostFree(ctl_host_);
     for (void* p : allocs_) (void)hipFree(p);
     if (hist_u_) (void)hipFree(hist_u_);
     if (hist_p_) (void)hipFree(hist_p_);
@@ -499,8 +708,12 @@ class BatchSolver {
   double* io_v() { return io_v_; }
   double* io_p() { return io_p_; }
   std::string error;
-  int use_graph = 1;        // replay one captured CG iteration instead of launching it
+  int use_graph = 1;        // 1: replay one captured CG iteration instead of launching it;
+                            // 2 (mixed form): the stage graph, one launch per stage
   int iters_per_check = 1;  // CG iterations queued between host reads of the flags
+  static constexpr int kStageRounds = 4;  // stage graph: most rounds in a first launch
+  int stage_iters_ = 4;     // stage graph: CG iterations per round (then the last stage's longest CG)
+  int stage_rounds_ = kStageRounds;  // rounds of a stage's first launch (then the last stage's sweeps + 1)
   int last_sweeps = 0;      // outer sweeps of the last solve (1 for the FP64 form)
 
   bool mixed() const { return mixed_; }
@@ -544,8 +757,8 @@ class BatchSolver {
   // f, vel, pres: device buffers in the batch layout.  mass, visc: per cavity.
   // Per-segment max|f| (non-finite values reported as inf).
   std::vector<double> rhs_max(const double* f) {
-    k_breduce<RED_MAXABS, false><<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(f, nullptr, segs_, LEN_FACES,
-                                                                                partials_, pstride_, nullptr);
+    k_breduce<RED_MAXABS, false><<<igrid(LEN_FACES), kThreads, 0, stream_>>>(
+        items(LEN_FACES), f, nullptr, segs_, LEN_FACES, partials_, pstride_, nullptr);
     bfinalize(RED_MAXABS, LEN_FACES, sums_);
     std::vector<double> m(nseg_);
     HIP_CHECK(hipMemcpyAsync(m.data(), sums_, sizeof(double) * nseg_, hipMemcpyDeviceToHost, stream_));
@@ -561,14 +774,14 @@ class BatchSolver {
     velocity_inverse(false);
     bdiv(vout_, b_, -1.0, RED_SUM, nullptr);
     bfinalize(RED_SUM, LEN_CELLS, sums_);
-    k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, sums_, b_, segs_, partials_, pstride_,
-                                                                         nullptr);
+    k_bsub_mean_dot<true><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), b_, sums_, b_, segs_, partials_, pstride_, nullptr);
     bfinalize(RED_SUM, LEN_CELLS, sums2_);
-    k_breduce<RED_MAXABS, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, nullptr, segs_, LEN_CELLS,
-                                                                                partials_, pstride_, nullptr);
+    k_breduce<RED_MAXABS, false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), b_, nullptr, segs_, LEN_CELLS, partials_, pstride_, nullptr);
     bfinalize(RED_MAXABS, LEN_CELLS, sums3_);
     cg_init(cg_, sums2_, sums3_, opts_.krylov_tolerance, nseg_, stream_);
-    k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(x_, P_, 0.0);
+    k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(x_, P_, 0.0, nullptr);
     copy(b_, r_, P_);
     copy(sums2_, rr_, nseg_);
     run_cg();
@@ -584,21 +797,21 @@ class BatchSolver {
     }
     if (!out.failed_cg.empty()) return out;
     // p = x - mean(x); u = L^-1 (f + D^T p).
-    k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, nullptr, segs_, LEN_CELLS,
-                                                                             partials_, pstride_, nullptr);
+    k_breduce<RED_SUM, false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), x_, nullptr, segs_, LEN_CELLS, partials_, pstride_, nullptr);
     bfinalize(RED_SUM, LEN_CELLS, sums_);
-    k_bsub_mean_dot<false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, sums_, nullptr, segs_, nullptr,
-                                                                          pstride_, nullptr);
-    k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(x_, f, vin_, segs_, nullptr, nullptr);
+    k_bsub_mean_dot<false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), x_, sums_, nullptr, segs_, nullptr, pstride_, nullptr);
+    k_bdt<<<igrid(ITEMS_DT), kThreads, 0, stream_>>>(items(ITEMS_DT), x_, f, vin_, segs_, nullptr, nullptr);
     velocity_inverse(false);
     copy(vout_, vel, V_);
     copy(x_, pres, P_);
     // Checks: residual and divergence per cavity (max over its columns).
-    k_bdefect<false><<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(vel, f, pres, segs_, mv_, partials_, pstride_,
-                                                                   nullptr);
+    k_bdefect<false><<<igrid(LEN_FACES), kThreads, 0, stream_>>>(
+        items(LEN_FACES), vel, f, pres, segs_, mv_, partials_, pstride_, nseg_, nullptr, nullptr);
     for (int q = 0; q < 3; ++q)
       k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_ + (int64_t)q * nseg_ * pstride_, pstride_,
-                                                              segs_, LEN_FACES, chk_ + q * nseg_);
+                                                              segs_, LEN_FACES, chk_ + q * nseg_, nullptr);
     bdiv(vel, q_, 1.0, RED_MAXABS, nullptr);
     bfinalize(RED_MAXABS, LEN_CELLS, chk_ + 3 * nseg_);
     std::vector<double> h(4 * (size_t)nseg_);
@@ -652,8 +865,8 @@ class BatchSolver {
       extrapolate(su, w, steps, uacc_, V_, stream_);
       extrapolate(sp, w, steps, pacc_, P_, stream_);
     } else {
-      k_bfill<<<blocks_for(V_), kThreads, 0, stream_>>>(uacc_, V_, 0.0);
-      k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(pacc_, P_, 0.0);
+      k_bfill<<<blocks_for(V_), kThreads, 0, stream_>>>(uacc_, V_, 0.0, nullptr);
+      k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(pacc_, P_, 0.0, nullptr);
     }
     HIP_CHECK(hipMemsetAsync(iters_total_, 0, sizeof(int) * nseg_, stream_));
     const unsigned gb = (unsigned)((nseg_ + 63) / 64);
@@ -661,14 +874,16 @@ class BatchSolver {
     std::vector<int> act(nseg_);
     Outcome out;
     int sweep = 0;
-    for (;; ++sweep) {
-      k_bdefect<true><<<seg_grid(LEN_FACES), kThreads, 0, stream_>>>(uacc_, f, pacc_, segs_, mv_, partials_, pstride_,
-                                                                     tf0_);
-      for (int q = 0; q < 3; ++q)
-        k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_ + (int64_t)q * nseg_ * pstride_, pstride_,
-                                                                segs_, LEN_FACES, chk_ + q * nseg_);
-      bdiv(uacc_, rc_, -1.0, RED_MAXABS, nullptr);
-      bfinalize(RED_MAXABS, LEN_CELLS, chk_ + 3 * nseg_);
+    if (use_graph == 2) {
+      stage_launches(f);
+      sweep = ctl_host_->sweeps;
+      HIP_CHECK(hipMemcpyAsync(h.data(), chk_, sizeof(double) * 4 * nseg_, hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      for (int q = 0; q < nseg_; ++q)
+        if (!std::isfinite(h[nseg_ + q])) throw ArgFailure("Invalid Stokes RHS");
+    }
+    for (; use_graph != 2; ++sweep) {
+      outer_defect(f, nullptr);
       k_bouter<<<gb, 64, 0, stream_>>>(chk_, nseg_, ncols_, opts_.refine_tolerance, outer_);
       HIP_CHECK(hipMemcpyAsync(h.data(), chk_, sizeof(double) * 4 * nseg_, hipMemcpyDeviceToHost, stream_));
       HIP_CHECK(hipMemcpyAsync(act.data(), outer_, sizeof(int) * nseg_, hipMemcpyDeviceToHost, stream_));
@@ -691,39 +906,19 @@ class BatchSolver {
       if (active == 0 || sweep >= opts_.max_sweeps) break;
       if (changed) HIP_CHECK(hipMemcpy(outer_, act.data(), sizeof(int) * nseg_, hipMemcpyHostToDevice));
       // Inner solve for (du, dp) on the unfinished cavities.
-      k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_SET, nullptr);
-      velocity_inverse(true);  // W = L~^-1 r_u (r_u in the FP32 transform input)
-      copy(vout_, du_, V_);    // du starts at W; the CG adds alpha_k L~^-1 D^T p_k
-      bdiv(vout_, b_, -1.0, RED_SUM, cg_, rc_);
-      bfinalize(RED_SUM, LEN_CELLS, sums_);
-      k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, sums_, b_, segs_, partials_, pstride_,
-                                                                           cg_);
-      bfinalize(RED_SUM, LEN_CELLS, sums2_);
-      k_breduce<RED_MAXABS, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(b_, nullptr, segs_, LEN_CELLS,
-                                                                                  partials_, pstride_, cg_);
-      bfinalize(RED_MAXABS, LEN_CELLS, sums3_);
-      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, nseg_, stream_);
-      k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_MASK, nullptr);
-      k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(x_, P_, 0.0);
-      copy(b_, r_, P_);
-      copy(sums2_, rr_, nseg_);
+      k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_SET, nullptr, nullptr);
+      inner_setup(nullptr);
       run_cg();
-      k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_POST, iters_total_);
-      // dp = x - mean(x) (D^T annihilates the mean, so du needs no correction).
-      k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, nullptr, segs_, LEN_CELLS,
-                                                                               partials_, pstride_, cg_);
-      bfinalize(RED_SUM, LEN_CELLS, sums_);
-      k_bsub_mean_dot<false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, sums_, nullptr, segs_, nullptr,
-                                                                            pstride_, cg_);
-      k_baccum<<<acc_grid(), kThreads, 0, stream_>>>(uacc_, du_, pacc_, x_, segs_, cg_);
+      k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_POST, iters_total_, nullptr);
+      inner_fold(nullptr);
     }
     out.sweeps = sweep;
     // p mean-zero; hand out u and p.
-    k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(pacc_, nullptr, segs_, LEN_CELLS,
-                                                                             partials_, pstride_, nullptr);
+    k_breduce<RED_SUM, false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), pacc_, nullptr, segs_, LEN_CELLS, partials_, pstride_, nullptr);
     bfinalize(RED_SUM, LEN_CELLS, sums_);
-    k_bsub_mean_dot<false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(pacc_, sums_, nullptr, segs_, nullptr,
-                                                                          pstride_, nullptr);
+    k_bsub_mean_dot<false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), pacc_, sums_, nullptr, segs_, nullptr, pstride_, nullptr);
     copy(uacc_, vel, V_);
     copy(pacc_, pres, P_);
     if (slots > 0) {  // record this solve for the later same-stage starts
@@ -754,6 +949,117 @@ class BatchSolver {
   }
 
  private:
+  // The outer test's FP64 defect of every segment: r_u = f - (mass u + visc K u + G p) into
+  // the FP32 transform input, the maxima of the defect, f and u, and max|D u| (rc = -D u)
+  // into chk_.
+  void outer_defect(const double* f, const int* run) {
+    k_bdefect<true><<<igrid(LEN_FACES), kThreads, 0, stream_>>>(
+        items(LEN_FACES), uacc_, f, pacc_, segs_, mv_, partials_, pstride_, nseg_, tf0_, run);
+    for (int q = 0; q < 3; ++q)
+      k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_ + (int64_t)q * nseg_ * pstride_, pstride_,
+                                                              segs_, LEN_FACES, chk_ + q * nseg_, run);
+    bdiv(uacc_, rc_, -1.0, RED_MAXABS, nullptr, nullptr, run);
+    bfinalize(RED_MAXABS, LEN_CELLS, chk_ + 3 * nseg_, run);
+  }
+
+  // The inner solve's set-up on the segments whose state is active: W = L~^-1 r_u (r_u in
+  // the FP32 transform input), du = W (the CG adds alpha_k L~^-1 D^T p_k), b = r_c - D W
+  // with its mean removed, the CG states from (b, b) and max|b| (finished cavities kept
+  // inactive), x = 0, r = b.
+  void inner_setup(const int* run) {
+    const unsigned gb = (unsigned)((nseg_ + 63) / 64);
+    velocity_inverse(true, run);
+    copy(vout_, du_, V_, run);
+    bdiv(vout_, b_, -1.0, RED_SUM, cg_, rc_, run);
+    bfinalize(RED_SUM, LEN_CELLS, sums_, run);
+    k_bsub_mean_dot<true><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), b_, sums_, b_, segs_, partials_, pstride_, cg_);
+    bfinalize(RED_SUM, LEN_CELLS, sums2_, run);
+    k_breduce<RED_MAXABS, false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), b_, nullptr, segs_, LEN_CELLS, partials_, pstride_, cg_);
+    bfinalize(RED_MAXABS, LEN_CELLS, sums3_, run);
+    if (run == nullptr) cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, nseg_, stream_);
+    else k_stage_cg_init<<<gb, 64, 0, stream_>>>(cg_, sums2_, sums3_, opts_.inner_tolerance, nseg_, run);
+    k_bgate<<<gb, 64, 0, stream_>>>(cg_, outer_, nseg_, GATE_MASK, nullptr, run);
+    k_bfill<<<blocks_for(P_), kThreads, 0, stream_>>>(x_, P_, 0.0, run);
+    copy(b_, r_, P_, run);
+    copy(sums2_, rr_, nseg_, run);
+  }
+
+  // After the inner CG, on the segments whose state is active: dp = x - mean(x) (D^T
+  // annihilates the mean, so du needs no correction), u += du, p += dp.
+  void inner_fold(const int* run) {
+    k_breduce<RED_SUM, false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), x_, nullptr, segs_, LEN_CELLS, partials_, pstride_, cg_);
+    bfinalize(RED_SUM, LEN_CELLS, sums_, run);
+    k_bsub_mean_dot<false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), x_, sums_, nullptr, segs_, nullptr, pstride_, cg_);
+    k_baccum<<<igrid(LEN_FACES), kThreads, 0, stream_>>>(items(LEN_FACES), uacc_, du_, pacc_, x_, segs_, cg_);
+  }
+
+  // ------------------------------------------------------------ stage graph
+  // One round: the outer test (segments whose last inner CG has been folded in), the
+  // inner solve's set-up when a cavity is unfinished, `iters` CG iterations, and the fold
+  // once every CG has stopped.  Each segment goes through the same operations in the
+  // same order as in the host-driven loop; a CG that needs more iterations than a round
+  // holds continues in the next round, whose outer steps are then switched off.
+  void stage_round(int iters) {
+    const unsigned gb = (unsigned)((nseg_ + 63) / 64);
+    k_stage_round_begin<<<1, 1, 0, stream_>>>(ctl_);
+    outer_defect(io_f_, &ctl_->run_outer);
+    k_bouter_stage<<<gb, 64, 0, stream_>>>(chk_, nseg_, ncols_, opts_.refine_tolerance, outer_, &ctl_->run_outer);
+    k_stage_after_outer<<<1, kThreads, 0, stream_>>>(ctl_, outer_, nseg_, opts_.max_sweeps);
+    k_stage_gate<<<gb, 64, 0, stream_>>>(ctl_, cg_, outer_, run_, iters_total_, nseg_, STAGE_SETUP);
+    inner_setup(&ctl_->go_setup);
+    k_stage_gate<<<gb, 64, 0, stream_>>>(ctl_, cg_, outer_, run_, iters_total_, nseg_, STAGE_CG_START);
+    cg_run_ = &ctl_->in_cg;
+    for (int k = 0; k < iters; ++k) {
+      k_stage_cg_check<<<1, kThreads, 0, stream_>>>(ctl_, cg_, rr_, run_, nseg_, opts_.max_iterations, 1);
+      iteration_body();
+    }
+    k_stage_cg_check<<<1, kThreads, 0, stream_>>>(ctl_, cg_, rr_, run_, nseg_, opts_.max_iterations, 0);
+    cg_run_ = nullptr;
+    k_stage_gate<<<gb, 64, 0, stream_>>>(ctl_, cg_, outer_, run_, iters_total_, nseg_, STAGE_FOLD);
+    inner_fold(&ctl_->go_fold);
+    k_stage_round_end<<<1, 1, 0, stream_>>>(ctl_);
+  }
+
+  // The captured graph of `rounds` rounds of `iters` CG iterations (captured once per shape).
+  hipGraphExec_t stage_graph(int rounds, int iters) {
+    for (const StageGraph& g : stage_graphs_)
+      if (g.rounds == rounds && g.iters == iters) return g.exec;
+    StageGraph g{rounds, iters, nullptr, nullptr};
+    HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < rounds; ++r) stage_round(iters);
+    HIP_CHECK(hipStreamEndCapture(stream_, &g.graph));
+    HIP_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    stage_graphs_.push_back(g);
+    return g.exec;
+  }
+
+  // The mixed solve's sweeps as stage-graph launches until the device reports the stage
+  // finished (every cavity at the bar or stopped, or the sweep cap).  The first launch
+  // holds as many rounds as the previous stage needed (its sweeps and the final test) of
+  // as many CG iterations as its longest inner solve, so a stage like the last one takes
+  // one launch; a longer one continues with one-round launches.  The shape changes the
+  // time only, never the results.
+  void stage_launches(const double* f) {
+    if (f != io_f_) copy(f, io_f_, V_);  // the graphs read the right-hand side from io_f_
+    k_stage_init<<<1, 1, 0, stream_>>>(ctl_);
+    // Every round advances a CG iteration or a sweep, so this many launches always reach
+    // the end; the device stops itself at the caps.
+    const int64_t cap = (int64_t)(opts_.max_sweeps + 1) * (opts_.max_iterations + 2) + 1;
+    for (int64_t launches = 0;; ++launches) {
+      if (launches >= cap) throw HipFailure("stage graph did not finish");
+      HIP_CHECK(hipGraphLaunch(stage_graph(launches == 0 ? stage_rounds_ : 1, stage_iters_), stream_));
+      HIP_CHECK(hipMemcpyAsync(ctl_host_, ctl_, sizeof(StageCtl), hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      if (ctl_host_->halt) break;
+    }
+    stage_rounds_ = std::min(kStageRounds, std::max(1, ctl_host_->sweeps + 1));
+    stage_iters_ = std::min(16, std::max(1, ctl_host_->max_cg));
+  }
+
   // Schur PCG, every segment at once; converged segments drop out on the device.  As in
   // scipy's loop, the convergence test runs only at the top of an iteration, so none
   // follows the last of max_iterations.
@@ -794,45 +1100,50 @@ class BatchSolver {
 
   void iteration_body() {
     // z = mass (D D^T)^+ r + visc r (per cavity), mean removed; rho = (r, z)
-    pressure_precondition();
-    k_breduce<RED_SUM, false><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(z_, nullptr, segs_, LEN_CELLS,
-                                                                             partials_, pstride_, cg_);
+    pressure_precondition(cg_run_);
+    k_breduce<RED_SUM, false><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), z_, nullptr, segs_, LEN_CELLS, partials_, pstride_, cg_);
     bfinalize(RED_SUM, LEN_CELLS, sums_);
-    k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(z_, sums_, r_, segs_, partials_, pstride_,
-                                                                         cg_);
+    k_bsub_mean_dot<true><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), z_, sums_, r_, segs_, partials_, pstride_, cg_);
     bfinalize(RED_SUM, LEN_CELLS, sums2_);
     cg_rho(cg_, sums2_, nseg_, stream_);
-    k_bupdate_p<<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(p_, z_, segs_, cg_);
+    k_bupdate_p<<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(items(LEN_CELLS), p_, z_, segs_, cg_);
     // q = S p = D L^-1 D^T p, mean removed; alpha = rho / (p, q)
-    k_bdt<<<dt_grid(), kThreads, 0, stream_>>>(p_, nullptr, vin_, segs_, cg_, mixed_ ? tf0_ : nullptr);
-    velocity_inverse(true);
+    k_bdt<<<igrid(ITEMS_DT), kThreads, 0, stream_>>>(
+        items(ITEMS_DT), p_, nullptr, vin_, segs_, cg_, mixed_ ? tf0_ : nullptr);
+    velocity_inverse(true, cg_run_);
     bdiv(vout_, q_, 1.0, RED_SUM, cg_);
     bfinalize(RED_SUM, LEN_CELLS, sums_);
-    k_bsub_mean_dot<true><<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(q_, sums_, p_, segs_, partials_, pstride_,
-                                                                         cg_);
+    k_bsub_mean_dot<true><<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), q_, sums_, p_, segs_, partials_, pstride_, cg_);
     bfinalize(RED_SUM, LEN_CELLS, sums2_);
     cg_alpha(cg_, sums2_, nseg_, stream_);
-    if (mixed_) k_bacc_du<<<acc_grid(), kThreads, 0, stream_>>>(du_, vout_, segs_, cg_);
-    k_bupdate_xr<<<seg_grid(LEN_CELLS), kThreads, 0, stream_>>>(x_, r_, p_, q_, segs_, cg_, partials_, pstride_);
+    if (mixed_) k_bacc_du<<<igrid(LEN_FACES), kThreads, 0, stream_>>>(items(LEN_FACES), du_, vout_, segs_, cg_);
+    k_bupdate_xr<<<igrid(LEN_CELLS), kThreads, 0, stream_>>>(
+        items(LEN_CELLS), x_, r_, p_, q_, segs_, cg_, partials_, pstride_);
     bfinalize(RED_SUM, LEN_CELLS, rr_);
   }
 
   // ------------------------------------------------------------ transforms
   // Velocity: vout = L^-1 vin for every segment (per-cavity shifts in the tables); the
   // mixed form's tables run the split bf16 GEMM.
-  void velocity_inverse(bool use_state) {
+  void velocity_inverse(bool use_state, const int* run = nullptr) {
     for (int k = 0; k < 4; ++k) {
       GemmTable t = mixed_ ? velb_tab_[k] : vel_tab_[k];
       t.use_state = use_state ? 1 : 0;
+      t.run = run;
       if (mixed_) run_table(t, k == 0 ? DT_F32 : kPassTypes[k][0], kPassTypes[k][1], true);
       else gemm_f64_table(t, stream_);
     }
   }
   // Pressure: z = mass (D D^T)^+ r + visc r.
-  void pressure_precondition() {
+  void pressure_precondition(const int* run = nullptr) {
     for (int k = 0; k < 4; ++k) {
-      if (mixed_) run_table(preb_tab_[k], kPassTypes[k][0], kPassTypes[k][1], k == 0 ? preb_avec_ : true);
-      else gemm_f64_table(pre_tab_[k], stream_);
+      GemmTable t = mixed_ ? preb_tab_[k] : pre_tab_[k];
+      t.run = run;
+      if (mixed_) run_table(t, kPassTypes[k][0], kPassTypes[k][1], k == 0 ? preb_avec_ : true);
+      else gemm_f64_table(t, stream_);
     }
   }
   // A-operand / output types of the four passes in the mixed form: the input (FP64 for the
@@ -938,6 +1249,9 @@ class BatchSolver {
     rc_ = static_cast<double*>(dev_alloc(sizeof(double) * P_));
     outer_ = static_cast<int*>(dev_alloc(sizeof(int) * nseg_));
     iters_total_ = static_cast<int*>(dev_alloc(sizeof(int) * nseg_));
+    run_ = static_cast<int*>(dev_alloc(sizeof(int) * nseg_));
+    ctl_ = static_cast<StageCtl*>(dev_alloc(sizeof(StageCtl)));
+    HIP_CHECK(hipHostMalloc((void**)&ctl_host_, sizeof(StageCtl), hipHostMallocDefault));
     build_tables_bf16();
   }
 
@@ -1106,24 +1420,27 @@ class BatchSolver {
   }
 
   // ------------------------------------------------------------ helpers
-  dim3 seg_grid(int len) const { return dim3((unsigned)(len == LEN_FACES ? nbv_max_ : nbp_max_), (unsigned)nseg_); }
-  dim3 dt_grid() const { return dim3((unsigned)((maxfaces_ + kThreads - 1) / kThreads), (unsigned)nseg_); }
-  dim3 acc_grid() const {
-    return dim3((unsigned)std::min<int64_t>((maxfaces_ + kThreads - 1) / kThreads, 64), (unsigned)nseg_);
-  }
-  void bdiv(const double* vel, double* out, double sign, int mode, const CgState* st, const double* add = nullptr) {
-    const dim3 g = seg_grid(LEN_CELLS);
+  // Item lists (LEN_FACES: the segments' reduction blocks over faces, LEN_CELLS: over
+  // cells, ITEMS_DT: one thread per face) and the bounded grids that walk them.
+  Items items(int kind) const { return Items{items_[kind], items_n_[kind]}; }
+  unsigned igrid(int kind) const { return (unsigned)std::max(1, std::min(items_n_[kind], kMaxItemGrid)); }
+  void bdiv(const double* vel, double* out, double sign, int mode, const CgState* st, const double* add = nullptr,
+            const int* run = nullptr) {
+    const unsigned g = igrid(LEN_CELLS);
+    const Items w = items(LEN_CELLS);
     if (mode == RED_SUM)
-      k_bdiv<RED_SUM><<<g, kThreads, 0, stream_>>>(vel, out, segs_, sign, partials_, pstride_, st, add);
+      k_bdiv<RED_SUM><<<g, kThreads, 0, stream_>>>(w, vel, out, segs_, sign, partials_, pstride_, st, add, run);
     else
-      k_bdiv<RED_MAXABS><<<g, kThreads, 0, stream_>>>(vel, out, segs_, sign, partials_, pstride_, st, add);
+      k_bdiv<RED_MAXABS><<<g, kThreads, 0, stream_>>>(w, vel, out, segs_, sign, partials_, pstride_, st, add, run);
   }
-  void bfinalize(int mode, int len, double* out) {
-    if (mode == RED_SUM) k_bfinalize<RED_SUM><<<nseg_, kThreads, 0, stream_>>>(partials_, pstride_, segs_, len, out);
-    else k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_, pstride_, segs_, len, out);
+  void bfinalize(int mode, int len, double* out, const int* run = nullptr) {
+    if (mode == RED_SUM)
+      k_bfinalize<RED_SUM><<<nseg_, kThreads, 0, stream_>>>(partials_, pstride_, segs_, len, out, run);
+    else
+      k_bfinalize<RED_MAXABS><<<nseg_, kThreads, 0, stream_>>>(partials_, pstride_, segs_, len, out, run);
   }
-  void copy(const double* src, double* dst, int64_t n) {
-    k_bcopy<<<blocks_for(n), kThreads, 0, stream_>>>(src, dst, n);
+  void copy(const double* src, double* dst, int64_t n, const int* run = nullptr) {
+    k_bcopy<<<blocks_for(n), kThreads, 0, stream_>>>(src, dst, n, run);
   }
 
   const DevMatrix64* matrix(int n, int kind) {
@@ -1157,7 +1474,10 @@ class BatchSolver {
   }
 
   mac_stokes_opts opts_;
+  static constexpr int kMaxItemGrid = 4096;  // blocks of an item launch
   int ncols_ = 0, nseg_ = 0, pstride_ = 1, nbv_max_ = 1, nbp_max_ = 1;
+  const int2* items_[3] = {};
+  int items_n_[3] = {};
   int64_t V_ = 0, P_ = 0, maxfaces_ = 1;
   hipStream_t stream_ = nullptr;
   std::vector<Cavity> cav_;
@@ -1196,6 +1516,17 @@ class BatchSolver {
   std::vector<double> hist_shift_;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
+  // stage launch shape (mixed form)
+  struct StageGraph {
+    int rounds, iters;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  StageCtl* ctl_ = nullptr;       // device control block
+  StageCtl* ctl_host_ = nullptr;  // pinned copy read after each launch
+  int* run_ = nullptr;            // per segment: its inner CG is iterating
+  const int* cg_run_ = nullptr;   // while capturing a stage graph: the inner CG's flag
+  std::vector<StageGraph> stage_graphs_;
 };
 
 std::string g_batch_create_error;
@@ -1308,9 +1639,15 @@ int32_t mac_batch_last_sweeps(void* h) {
 
 int32_t mac_batch_set_launch(void* h, int32_t use_graph, int32_t iters_per_check) {
   BatchSolver* s = static_cast<BatchSolver*>(h);
-  if (!s || iters_per_check < 1 || iters_per_check > 16) return MAC_INVALID_ARGUMENT;
-  s->use_graph = use_graph ? 1 : 0;
-  s->iters_per_check = iters_per_check;
+  if (!s || iters_per_check < 1 || iters_per_check > 16 || use_graph < 0 || use_graph > 2 ||
+      (use_graph == 2 && !s->mixed()))
+    return MAC_INVALID_ARGUMENT;
+  s->use_graph = use_graph;
+  if (use_graph == 2) {
+    s->stage_iters_ = iters_per_check;
+    s->stage_rounds_ = BatchSolver::kStageRounds;
+  }
+  else s->iters_per_check = iters_per_check;
   return MAC_OK;
 }
 

@@ -153,6 +153,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmBatch batch) {
 // over the ascending tile_begin array; entries of converged columns are skipped.
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_f64_table(const GemmTable tb) {
+  if (tb.run != nullptr && *tb.run == 0) return;
   const int t = xcd_tile(blockIdx.x, tb.total_tiles);
   int lo = 0, hi = tb.count - 1;
   while (lo < hi) {
@@ -484,6 +485,7 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
 // inactive (a converged segment) return at once.
 template <int EPI, typename TA, typename TC, int AMODE>
 __global__ __launch_bounds__(256) void k_gemm_bf16_table(const GemmTable tb) {
+  if (tb.run != nullptr && *tb.run == 0) return;
   const int t = xcd_tile(blockIdx.x, tb.total_tiles);
   int lo = 0, hi = tb.count - 1;
   while (lo < hi) {

@@ -56,6 +56,7 @@ struct GemmTable {
   const GemmEntry* e;
   const int* begin;
   int count, total_tiles, epi, trans_c, use_state;
+  const int* run;  // device flag (may be null): a launch that reads 0 there does nothing
 };
 
 // Tile edge used by the launchers (entries are tiled by the launcher itself).

@@ -15,7 +15,14 @@ that have met the bar drop out of the remaining sweeps on the device.
 `solve(rhs_list, mass=..., viscosity=...)` returns one `StokesResult` per cavity; the
 zero-copy form `solve_packed` works on one flat device buffer in the batch layout given
 by `layout()`.  `autotune` times the launch shapes (graph replay or direct launches,
-CG iterations queued per host check) on a real right-hand side and keeps the fastest.
+CG iterations queued per host check; for the mixed form also the stage graph with its CG
+iterations per round) on a real right-hand side and keeps the fastest.
+
+Launch shapes `(use_graph, n)`: `(0, n)` direct launches and `(1, n)` one captured CG
+iteration replayed, with n iterations queued per host read of the convergence flags;
+`(2, n)` (mixed form, the default there) the stage graph: one captured hipGraph launch per
+stage, whose outer tests, inner set-ups, n CG iterations per round and updates are switched
+by device flags, with one host read per launch.  Results do not depend on the shape.
 """
 from __future__ import annotations
 
@@ -46,6 +53,7 @@ class GpuBatchedStokes:
     """Batched stage solve over cavities of uneven sizes on one GPU (FP64 or mixed form)."""
 
     LAUNCH_SHAPES = ((1, 1), (1, 2), (1, 3), (0, 1))
+    STAGE_SHAPES = ((2, 2), (2, 4), (2, 8))
 
     def __init__(self, grids: Sequence[Any], ncols: int = 2, tolerance: float = 1e-9,
                  krylov_tolerance: float = 1e-12, max_iterations: int = 100, device: int = 0,
@@ -93,6 +101,8 @@ class GpuBatchedStokes:
         self.velocity_offsets = np.frombuffer(vo, dtype=np.int64).copy()
         self.pressure_offsets = np.frombuffer(po, dtype=np.int64).copy()
         self.launch = (1, 1)
+        if precision == "mixed":
+            self.set_launch(2, 4)
         self.setup_seconds = perf_counter() - start
         self.solve_seconds = 0.0
         self.solves = 0
@@ -233,16 +243,26 @@ class GpuBatchedStokes:
             results.append(StokesResult(vel, pre, out["residual"][i], out["divergence_inf"][i]))
         return results
 
+    STAGE_PREFERENCE = 1.02  # a stage-graph shape within 2% of the fastest is kept
+
     def autotune(self, rhs: Any, *, mass: Any, viscosity: Any, repeats: int = 2) -> dict:
         """Time every launch shape on this right-hand side (packed layout) and keep the
-        fastest; returns {shape: seconds}."""
+        fastest, or, in the mixed form, the fastest stage-graph shape when it is within 2%
+        of that (one launch and one host read per stage); returns {shape: seconds}."""
         timings = {}
-        for shape in self.LAUNCH_SHAPES:
+        shapes = self.LAUNCH_SHAPES + (self.STAGE_SHAPES if self.precision == "mixed" else ())
+        for shape in shapes:
             self.set_launch(*shape)
             self.solve_packed(rhs, mass=mass, viscosity=viscosity)  # warm (graph capture)
             best = float("inf")
             for _ in range(repeats):
                 best = min(best, self.solve_packed(rhs, mass=mass, viscosity=viscosity)["seconds"])
             timings[shape] = best
-        self.set_launch(*min(timings, key=timings.get))
+        best = min(timings, key=timings.get)
+        stage = [k for k in timings if k[0] == 2]
+        if stage:
+            fastest_stage = min(stage, key=timings.get)
+            if timings[fastest_stage] <= self.STAGE_PREFERENCE * timings[best]:
+                best = fastest_stage
+        self.set_launch(*best)
         return timings

@@ -171,7 +171,10 @@ def test_mixed_batch_launch_shapes_give_identical_results_and_finished_cavities_
     assert first["iterations"][3] == [0, 0]
     assert all(sum(its) > 0 for k, its in enumerate(first["iterations"]) if k != 3)
     assert float(base_v[vo[6]:vo[8]].abs().max()) == 0.0 and float(base_p[po[6]:po[8]].abs().max()) == 0.0
-    for shape in [(1, 1), (1, 3), (0, 1), (0, 2)]:
+    assert batch.launch == (2, 4)  # the mixed form's default: one stage graph launch per stage
+    # Every launch shape, the stage graph with CG rounds shorter and longer than the inner
+    # solves included, gives the same results bitwise.
+    for shape in [(1, 1), (1, 3), (0, 1), (0, 2), (2, 1), (2, 3), (2, 16)]:
         batch.set_launch(*shape)
         again = batch.solve_packed(flat, mass=mass, viscosity=visc)
         assert torch.equal(again["velocity"], base_v) and torch.equal(again["pressure"], base_p), shape
