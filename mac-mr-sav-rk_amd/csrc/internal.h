@@ -196,6 +196,10 @@ void galerkin2_start(double* x, double* r, const double* g, const double* c, con
 // out = a - b column-wise (skips inactive columns when st is given).
 void sub_cols(const double* a, const double* b, double* out, int64_t n, int64_t stride, int ncols,
               const CgState* st, hipStream_t s);
+// dst_k[c * s_k + i] = src[c * n + i] - sums[c] / n (sums null: the plain copy) for the
+// non-null destinations, one pass over the contiguous columns of src.
+void finish_cols(const double* src, int64_t n, int ncols, const double* sums, double* d0, int64_t s0, double* d1,
+                 int64_t s1, double* d2, int64_t s2, hipStream_t s);
 void copy_cols(const double* src, int64_t src_stride, double* dst, int64_t dst_stride, int64_t n, int ncols,
                hipStream_t s);
 

@@ -359,6 +359,22 @@ __global__ void k_fill(double* x, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) x[i] = v;
 }
 
+// A solve's hand-out in one pass: v = src - sums[c] / n (sums null: v = src) written to up
+// to three destinations (null ones skipped), column c at c * stride of each.
+__global__ void k_finish_cols(const double* __restrict__ src, int64_t n, const double* __restrict__ sums,
+                              double* __restrict__ d0, int64_t s0, double* __restrict__ d1, int64_t s1,
+                              double* __restrict__ d2, int64_t s2) {
+  const int c = blockIdx.y;
+  const double mean = sums != nullptr ? sums[c] / (double)n : 0.0;
+  const double* sc = src + (int64_t)c * n;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double v = sums != nullptr ? sc[i] - mean : sc[i];
+    if (d0 != nullptr) d0[(int64_t)c * s0 + i] = v;
+    if (d1 != nullptr) d1[(int64_t)c * s1 + i] = v;
+    if (d2 != nullptr) d2[(int64_t)c * s2 + i] = v;
+  }
+}
+
 __global__ void k_copy_cols(const double* src, int64_t ss, double* dst, int64_t ds, int64_t n) {
   const int c = blockIdx.y;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
@@ -816,6 +832,10 @@ void fill(double* x, int64_t n, double v, hipStream_t s) {
   k_fill<<<reduce_blocks(n), kThreads, 0, s>>>(x, n, v);
 }
 
+void finish_cols(const double* src, int64_t n, int ncols, const double* sums, double* d0, int64_t s0, double* d1,
+                 int64_t s1, double* d2, int64_t s2, hipStream_t s) {
+  k_finish_cols<<<col_grid(n, ncols), kThreads, 0, s>>>(src, n, sums, d0, s0, d1, s1, d2, s2);
+}
 void copy_cols(const double* src, int64_t ss, double* dst, int64_t ds, int64_t n, int ncols, hipStream_t s) {
   k_copy_cols<<<col_grid(n, ncols), kThreads, 0, s>>>(src, ss, dst, ds, n);
 }

@@ -500,13 +500,25 @@ class Solver {
   // iterate starts from both) for the warm starts, with the zero start's Schur
   // right-hand side b when the solve formed it (the network's input).
   void remember_pressure(const double* p, int ncols, const double* u = nullptr, const double* b = nullptr) {
+    const Record r = record(ncols, u != nullptr, b);
     copy_cols(p, np_, pprev_, np_, np_, ncols, stream_);
+    if (r.p) copy_cols(p, np_, r.p, np_, np_, ncols, stream_);
+    if (r.u) copy_cols(u, size_, r.u, size_, size_, ncols, stream_);
+  }
+  // Where a solve's pressures and velocities go in the history ring (null: not recorded).
+  struct Record {
+    double* p;
+    double* u;
+  };
+  // The bookkeeping of recording a solve (b is copied here), without the copies of p and
+  // u: those go to pprev_ and the returned slots.
+  Record record(int ncols, bool with_u, const double* b) {
+    Record rec{nullptr, nullptr};
     prev_ncols_ = ncols;
     prev_b_ok_ = b != nullptr;
     if (b) copy_cols(b, np_, bprev_, np_, np_, ncols, stream_);
     if (history_period_ > 0) {
       const int slots = hist_ring_steps() * history_period_;
-      const bool with_u = u != nullptr;
       if (hist_ncols_ != ncols || hist_slots_ < slots || hist_has_u_ != with_u) {
         HIP_CHECK(hipStreamSynchronize(stream_));
         if (hist_) (void)hipFree(hist_);
@@ -525,11 +537,12 @@ class Solver {
       const int slot = (int)(hist_count_ % slots);
       hist_shift_[slot] = shift_cur_;
       hist_b_ok_[slot] = b != nullptr;
-      copy_cols(p, np_, hist_ + (int64_t)slot * np_ * ncols, np_, np_, ncols, stream_);
+      rec.p = hist_ + (int64_t)slot * np_ * ncols;
       if (b) copy_cols(b, np_, hist_b_ + (int64_t)slot * np_ * ncols, np_, np_, ncols, stream_);
-      if (with_u) copy_cols(u, size_, hist_u_ + (int64_t)slot * size_ * ncols, size_, size_, ncols, stream_);
+      if (with_u) rec.u = hist_u_ + (int64_t)slot * size_ * ncols;
       ++hist_count_;
     }
+    return rec;
   }
   // The history extrapolation of the velocities (mixed solve), into out; false when
   // there is none.
@@ -728,6 +741,16 @@ class Solver {
       defect_max(uacc_, fc, pacc_, g_, ncols, mass, visc, partials_, stream_, ru_buf, fin(sums_), rc_, -1.0,
                  fin(sums2_), ru_type, &lay_b_);
       post_values(sums_, 3, sums2_, ncols, mailbox_dev_, ++seq_, stream_);
+      // From a history start a sweep is all but certain (the extrapolation's defect lies
+      // above the bar), so the inner solve's first transforms are queued before the host
+      // reads the defect and run while it does; should the start already meet the bar,
+      // they are dropped (they write scratch only).
+      const bool ahead = sweep == 0 && u_set;
+      if (ahead) {
+        mixed_ = true;
+        velocity_inverse_bf16(ru_buf, ru_type, lay_b_, du_, DT_F32, size_, all, mass, visc);
+        mixed_ = false;
+      }
       const volatile Mailbox* mb = wait_post(seq_);
       out.residual = mb->vals[0] / (1.0 + mb->vals[1]);
       out.fmax = mb->vals[1];
@@ -746,7 +769,7 @@ class Solver {
         break;
       // Inner solve for the correction (dp, du).
       mixed_ = true;
-      velocity_inverse_bf16(ru_buf, ru_type, lay_b_, du_, DT_F32, size_, all, mass, visc);
+      if (!ahead) velocity_inverse_bf16(ru_buf, ru_type, lay_b_, du_, DT_F32, size_, all, mass, visc);
       divergence(DT_F32, du_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_,
                  fin(sums_));
       // b -= mean, r = b, x = 0, with (b, b) and max|b| for the start test.
@@ -768,12 +791,11 @@ class Solver {
       mixed_ = false;
       out.sweeps = sweep + 1;
     }
-    // p mean-zero; hand out u and p.
+    // p mean-zero; hand out u and p, and record them for the warm starts, in one pass each.
     reduce(pacc_, nullptr, np_, np_, ncols, RED_SUM, partials_, nullptr, stream_, fin(sums_));
-    sub_mean_dot(pacc_, np_, np_, ncols, sums_, nullptr, nullptr, nullptr, stream_);
-    copy_cols(pacc_, np_, pres, pres_stride, np_, ncols, stream_);
-    copy_cols(uacc_, size_, vel, vel_stride, size_, ncols, stream_);
-    remember_pressure(pacc_, ncols, uacc_, bzero_ok ? bzero_ : nullptr);
+    const Record rec = record(ncols, true, bzero_ok ? bzero_ : nullptr);
+    finish_cols(pacc_, np_, ncols, sums_, pres, pres_stride, pprev_, np_, rec.p, np_, stream_);
+    finish_cols(uacc_, size_, ncols, nullptr, vel, vel_stride, rec.u, size_, nullptr, 0, stream_);
     return out;
   }
 
