@@ -54,7 +54,9 @@ typedef struct mac_stokes_opts {
   int32_t device;          /* HIP device ordinal                                      */
   int32_t mixed_split;     /* MIXED: bit 0 -> velocity inverses, bit 1 -> pressure
                               preconditioner use the split bf16x3 GEMM (hi/lo parts
-                              of data and matrix, ~16-bit operands; default 3)      */
+                              of data and matrix, ~16-bit operands); bit 2 -> their
+                              forward passes run as even/odd pairs of half the
+                              contraction length; default 7                         */
   int32_t fp64_refine;     /* FP64: after the reference solve, up to this many FP64
                               defect-correction sweeps (inner Schur-CG rtol
                               inner_tolerance) until the residual and max|D u| are

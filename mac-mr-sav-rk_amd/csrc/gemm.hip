@@ -13,6 +13,7 @@
 #include <hip/hip_ext.h>
 
 #include <cstdint>
+#include <stdexcept>
 #include <type_traits>
 
 #include "internal.h"
@@ -238,8 +239,14 @@ __device__ inline void store4<double>(double* p, const f4& v) {
 // AMODE: how A reaches LDS.  0 = bf16 rows (16-byte loads of 8), 1 = fp32 rows with
 // lda % 4 == 0 and 16-byte aligned bases (float4 loads), 2 = one element per load,
 // 3 = fp64 rows with even lda and 16-byte aligned bases (two double2 loads per 4).
+// PAIRED (forward passes of the even/odd split): the A element at k is staged as
+// a[k] + s a[n - 1 - k] (n = E.pair_n, s = E.pair_sign = +1 or -1), combined in the input's
+// precision before the bf16 split.  Every transform here has Q[k', n-1-j] = (-1)^k' Q[k', j],
+// so the even (odd) rows of Q applied to the paired halves give the even (odd)
+// coefficients with half the contraction length.
+//
 // One BM x BN output tile (index lt of entry E, tile-rows visited in groups of GROUP).
-template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
+template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW, bool PAIRED = false>
 __device__ __forceinline__ void bf16_tile(const GemmEntry& E, const int lt, const int GROUP, const int trans_c) {
   using C = Cfg<BM, BN, SPLIT, NW>;
   constexpr int BK = C::BK, LDS = C::LDS, FM = C::FM, FN = C::FN, AE = C::AE, BE = C::BE, NT = C::NT;
@@ -269,13 +276,21 @@ __device__ __forceinline__ void bf16_tile(const GemmEntry& E, const int lt, cons
   // clamped to the last row: their products only reach output rows (columns) that the
   // epilogue discards, so no per-element guard is needed except on the k tail.
   const TA* a_ptr[NA];
+  const TA* a_mir[NA];  // PAIRED: the mirrored element of column c (k = 0), read downwards
   int a_col[NA];
 #pragma unroll
   for (int e = 0; e < NA; ++e) {
     const int v = e * NT + tid, r = v / APR, c = (v % APR) * AV;
     a_ptr[e] = static_cast<const TA*>(E.A) + (int64_t)min(m0 + r, M - 1) * E.lda + c;
+    a_mir[e] = PAIRED ? a_ptr[e] + (E.pair_n - 1 - 2 * c) : nullptr;
     a_col[e] = c;
   }
+  // PAIRED: value of A column (k0 + q) of staging group e, with its mirror combined.
+  const double psign = PAIRED ? (double)E.pair_sign : 0.0;
+  const float fsign = (float)psign;
+  auto paired = [&](int e, int k0, int q, double a) -> float {
+    return (float)(a + psign * (double)a_mir[e][-(k0 + q)]);
+  };
   const __bf16* b_ptr[C::PARTS][NB];
   int b_col[NB];
 #pragma unroll
@@ -315,14 +330,28 @@ __device__ __forceinline__ void bf16_tile(const GemmEntry& E, const int lt, cons
         st.rab[e] = *reinterpret_cast<const uint4*>(a_ptr[e] + k0);
       } else if constexpr (AMODE == 1) {
         const float4 f = *reinterpret_cast<const float4*>(a_ptr[e] + k0);
-        st.ra[e * 4 + 0] = f.x; st.ra[e * 4 + 1] = f.y; st.ra[e * 4 + 2] = f.z; st.ra[e * 4 + 3] = f.w;
+        if constexpr (PAIRED) {
+          // The mirrors one element each (their groups are not 16-byte aligned for every n,
+          // and a branch on the alignment would serialise the loop's loads).
+          const float* mp = reinterpret_cast<const float*>(a_mir[e]) - k0;
+          st.ra[e * 4 + 0] = f.x + fsign * mp[0]; st.ra[e * 4 + 1] = f.y + fsign * mp[-1];
+          st.ra[e * 4 + 2] = f.z + fsign * mp[-2]; st.ra[e * 4 + 3] = f.w + fsign * mp[-3];
+        } else {
+          st.ra[e * 4 + 0] = f.x; st.ra[e * 4 + 1] = f.y; st.ra[e * 4 + 2] = f.z; st.ra[e * 4 + 3] = f.w;
+        }
       } else if constexpr (AMODE == 3) {
         const double2 d0 = *reinterpret_cast<const double2*>(a_ptr[e] + k0);
         const double2 d1 = *reinterpret_cast<const double2*>(a_ptr[e] + k0 + 2);
-        st.ra[e * 4 + 0] = (float)d0.x; st.ra[e * 4 + 1] = (float)d0.y;
-        st.ra[e * 4 + 2] = (float)d1.x; st.ra[e * 4 + 3] = (float)d1.y;
+        if constexpr (PAIRED) {
+          st.ra[e * 4 + 0] = paired(e, k0, 0, d0.x); st.ra[e * 4 + 1] = paired(e, k0, 1, d0.y);
+          st.ra[e * 4 + 2] = paired(e, k0, 2, d1.x); st.ra[e * 4 + 3] = paired(e, k0, 3, d1.y);
+        } else {
+          st.ra[e * 4 + 0] = (float)d0.x; st.ra[e * 4 + 1] = (float)d0.y;
+          st.ra[e * 4 + 2] = (float)d1.x; st.ra[e * 4 + 3] = (float)d1.y;
+        }
       } else {
-        st.ra[e] = to_f(a_ptr[e][k0]);
+        if constexpr (PAIRED) st.ra[e] = paired(e, k0, 0, (double)a_ptr[e][k0]);
+        else st.ra[e] = to_f(a_ptr[e][k0]);
       }
     }
 #pragma unroll
@@ -342,8 +371,11 @@ __device__ __forceinline__ void bf16_tile(const GemmEntry& E, const int lt, cons
           st.rab[e] = load8_tail(a_ptr[e] + k0, K - (k0 + a_col[e]));
         } else {
 #pragma unroll
-          for (int q = 0; q < AV; ++q)
-            st.ra[e * AV + q] = k0 + a_col[e] + q < K ? to_f(a_ptr[e][k0 + q]) : 0.f;
+          for (int q = 0; q < AV; ++q) {
+            const bool in = k0 + a_col[e] + q < K;
+            if constexpr (PAIRED) st.ra[e * AV + q] = in ? paired(e, k0, q, (double)a_ptr[e][k0 + q]) : 0.f;
+            else st.ra[e * AV + q] = in ? to_f(a_ptr[e][k0 + q]) : 0.f;
+          }
         }
       }
 #pragma unroll
@@ -456,6 +488,45 @@ __device__ __forceinline__ void bf16_tile(const GemmEntry& E, const int lt, cons
   if (kt < nk) compute(0);
 
   // f32-result MFMA layout: register r of lane l holds row 4*(l/16) + r, column l%16.
+  if constexpr ((EPI == EPI_SHIFT || EPI == EPI_LAMBDA) && std::is_same<TC, float>::value) {
+    // The divide passes of the split form (FP32 out): eigenvalues read once per row and
+    // column, the sums and the divide in FP32 (far inside the split's 16-bit operands).
+    float eb[FN], ea[FM][4];
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int n = n0 + wn + j * 16 + li;
+      eb[j] = n < N ? (float)E.eb[n] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + i * 16 + 4 * lg + r;
+        ea[i][r] = m < M ? (float)E.ea[m] : 0.f;
+      }
+    const float s0 = (float)E.s0, s1 = (float)E.s1, th = (float)E.thresh;
+    float* Cp = static_cast<float*>(E.C);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int mb = m0 + wm + i * 16 + 4 * lg, n = n0 + wn + j * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = mb + r;
+          if (m >= M || n >= N) continue;
+          float v;
+          if constexpr (EPI == EPI_SHIFT) {
+            v = acc[i][j][r] / (s0 + s1 * (ea[i][r] + eb[j]));
+          } else {
+            const float lam = ea[i][r] + eb[j];
+            v = lam > th ? acc[i][j][r] / lam : 0.f;
+          }
+          Cp[trans_c ? (int64_t)n * E.ldc + m : (int64_t)m * E.ldc + n] = v;
+        }
+      }
+    return;
+  }
   const bool vec_t = EPI == EPI_STORE && trans_c && (E.ldc & 3) == 0;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -473,11 +544,11 @@ __device__ __forceinline__ void bf16_tile(const GemmEntry& E, const int lt, cons
 }
 
 
-template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
+template <int EPI, typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW, bool PAIRED = false>
 __global__ __launch_bounds__(64 * NW) void k_gemm_bf16(const GemmBatch batch) {
   const int t = xcd_tile(blockIdx.x, batch.total_tiles);
   const GemmEntry& E = batch.e[find_entry(batch, t)];
-  bf16_tile<EPI, TA, TC, SPLIT, BM, BN, AMODE, NW>(E, t - E.tile_begin, batch.group, batch.trans_c);
+  bf16_tile<EPI, TA, TC, SPLIT, BM, BN, AMODE, NW, PAIRED>(E, t - E.tile_begin, batch.group, batch.trans_c);
 }
 
 // Grouped split-bf16 launch over a device table (64x64 tiles, 4 waves): each block finds
@@ -525,9 +596,19 @@ void launch(void (*kernel)(Args...), int blocks, int threads, hipStream_t s, Arg
   }
 }
 
-template <typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW>
+template <typename TA, typename TC, bool SPLIT, int BM, int BN, int AMODE, int NW, bool PAIRED = false>
 void launch_bf16_mode(GemmBatch& b, int total, hipStream_t s) {
   constexpr int nt = 64 * NW;
+  if constexpr (PAIRED) {
+    // Forward passes only: plain store, velocity shift, pressure eigenvalue divide.
+    switch (b.epi) {
+      case EPI_STORE: launch(k_gemm_bf16<EPI_STORE, TA, TC, SPLIT, BM, BN, AMODE, NW, true>, total, nt, s, (const GemmBatch)b); break;
+      case EPI_SHIFT: launch(k_gemm_bf16<EPI_SHIFT, TA, TC, SPLIT, BM, BN, AMODE, NW, true>, total, nt, s, (const GemmBatch)b); break;
+      case EPI_LAMBDA: launch(k_gemm_bf16<EPI_LAMBDA, TA, TC, SPLIT, BM, BN, AMODE, NW, true>, total, nt, s, (const GemmBatch)b); break;
+      default: throw std::invalid_argument("a paired transform pass stores, shifts or divides");
+    }
+    return;
+  }
   switch (b.epi) {
     case EPI_STORE: launch(k_gemm_bf16<EPI_STORE, TA, TC, SPLIT, BM, BN, AMODE, NW>, total, nt, s, (const GemmBatch)b); break;
     case EPI_SHIFT: launch(k_gemm_bf16<EPI_SHIFT, TA, TC, SPLIT, BM, BN, AMODE, NW>, total, nt, s, (const GemmBatch)b); break;
@@ -537,7 +618,7 @@ void launch_bf16_mode(GemmBatch& b, int total, hipStream_t s) {
   }
 }
 
-template <typename TA, typename TC, bool SPLIT, int BM, int BN, int NW = 4>
+template <typename TA, typename TC, bool SPLIT, int BM, int BN, int NW = 4, bool PAIRED = false>
 void launch_bf16_tile(GemmBatch& b, hipStream_t s) {
   const int total = tile_batch(b, BM, BN);
   if constexpr (std::is_same<TA, __bf16>::value) {
@@ -546,14 +627,14 @@ void launch_bf16_tile(GemmBatch& b, hipStream_t s) {
     bool vec = true;
     for (int i = 0; i < b.count; ++i)
       vec = vec && (b.e[i].lda % 4 == 0) && (reinterpret_cast<uintptr_t>(b.e[i].A) % 16 == 0);
-    if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 1, NW>(b, total, s);
-    else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW>(b, total, s);
+    if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 1, NW, PAIRED>(b, total, s);
+    else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW, PAIRED>(b, total, s);
   } else {
     bool vec = true;
     for (int i = 0; i < b.count; ++i)
       vec = vec && (b.e[i].lda % 2 == 0) && (reinterpret_cast<uintptr_t>(b.e[i].A) % 16 == 0);
-    if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 3, NW>(b, total, s);
-    else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW>(b, total, s);
+    if (vec) launch_bf16_mode<TA, TC, SPLIT, BM, BN, 3, NW, PAIRED>(b, total, s);
+    else launch_bf16_mode<TA, TC, SPLIT, BM, BN, 2, NW, PAIRED>(b, total, s);
   }
 }
 
@@ -562,7 +643,7 @@ void launch_bf16_tile(GemmBatch& b, hipStream_t s) {
 int g_bf16_tile = 0;
 int g_bf16_group = 4;
 
-template <typename TA, typename TC, bool SPLIT>
+template <typename TA, typename TC, bool SPLIT, bool PAIRED = false>
 void launch_bf16(GemmBatch& b, hipStream_t s) {
   int tile = g_bf16_tile, group = g_bf16_group;
   // Measured at 1024^2 (two or four 1024^3 products per launch): 128x128 tiles with 8
@@ -580,13 +661,13 @@ void launch_bf16(GemmBatch& b, hipStream_t s) {
     group = tile == 64 ? 8 : 4;
   }
   if (b.group <= 0) b.group = group;
-  if (tile == 128) launch_bf16_tile<TA, TC, SPLIT, 128, 128>(b, s);
-  else if (tile == 1288) launch_bf16_tile<TA, TC, SPLIT, 128, 128, 8>(b, s);
+  if (tile == 128) launch_bf16_tile<TA, TC, SPLIT, 128, 128, 4, PAIRED>(b, s);
+  else if (tile == 1288) launch_bf16_tile<TA, TC, SPLIT, 128, 128, 8, PAIRED>(b, s);
   else if (tile == 641288 && !std::is_same<TA, __bf16>::value) {
-    if constexpr (!std::is_same<TA, __bf16>::value) launch_bf16_tile<TA, TC, SPLIT, 64, 128, 8>(b, s);
+    if constexpr (!std::is_same<TA, __bf16>::value) launch_bf16_tile<TA, TC, SPLIT, 64, 128, 8, PAIRED>(b, s);
   }
-  else if (tile == 12864) launch_bf16_tile<TA, TC, SPLIT, 128, 64>(b, s);
-  else launch_bf16_tile<TA, TC, SPLIT, 64, 64>(b, s);
+  else if (tile == 12864) launch_bf16_tile<TA, TC, SPLIT, 128, 64, 4, PAIRED>(b, s);
+  else launch_bf16_tile<TA, TC, SPLIT, 64, 64, 4, PAIRED>(b, s);
 }
 
 template <typename TA, bool SPLIT>
@@ -687,6 +768,13 @@ int gemm_bf16_table(const GemmTable& t, int a_type, int c_type, bool a_vec, hipS
 void gemm_bf16(GemmBatch& b, int a_type, int c_type, bool split, hipStream_t s) {
   if (b.count <= 0) return;
   set_scalars(b);
+  if (b.e[0].pair_sign != 0) {
+    // Paired forward passes: split operands, FP32 intermediates.
+    if (!split || c_type != DT_F32) throw std::invalid_argument("a paired pass is split and writes FP32");
+    if (a_type == DT_F64) launch_bf16<double, float, true, true>(b, s);
+    else launch_bf16<float, float, true, true>(b, s);
+    return;
+  }
   if (split) {
     // Split operands come from FP32/FP64 data (a bf16 input has no low part).
     if (a_type == DT_F64) launch_bf16_c<double, true>(b, c_type, s);

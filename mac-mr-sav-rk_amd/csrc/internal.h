@@ -36,6 +36,8 @@ struct GemmEntry {
   double s0, s1;       // EPI_SHIFT / EPI_COMBINE scalars (the launcher copies the batch's)
   double thresh;       // EPI_LAMBDA zero-mode threshold (likewise)
   const CgState* st;   // table launches with use_state: skip the entry if !st->active
+  int pair_n;          // paired forward pass (bf16 split): A's row length n, whose column k
+  int pair_sign;       // enters as a[k] + pair_sign * a[n - 1 - k]; pair_sign 0: not paired
 };
 
 constexpr int kMaxEntries = 8;

@@ -41,7 +41,21 @@ struct DevMatrix {
   __bf16* qtb = nullptr;
   __bf16* qb_lo = nullptr;   // bf16(q - qb): the split form's second term
   __bf16* qtb_lo = nullptr;
+  // Even/odd split of the mixed form's forward passes (Q[k, n-1-j] = (-1)^k Q[k, j]): the
+  // even rows Q[2r, j] and the odd rows Q[2r+1, j] for j < h = ceil(n/2), rows of ldh
+  // (for odd n the middle column of the even rows is halved, of the odd rows zero), hi/lo;
+  // and Q^T with its columns in that coefficient order (even k first), for the inverse
+  // passes.
+  int h = 0, ldh = 0;
+  __bf16 *qe = nullptr, *qe_lo = nullptr, *qo = nullptr, *qo_lo = nullptr;
+  __bf16 *qtp = nullptr, *qtp_lo = nullptr;
 };
+
+// Coefficient order of the even/odd split: stored position i holds coefficient k.
+inline int eo_coefficient(int i, int n) {
+  const int h = (n + 1) / 2;
+  return i < h ? 2 * i : 2 * (i - h) + 1;
+}
 
 // One 2-D block (u, v or pressure) of shape ny x nx with its y/x bases.
 struct Block {
@@ -50,6 +64,8 @@ struct Block {
   const DevMatrix* mx = nullptr;
   double* lam_y = nullptr;
   double* lam_x = nullptr;
+  double* lam_y_eo = nullptr;  // the same eigenvalues in the even/odd coefficient order
+  double* lam_x_eo = nullptr;
   std::vector<double> host_lam_y, host_lam_x;
 };
 
@@ -187,12 +203,14 @@ class Solver {
   void velocity_inverse_bf16(const void* in, int in_type, const VelLayout& Lin, void* out, int out_type,
                              int64_t out_stride, const std::vector<int>& cols, double mass, double visc) {
     const bool sp = split_velocity();
+    const bool eo = sp && even_odd();                // paired forward passes
     const int mid = sp ? DT_F32 : DT_BF16;           // intermediate storage type
     const size_t mid_bytes = sp ? 4 : 2;
     char* t1base = sp ? reinterpret_cast<char*>(tf1_) : reinterpret_cast<char*>(tb1_);
     char* t2base = sp ? reinterpret_cast<char*>(tf2_) : reinterpret_cast<char*>(tb2_);
-    for (size_t base = 0; base < cols.size(); base += kMaxEntries / 2) {
-      const size_t count = std::min<size_t>(kMaxEntries / 2, cols.size() - base);
+    const size_t per_launch = eo ? kMaxEntries / 4 : kMaxEntries / 2;  // columns per launch
+    for (size_t base = 0; base < cols.size(); base += per_launch) {
+      const size_t count = std::min<size_t>(per_launch, cols.size() - base);
       GemmBatch ga{}, gb{}, gc{}, gd{};
       const size_t in_bytes = in_type == DT_F64 ? 8 : (in_type == DT_F32 ? 4 : 2);
       const size_t out_bytes = out_type == DT_F64 ? 8 : 4;
@@ -210,6 +228,24 @@ class Solver {
           char* t2 = t2base + ((int64_t)c * tstride_ + t_off[bi]) * mid_bytes;
           char* dst = static_cast<char*>(out) + ((int64_t)c * out_stride + out_off[bi]) * out_bytes;
           const int ry = round8(B.ny), rx = round8(B.nx);
+          if (eo) {
+            // x-forward and y-forward as even/odd pairs (coefficients stored even k first;
+            // the eigenvalues and the inverse matrices follow that order).
+            const DevMatrix &X = *B.mx, &Y = *B.my;
+            add_pass_any(ga, src, in_ld[bi], X.qe, X.ldh, t1, ry, B.ny, X.h, X.h, nullptr, nullptr, X.qe_lo);
+            pair_last(ga, B.nx, 1);
+            add_pass_any(ga, src, in_ld[bi], X.qo, X.ldh, t1 + (int64_t)X.h * ry * mid_bytes, ry, B.ny, B.nx / 2, X.h,
+                         nullptr, nullptr, X.qo_lo);
+            pair_last(ga, B.nx, -1);
+            add_pass_any(gb, t1, ry, Y.qe, Y.ldh, t2, ry, B.nx, Y.h, Y.h, B.lam_x_eo, B.lam_y_eo, Y.qe_lo);
+            pair_last(gb, B.ny, 1);
+            add_pass_any(gb, t1, ry, Y.qo, Y.ldh, t2 + (int64_t)Y.h * mid_bytes, ry, B.nx, B.ny / 2, Y.h, B.lam_x_eo,
+                         B.lam_y_eo + Y.h, Y.qo_lo);
+            pair_last(gb, B.ny, -1);
+            add_pass_any(gc, t2, ry, Y.qtp, Y.ld8, t1, rx, B.nx, B.ny, B.ny, nullptr, nullptr, Y.qtp_lo);
+            add_pass_any(gd, t1, rx, X.qtp, X.ld8, dst, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr, X.qtp_lo);
+            continue;
+          }
           add_pass_any(ga, src, in_ld[bi], B.mx->qb, B.mx->ld8, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr,
                        sp ? B.mx->qb_lo : nullptr);
           add_pass_any(gb, t1, ry, B.my->qb, B.my->ld8, t2, ry, B.nx, B.ny, B.ny, B.lam_x, B.lam_y,
@@ -237,12 +273,14 @@ class Solver {
   void pressure_inverse_bf16(const double* in, double* out, const std::vector<int>& cols, double s0, double s1,
                              const float* in32 = nullptr) {
     const bool sp = split_pressure();
+    const bool eo = sp && even_odd();
     const int mid = sp ? DT_F32 : DT_BF16;
     const size_t mid_bytes = sp ? 4 : 2;
     char* t1base = sp ? reinterpret_cast<char*>(tf1_) : reinterpret_cast<char*>(tb1_);
     char* t2base = sp ? reinterpret_cast<char*>(tf2_) : reinterpret_cast<char*>(tb2_);
-    for (size_t base = 0; base < cols.size(); base += kMaxEntries) {
-      const size_t count = std::min<size_t>(kMaxEntries, cols.size() - base);
+    const size_t per_launch = eo ? kMaxEntries / 2 : kMaxEntries;
+    for (size_t base = 0; base < cols.size(); base += per_launch) {
+      const size_t count = std::min<size_t>(per_launch, cols.size() - base);
       GemmBatch ga{}, gb{}, gc{}, gd{};
       const Block& B = pblk_;
       const int ry = round8(B.ny), rx = round8(B.nx);
@@ -254,14 +292,30 @@ class Solver {
         double* dst = out + (int64_t)c * np_;
         char* t1 = t1base + (int64_t)c * tstride_ * mid_bytes;
         char* t2 = t2base + (int64_t)c * tstride_ * mid_bytes;
-        add_pass_any(ga, a0, B.nx, B.mx->qb, B.mx->ld8, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr,
-                     sp ? B.mx->qb_lo : nullptr);
-        add_pass_any(gb, t1, ry, B.my->qb, B.my->ld8, t2, ry, B.nx, B.ny, B.ny, B.lam_x, B.lam_y,
-                     sp ? B.my->qb_lo : nullptr);
-        add_pass_any(gc, t2, ry, B.my->qtb, B.my->ld8, t1, rx, B.nx, B.ny, B.ny, nullptr, nullptr,
-                     sp ? B.my->qtb_lo : nullptr);
-        add_pass_any(gd, t1, rx, B.mx->qtb, B.mx->ld8, dst, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr,
-                     sp ? B.mx->qtb_lo : nullptr);
+        if (eo) {
+          const DevMatrix &X = *B.mx, &Y = *B.my;
+          add_pass_any(ga, a0, B.nx, X.qe, X.ldh, t1, ry, B.ny, X.h, X.h, nullptr, nullptr, X.qe_lo);
+          pair_last(ga, B.nx, 1);
+          add_pass_any(ga, a0, B.nx, X.qo, X.ldh, t1 + (int64_t)X.h * ry * mid_bytes, ry, B.ny, B.nx / 2, X.h,
+                       nullptr, nullptr, X.qo_lo);
+          pair_last(ga, B.nx, -1);
+          add_pass_any(gb, t1, ry, Y.qe, Y.ldh, t2, ry, B.nx, Y.h, Y.h, B.lam_x_eo, B.lam_y_eo, Y.qe_lo);
+          pair_last(gb, B.ny, 1);
+          add_pass_any(gb, t1, ry, Y.qo, Y.ldh, t2 + (int64_t)Y.h * mid_bytes, ry, B.nx, B.ny / 2, Y.h, B.lam_x_eo,
+                       B.lam_y_eo + Y.h, Y.qo_lo);
+          pair_last(gb, B.ny, -1);
+          add_pass_any(gc, t2, ry, Y.qtp, Y.ld8, t1, rx, B.nx, B.ny, B.ny, nullptr, nullptr, Y.qtp_lo);
+          add_pass_any(gd, t1, rx, X.qtp, X.ld8, dst, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr, X.qtp_lo);
+        } else {
+          add_pass_any(ga, a0, B.nx, B.mx->qb, B.mx->ld8, t1, ry, B.ny, B.nx, B.nx, nullptr, nullptr,
+                       sp ? B.mx->qb_lo : nullptr);
+          add_pass_any(gb, t1, ry, B.my->qb, B.my->ld8, t2, ry, B.nx, B.ny, B.ny, B.lam_x, B.lam_y,
+                       sp ? B.my->qb_lo : nullptr);
+          add_pass_any(gc, t2, ry, B.my->qtb, B.my->ld8, t1, rx, B.nx, B.ny, B.ny, nullptr, nullptr,
+                       sp ? B.my->qtb_lo : nullptr);
+          add_pass_any(gd, t1, rx, B.mx->qtb, B.mx->ld8, dst, B.nx, B.ny, B.nx, B.nx, nullptr, nullptr,
+                       sp ? B.mx->qtb_lo : nullptr);
+        }
         gd.e[gd.count - 1].aux = src;
       }
       ga.epi = EPI_STORE; ga.trans_c = 1;
@@ -277,6 +331,12 @@ class Solver {
 
   bool split_velocity() const { return (opts_.mixed_split & 1) != 0; }
   bool split_pressure() const { return (opts_.mixed_split & 2) != 0; }
+  // Bit 2: the split forward passes run as even/odd pairs of half contraction length.
+  bool even_odd() const { return (opts_.mixed_split & 4) != 0; }
+  static void pair_last(GemmBatch& b, int n, int sign) {
+    b.e[b.count - 1].pair_n = n;
+    b.e[b.count - 1].pair_sign = sign;
+  }
 
   // ------------------------------------------------------------- profiling
   // Opt-in HIP-event timing of every transform-GEMM launch on the solver's stream,
@@ -1040,6 +1100,14 @@ class Solver {
     B.host_lam_x = eigenvalues_1d(B.nx, g_.hx, kx);
     B.lam_y = upload(B.host_lam_y);
     B.lam_x = upload(B.host_lam_x);
+    auto eo = [](const std::vector<double>& lam) {
+      const int n = (int)lam.size();
+      std::vector<double> out(n);
+      for (int i = 0; i < n; ++i) out[i] = lam[eo_coefficient(i, n)];
+      return out;
+    };
+    B.lam_y_eo = upload(eo(B.host_lam_y));
+    B.lam_x_eo = upload(eo(B.host_lam_x));
   }
 
   const DevMatrix* matrix(int n, int kind) {
@@ -1070,6 +1138,33 @@ class Solver {
     m->qtb = static_cast<__bf16*>(upload_raw(qtb.data(), qtb.size() * sizeof(__bf16)));
     m->qb_lo = static_cast<__bf16*>(upload_raw(qbl.data(), qbl.size() * sizeof(__bf16)));
     m->qtb_lo = static_cast<__bf16*>(upload_raw(qtbl.data(), qtbl.size() * sizeof(__bf16)));
+    // Even/odd halves and the coefficient-ordered transpose.
+    m->h = (n + 1) / 2;
+    m->ldh = round8(m->h);
+    const int h = m->h, hf = n / 2;
+    std::vector<__bf16> qe((size_t)h * m->ldh, (__bf16)0.0f), qel(qe.size(), (__bf16)0.0f);
+    std::vector<__bf16> qo((size_t)h * m->ldh, (__bf16)0.0f), qol(qo.size(), (__bf16)0.0f);
+    std::vector<__bf16> qtp((size_t)n * m->ld8, (__bf16)0.0f), qtpl(qtp.size(), (__bf16)0.0f);
+    auto split = [](double v, __bf16& hi, __bf16& lo) {
+      hi = (__bf16)(float)v;
+      lo = (__bf16)(float)(v - (double)(float)hi);
+    };
+    for (int r = 0; r < h; ++r)
+      for (int j = 0; j < h; ++j) {
+        const bool middle = j >= hf;  // odd n: the self-mirrored column
+        split(q[(size_t)(2 * r) * n + j] * (middle ? 0.5 : 1.0), qe[(size_t)r * m->ldh + j],
+              qel[(size_t)r * m->ldh + j]);
+        if (r < hf && !middle) split(q[(size_t)(2 * r + 1) * n + j], qo[(size_t)r * m->ldh + j], qol[(size_t)r * m->ldh + j]);
+      }
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i)
+        split(q[(size_t)eo_coefficient(i, n) * n + j], qtp[(size_t)j * m->ld8 + i], qtpl[(size_t)j * m->ld8 + i]);
+    m->qe = static_cast<__bf16*>(upload_raw(qe.data(), qe.size() * sizeof(__bf16)));
+    m->qe_lo = static_cast<__bf16*>(upload_raw(qel.data(), qel.size() * sizeof(__bf16)));
+    m->qo = static_cast<__bf16*>(upload_raw(qo.data(), qo.size() * sizeof(__bf16)));
+    m->qo_lo = static_cast<__bf16*>(upload_raw(qol.data(), qol.size() * sizeof(__bf16)));
+    m->qtp = static_cast<__bf16*>(upload_raw(qtp.data(), qtp.size() * sizeof(__bf16)));
+    m->qtp_lo = static_cast<__bf16*>(upload_raw(qtpl.data(), qtpl.size() * sizeof(__bf16)));
     mats_.push_back(std::move(m));
     return mats_.back().get();
   }
@@ -1290,7 +1385,7 @@ void mac_stokes_default_opts(mac_stokes_opts* o) {
   o->refine_tolerance = 1e-10;
   o->inner_tolerance = 1e-3;
   o->max_sweeps = 30;
-  o->mixed_split = 3;
+  o->mixed_split = 7;
   o->device = 0;
 }
 
@@ -1306,7 +1401,7 @@ void* mac_stokes_create(int32_t nx, int32_t ny, double lx, double ly, const mac_
   if (!(o.tolerance > 0) || !(o.krylov_tolerance > 0) || !is_finite(o.tolerance) || !is_finite(o.krylov_tolerance) ||
       o.max_iterations < 1 || (o.precision != MAC_PRECISION_FP64 && o.precision != MAC_PRECISION_MIXED) ||
       !(o.refine_tolerance > 0) || !(o.inner_tolerance > 0) || o.max_sweeps < 1 || o.mixed_split < 0 ||
-      o.mixed_split > 3 || o.fp64_refine < 0 || o.fp64_refine > 30) {
+      o.mixed_split > 7 || o.fp64_refine < 0 || o.fp64_refine > 30) {
     g_create_error = "Invalid solver controls";
     return nullptr;
   }

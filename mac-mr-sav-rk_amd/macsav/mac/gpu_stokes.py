@@ -46,7 +46,7 @@ class GpuTensorStokes:
                  max_iterations: int = 100, matrix_free_operators: bool = True,
                  reuse_transform_buffers: bool = True, *, precision: str = "fp64",
                  refine_tolerance: float = 1e-10, inner_tolerance: float = 1e-3, max_sweeps: int = 30,
-                 mixed_split: int = 3, fp64_refine: int = 0, device: int = 0) -> None:
+                 mixed_split: int = 7, fp64_refine: int = 0, device: int = 0) -> None:
         if (tolerance <= 0 or krylov_tolerance <= 0 or not np.isfinite([tolerance, krylov_tolerance]).all()
                 or max_iterations < 1):
             raise ValueError("Invalid solver controls")

@@ -83,11 +83,14 @@ def test_mixed_meets_the_absolute_bar_and_matches_the_oracle(headline):
     assert got.residual <= 1e-10 and got.divergence_inf <= 1e-10
 
 
-@pytest.mark.parametrize("split", [0, 1, 2, 3])
-@pytest.mark.parametrize("shape,visc", [((7, 5, 2.0, 1.0), 0.07), ((64, 64, 1.0, 1.0), 2.929e-6)])
+@pytest.mark.parametrize("split", range(8))
+@pytest.mark.parametrize("shape,visc", [((7, 5, 2.0, 1.0), 0.07), ((64, 64, 1.0, 1.0), 2.929e-6),
+                                        ((33, 20, 1.3, 0.8), 1e-3)])
 def test_every_mixed_split_variant_converges_to_the_oracle(shape, visc, split):
     """mixed_split selects plain bf16 (0) or the split bf16x3 form for the velocity (bit 0)
-    and pressure (bit 1) transforms; every variant must reach the same solution."""
+    and pressure (bit 1) transforms, and (bit 2) even/odd pairs for the split forward
+    passes; every variant must reach the same solution.  Odd and even sizes cover the
+    self-mirrored middle column of the pairs."""
     from tests.test_gpu_stokes import _stage_rhs
     g = mo.Grid(*shape)
     rhs = _stage_rhs(g, seed=3)
