@@ -107,6 +107,9 @@ typedef struct mac_kernel_profile {
                             k-step in the split bf16x3 form, one otherwise)         */
   double gemm_bytes;     /* their algorithmic bytes: A, Bt and C elements once each  */
   int64_t gemm_launches; /* number of launches timed                                */
+  double gemm_transform_flops; /* the transform work of the passes they compute:
+                            2*M*N*n per pass over a length-n transform (one product;
+                            an even/odd pair counts as the one pass it computes)    */
 } mac_kernel_profile;
 
 /* No counterpart in the reference (its counters are wall-clock only,

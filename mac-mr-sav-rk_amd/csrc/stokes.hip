@@ -340,19 +340,23 @@ class Solver {
 
   // ------------------------------------------------------------- profiling
   // Opt-in HIP-event timing of every transform-GEMM launch on the solver's stream,
-  // with the launches' algorithmic flops and bytes (2MNK and the A, Bt, C elements).
+  // with the launches' MFMA work (2MNK per issued product), the transform work of the
+  // passes they compute (2MNn per pass over a length-n transform, one product: an
+  // even/odd pair does the one pass with two half-length entries), and their bytes (the
+  // A, Bt and C elements once each).
   void set_profiling(bool on) {
     collect_profile();
     profiling_ = on;
-    prof_ms_ = prof_flops_ = prof_bytes_ = 0.0;
+    prof_ms_ = prof_flops_ = prof_bytes_ = prof_tflops_ = 0.0;
     prof_launches_ = 0;
   }
-  void profile(double* ms, double* flops, double* bytes, int64_t* launches) {
+  void profile(double* ms, double* flops, double* bytes, int64_t* launches, double* tflops) {
     collect_profile();
     *ms = prof_ms_;
     *flops = prof_flops_;
     *bytes = prof_bytes_;
     *launches = prof_launches_;
+    *tflops = prof_tflops_;
   }
 
   void run_gemm(GemmBatch& b, int a_type = DT_F64, int c_type = DT_F64, bool bf16 = false, bool split = false) {
@@ -371,20 +375,24 @@ class Solver {
       const double in_a = a_type == DT_F64 ? 8 : (a_type == DT_F32 ? 4 : 2);
       const double in_b = bf16 ? (split ? 4 : 2) : 8;
       const double out_c = c_type == DT_F64 ? 8 : (c_type == DT_F32 ? 4 : 2);
-      double flops = 0, bytes = 0;
+      double flops = 0, bytes = 0, tflops = 0;
       for (int i = 0; i < b.count; ++i) {
         const GemmEntry& e = b.e[i];
         flops += (split ? 6.0 : 2.0) * e.M * e.N * e.K;
-        bytes += in_a * e.M * e.K + in_b * (double)e.N * e.K + out_c * (double)e.M * e.N;
+        // An even/odd entry reads its rows' two halves and computes half the coefficients
+        // of a length-pair_n transform.
+        const double n = e.pair_sign != 0 ? e.pair_n : e.K;
+        tflops += 2.0 * e.M * e.N * n;
+        bytes += in_a * e.M * (e.pair_sign != 0 ? 0.5 * n : e.K) + in_b * (double)e.N * e.K + out_c * (double)e.M * e.N;
       }
-      pending_.push_back({e0, e1, flops, bytes});
+      pending_.push_back({e0, e1, flops, bytes, tflops});
     }
   }
 
  private:
   struct Pending {
     hipEvent_t e0, e1;
-    double flops, bytes;
+    double flops, bytes, tflops;
   };
   hipEvent_t take_event() {
     if (!event_pool_.empty()) {
@@ -406,6 +414,7 @@ class Solver {
       prof_ms_ += ms;
       prof_flops_ += p.flops;
       prof_bytes_ += p.bytes;
+      prof_tflops_ += p.tflops;
       prof_launches_ += 1;
       event_pool_.push_back(p.e0);
       event_pool_.push_back(p.e1);
@@ -413,7 +422,7 @@ class Solver {
     pending_.clear();
   }
   bool profiling_ = false;
-  double prof_ms_ = 0, prof_flops_ = 0, prof_bytes_ = 0;
+  double prof_ms_ = 0, prof_flops_ = 0, prof_bytes_ = 0, prof_tflops_ = 0;
   int64_t prof_launches_ = 0;
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> event_pool_, all_events_;
@@ -1517,7 +1526,9 @@ void mac_stokes_set_profiling(void* h, int32_t on) {
 int32_t mac_stokes_get_profile(void* h, mac_kernel_profile* out) {
   Solver* s = static_cast<Solver*>(h);
   if (!s || !out) return MAC_INVALID_ARGUMENT;
-  return guarded(s, [&] { s->profile(&out->gemm_ms, &out->gemm_flops, &out->gemm_bytes, &out->gemm_launches); });
+  return guarded(s, [&] {
+    s->profile(&out->gemm_ms, &out->gemm_flops, &out->gemm_bytes, &out->gemm_launches, &out->gemm_transform_flops);
+  });
 }
 
 int32_t mac_stokes_set_warm_history(void* h, int32_t period) {

@@ -119,6 +119,7 @@ class KernelProfile(ctypes.Structure):
         ("gemm_flops", ctypes.c_double),
         ("gemm_bytes", ctypes.c_double),
         ("gemm_launches", ctypes.c_int64),
+        ("gemm_transform_flops", ctypes.c_double),
     ]
 
 

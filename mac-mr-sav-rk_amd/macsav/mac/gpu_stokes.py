@@ -258,7 +258,7 @@ class GpuTensorStokes:
         code = self._lib.mac_stokes_get_profile(self._handle, ctypes.byref(prof))
         _native.raise_for(code, self._lib.mac_stokes_last_error(self._handle).decode())
         return {"gemm_ms": prof.gemm_ms, "gemm_flops": prof.gemm_flops, "gemm_bytes": prof.gemm_bytes,
-                "gemm_launches": int(prof.gemm_launches)}
+                "gemm_launches": int(prof.gemm_launches), "gemm_transform_flops": prof.gemm_transform_flops}
 
     # ----------------------------------------------- single operators (device)
     def apply_divergence(self, velocity: Any) -> Any:
