@@ -49,6 +49,7 @@ for _p in (str(ROOT), str(ROOT / "mac-mr-sav-rk_amd")):
         sys.path.insert(0, _p)
 
 PEAK_TFLOPS = {"fp64": 78.6, "bf16": 2500.0}     # MI355X dense: FP64 MFMA, BF16 MFMA
+PEAK_HBM_GBS = 8000.0                            # MI355X HBM3E
 NU, DT, STAGE_DIAG = 1e-3, 2e-4, 0.25            # sdirk3 D[i,i] = 1/4
 STAGES = 4                                       # sdirk3_mrsav stage solves per step
 HISTORY_STEPS = 6                                # extrapolation depth of the history forms
@@ -56,6 +57,7 @@ FILL_STEPS = 6                                   # run steps solved before the r
 BAR = 1e-10
 CALIBRATION_BYTES = 512 << 20                    # one plain copy of known size, before timing
 TRAFFIC_FILE = ROOT / "profiles" / "k_gemm_traffic_1024.md"
+FFT_TRAFFIC_FILE = ROOT / "profiles" / "fft_traffic_1024.md"
 
 FORMS = {
     "fp64": dict(precision="fp64", krylov=1e-14),
@@ -64,6 +66,8 @@ FORMS = {
     "mixed": dict(precision="mixed"),
     "mixed+history": dict(precision="mixed", warm="history"),
     "mixed+network": dict(precision="mixed", warm="network"),
+    # The same history form with the split-bf16 transform GEMMs in place of the FFT passes.
+    "mixed+history/gemm": dict(precision="mixed", warm="history", split=7),
 }
 
 
@@ -83,7 +87,7 @@ def measured_traffic(precision: str):
     """HBM bytes per transform-GEMM launch from the committed counter summary (a line
     'traffic_bytes_per_launch <precision> <bytes>'), or None."""
     try:
-        for line in TRAFFIC_FILE.read_text().splitlines():
+        for line in (FFT_TRAFFIC_FILE if precision == "fft" else TRAFFIC_FILE).read_text().splitlines():
             parts = line.strip().strip("`").split()
             if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == precision:
                 return float(parts[2])
@@ -133,11 +137,46 @@ def resume():
     gc.enable()
 
 
+def transform_roofline(prof):
+    """Roofline of the solve's transform kernels from the event profile of the timed solves.
+
+    FFT passes (the mixed form where every length has a power-of-two FFT): HBM-bound;
+    `achieved` = their algorithmic bytes (every element read once and written once per
+    pass, at the pass's input and output widths) / their device time, against 8 TB/s.
+    Split-bf16 GEMMs: MFMA-bound; `achieved` = the transform flops of the passes (2MNn per
+    pass over a length-n transform, one product) / their device time, against 2.5 PF, with
+    the MFMA work issued (three products per k-step) beside it."""
+    if prof.get("fft_launches", 0) >= prof["gemm_launches"] and prof.get("fft_launches", 0) > 0:
+        fft_s = prof["fft_ms"] * 1e-3
+        n = prof["fft_launches"]
+        achieved = prof["fft_bytes"] / fft_s / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic("fft"),
+                "kernel": "k_fft_rows + k_fft_strips (the three FFT passes of each transform)",
+                "work": "algorithmic bytes: each pass reads and writes every element once",
+                "transform_tflops": prof["fft_transform_flops"] / fft_s / 1e12,
+                "launches": n, "avg_launch_us": 1e6 * fft_s / n,
+                "algorithmic_bytes_per_launch": prof["fft_bytes"] / n,
+                "transform_flops_per_launch": prof["fft_transform_flops"] / n}
+    gemm_s = prof["gemm_ms"] * 1e-3
+    launches = max(prof["gemm_launches"], 1)
+    executed = prof["gemm_flops"] / gemm_s / 1e12
+    algorithmic = prof["gemm_transform_flops"] / gemm_s / 1e12
+    return {"bound": "mfma", "achieved": algorithmic, "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
+            "frac": algorithmic / PEAK_TFLOPS["bf16"], "traffic": measured_traffic("mixed"),
+            "kernel": "k_gemm_bf16",
+            "work": "transform flops 2MNn per pass (one product; n the transform length)",
+            "mfma_executed_tflops": executed, "mfma_executed_frac": executed / PEAK_TFLOPS["bf16"],
+            "launches": prof["gemm_launches"], "avg_launch_us": 1e6 * gemm_s / launches,
+            "flops_per_launch": prof["gemm_transform_flops"] / launches,
+            "algorithmic_bytes_per_launch": prof["gemm_bytes"] / launches}
+
+
 def make_solver(grid, form: str, net=None, history_steps: int = HISTORY_STEPS):
     from macsav import GpuTensorStokes
     f = FORMS[form]
     s = GpuTensorStokes(grid, tolerance=BAR, krylov_tolerance=f.get("krylov", 1e-14), precision=f["precision"],
-                        refine_tolerance=BAR, fp64_refine=f.get("refine", 0))
+                        refine_tolerance=BAR, fp64_refine=f.get("refine", 0), mixed_split=f.get("split", 15))
     if f.get("warm") == "history":
         s.set_warm_start("history", period=STAGES, history_steps=history_steps)
     elif f.get("warm") == "network":
@@ -195,7 +234,7 @@ def replay(grid, rec, fill: int, warmup: int, steps: int, form: str, net=None, p
         _, prof_elapsed = run_once(profiled=True)
         prof = solver.kernel_profile()
         solver.set_profiling(False)
-        stats["gemm_share_of_step"] = prof["gemm_ms"] * 1e-3 / (prof_elapsed + 1e-30)
+        stats["transform_share_of_step"] = (prof["gemm_ms"] + prof["fft_ms"]) * 1e-3 / (prof_elapsed + 1e-30)
     return stats, prof, velocities
 
 
@@ -730,13 +769,6 @@ def bench_stage1024(args) -> dict:
             print(f"[stage1024] whole run {form}: {runs[form]}", file=sys.stderr, flush=True)
     head = next(f for f in ("mixed+network", "mixed+history", "mixed", "fp64+refine") if f in forms)
     main_stats, prof = forms[head], profs[head]
-    gemm_s = prof["gemm_ms"] * 1e-3
-    launches = max(prof["gemm_launches"], 1)
-    # The profile counts the MFMA work issued (three bf16 products per k-step in the split
-    # form, over half the contraction length in the even/odd forward passes) and the
-    # algorithmic transform work: one product, 2MNn, per pass over a length-n transform.
-    executed = prof["gemm_flops"] / gemm_s / 1e12
-    algorithmic = prof["gemm_transform_flops"] / gemm_s / 1e12
     line = {
         "metric": "stage solves/s at 1024^2 MAC, mixed+learned vs plain FP64 GPU, equal residual",
         "value": main_stats["stage_solves_per_s"],
@@ -748,7 +780,7 @@ def bench_stage1024(args) -> dict:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16-gemm/f64-refine",
+        "dtype": "f32-fft/f64-refine" if prof.get("fft_launches", 0) > 0 else "bf16-gemm/f64-refine",
         "data": f"synthetic: stage solves {fill + W}..{fill + W + K - 1} of a recorded {run_steps}-step "
                 f"sdirk3_mrsav run from the seeded stream-function field (modes<={args.modes}); the first "
                 f"{fill} (history fill) and the next {W} are solved untimed",
@@ -763,14 +795,7 @@ def bench_stage1024(args) -> dict:
         "whole_run": {"stage_solves": len(rec), "forms": runs},
         "warm_start_network": net_info,
         "record_seconds": record_s,
-        "roofline": {"bound": "mfma", "achieved": algorithmic, "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
-                     "frac": algorithmic / PEAK_TFLOPS["bf16"], "traffic": measured_traffic("mixed"),
-                     "kernel": "k_gemm_bf16",
-                     "work": "transform flops 2MNn per pass (one product; n the transform length)",
-                     "mfma_executed_tflops": executed, "mfma_executed_frac": executed / PEAK_TFLOPS["bf16"],
-                     "launches": prof["gemm_launches"], "avg_launch_us": 1e6 * gemm_s / launches,
-                     "flops_per_launch": prof["gemm_transform_flops"] / launches,
-                     "algorithmic_bytes_per_launch": prof["gemm_bytes"] / launches},
+        "roofline": transform_roofline(prof),
     }
     if not args.no_cpu:
         f0, m0, v0 = rec[fill + W]

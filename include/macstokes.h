@@ -56,7 +56,10 @@ typedef struct mac_stokes_opts {
                               preconditioner use the split bf16x3 GEMM (hi/lo parts
                               of data and matrix, ~16-bit operands); bit 2 -> their
                               forward passes run as even/odd pairs of half the
-                              contraction length; default 7                         */
+                              contraction length; bit 3 -> where every transform
+                              length allows it (FFT length a power of two, n <= 2048),
+                              the split transforms run as FP32 FFT passes instead;
+                              default 15                                            */
   int32_t fp64_refine;     /* FP64: after the reference solve, up to this many FP64
                               defect-correction sweeps (inner Schur-CG rtol
                               inner_tolerance) until the residual and max|D u| are
@@ -110,6 +113,11 @@ typedef struct mac_kernel_profile {
   double gemm_transform_flops; /* the transform work of the passes they compute:
                             2*M*N*n per pass over a length-n transform (one product;
                             an even/odd pair counts as the one pass it computes)    */
+  double fft_ms;         /* the same for the FFT transform passes (mixed_split bit 3): */
+  double fft_bytes;      /* device time, algorithmic bytes (each element read once and */
+  int64_t fft_launches;  /* written once), launches, and the transform work of the    */
+  double fft_transform_flops; /* passes, counted as for the GEMMs (2*rows*n*n per
+                            length-n transform of `rows` rows)                      */
 } mac_kernel_profile;
 
 /* No counterpart in the reference (its counters are wall-clock only,

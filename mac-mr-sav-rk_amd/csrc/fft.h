@@ -1,0 +1,63 @@
+// FFT transform passes of the mixed solve (fft.hip): declarations shared with the
+// solver's host code.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "internal.h"
+
+namespace macs {
+
+// ------------------------------------------------------- FFT transforms (fft.hip)
+// Orthonormal DCT-II / DST-II / DST-I through FP32 complex FFTs (two real rows per FFT)
+// for lengths whose FFT length M is a power of two: M = n (DCT-II, DST-II) or 2(n + 1)
+// (DST-I), n <= kFftMaxLength.
+constexpr int kFftMaxLength = 2048;
+struct FftPlan {
+  int n, kind, M;       // transform length, Kind (host_common.h), complex FFT length
+  const float* ct;      // cos(2 pi j / M), j = 0 .. M/4: a quarter wave of the twiddles
+  const float2* cs;     // (cos, sin)(pi k / 2n), k < n: DCT-II / DST-II rotations
+};
+// Columns per block of the strip pass.
+constexpr int kFftStrip = 4;
+enum FftMode : int {
+  FFT_ROWS_FWD = 0,  // rows of src (src_type) -> ortho coefficients, FP32 dst (natural layout)
+  FFT_STRIPS = 1,    // columns of the FP32 dst, in place: forward, divide, inverse
+  FFT_ROWS_INV = 2,  // rows of FP32 coefficients in src -> inverse, dst_type with epi
+};
+struct FftJob {
+  const void* src;          // row r at src + r * src_ld (row modes)
+  void* dst;                // row r at dst + r * dst_ld
+  const void* aux;          // FFT_ROWS_INV, EPI_COMBINE: FP64 in dst's layout
+  const double* lam_row;    // FFT_STRIPS: eigenvalue per column (x coefficient)
+  const double* lam_col;    // FFT_STRIPS: eigenvalue per coefficient along the strip
+  FftPlan plan;             // the transform along this pass's direction
+  int rows;                 // row modes: rows of length plan.n
+  int cols;                 // FFT_STRIPS: valid columns (the field has plan.n rows;
+                            // dst_ld a multiple of 16, 16-byte aligned rows)
+  int src_ld, dst_ld;
+  int block_begin;          // set by the launcher
+};
+constexpr int kMaxFftJobs = 8;
+struct FftBatch {
+  FftJob j[kMaxFftJobs];
+  int count, total_blocks;
+  int mode;                 // FftMode
+  int src_type, dst_type;   // FFT_ROWS_FWD: DT_F64 / DT_F32 src; FFT_ROWS_INV: DT_F32 / DT_F64 dst
+  int epi;                  // FFT_STRIPS: EPI_SHIFT / EPI_LAMBDA; FFT_ROWS_INV: EPI_STORE /
+                            // EPI_ACCUM / EPI_COMBINE (the last two with a DT_F64 dst)
+  double s0, s1, thresh;    // as the GEMM epilogues use them
+  const float* ct;          // set by fft_pass: the largest plan's twiddle table (MT = 2^lg_mt)
+  int lg_mt;
+};
+void fft_pass(FftBatch& b, hipStream_t s);
+bool fft_length_ok(int n, int kind);
+// Host tables of a plan: M, the quarter-wave twiddle table (M/4 + 1 floats) and the
+// DCT half-angle rotations (n entries; one for DST-I).
+void make_fft_tables(int n, int kind, int* M, std::vector<float>& ct, std::vector<float2>& cs);
+// Events the next fft_pass launch records at its kernel's own start and end.
+void set_fft_events(hipEvent_t start, hipEvent_t stop);
+
+}  // namespace macs

@@ -27,6 +27,7 @@
 
 #include "host_common.h"
 #include "internal.h"
+#include "fft.h"
 #include "macstokes.h"
 
 namespace macs {
@@ -49,6 +50,9 @@ struct DevMatrix {
   int h = 0, ldh = 0;
   __bf16 *qe = nullptr, *qe_lo = nullptr, *qo = nullptr, *qo_lo = nullptr;
   __bf16 *qtp = nullptr, *qtp_lo = nullptr;
+  // FFT form of the same transform (fft.hip), when its FFT length is a power of two.
+  bool fft_ok = false;
+  FftPlan fft{};
 };
 
 // Coefficient order of the even/odd split: stored position i holds coefficient k.
@@ -100,6 +104,10 @@ class Solver {
       lay_b_.ldv = round8(nx);
       lay_b_.voff = toff_v_;
       lay_b_.stride = tstride_;
+      // FFT scratch per column: x-coefficients of u then v (rows of round16 floats), or of p.
+      auto r16 = [](int64_t v) { return (v + 15) & ~int64_t(15); };
+      foff_v_ = (int64_t)ny * r16(nx - 1);
+      fstride_ = std::max<int64_t>(foff_v_ + (int64_t)(ny - 1) * r16(nx), (int64_t)ny * r16(nx));
     }
     HIP_CHECK(hipHostMalloc((void**)&host_flags_, sizeof(int) * kMaxCols, hipHostMallocDefault));
     HIP_CHECK(hipHostMalloc((void**)&mailbox_, sizeof(Mailbox), hipHostMallocCoherent | hipHostMallocMapped));
@@ -202,6 +210,10 @@ class Solver {
   // natural packed layout) or adds into an FP64 packed velocity (DT_F64, EPI_ACCUM).
   void velocity_inverse_bf16(const void* in, int in_type, const VelLayout& Lin, void* out, int out_type,
                              int64_t out_stride, const std::vector<int>& cols, double mass, double visc) {
+    if (fft_velocity()) {
+      velocity_inverse_fft(in, in_type, Lin, out, out_type, out_stride, cols, mass, visc);
+      return;
+    }
     const bool sp = split_velocity();
     const bool eo = sp && even_odd();                // paired forward passes
     const int mid = sp ? DT_F32 : DT_BF16;           // intermediate storage type
@@ -272,6 +284,10 @@ class Solver {
   // visc * in term reads the FP64 input either way.
   void pressure_inverse_bf16(const double* in, double* out, const std::vector<int>& cols, double s0, double s1,
                              const float* in32 = nullptr) {
+    if (fft_pressure()) {
+      pressure_inverse_fft(in, out, cols, s0, s1, in32);
+      return;
+    }
     const bool sp = split_pressure();
     const bool eo = sp && even_odd();
     const int mid = sp ? DT_F32 : DT_BF16;
@@ -329,6 +345,152 @@ class Solver {
     }
   }
 
+  // ------------------------------------------------------- FFT transforms
+  // The mixed form's velocity inverse as three FP32 FFT passes per block (fft.hip): rows
+  // of the input to x-coefficients (scratch fx_, rows of round16 floats), the y-transform,
+  // shift division and y-inverse in place on column strips, the x-inverse of each row into
+  // the output (FP32 store, or added into an FP64 output).  Input FP64 or FP32.
+  void velocity_inverse_fft(const void* in, int in_type, const VelLayout& Lin, void* out, int out_type,
+                            int64_t out_stride, const std::vector<int>& cols, double mass, double visc) {
+    const size_t in_bytes = in_type == DT_F64 ? 8 : 4;
+    const size_t out_bytes = out_type == DT_F64 ? 8 : 4;
+    const size_t per_launch = kMaxFftJobs / 2;
+    for (size_t base = 0; base < cols.size(); base += per_launch) {
+      const size_t count = std::min(per_launch, cols.size() - base);
+      FftBatch fa{}, fb{}, fc{};
+      fa.mode = FFT_ROWS_FWD;
+      fa.src_type = in_type;
+      fb.mode = FFT_STRIPS;
+      fb.epi = EPI_SHIFT;
+      fb.s0 = mass;
+      fb.s1 = visc;
+      fc.mode = FFT_ROWS_INV;
+      fc.dst_type = out_type;
+      fc.epi = out_type == DT_F64 ? EPI_ACCUM : EPI_STORE;
+      for (size_t t = 0; t < count; ++t) {
+        const int c = cols[base + t];
+        const Block* blocks[2] = {&ublk_, &vblk_};
+        const int64_t in_off[2] = {0, Lin.voff};
+        const int in_ld[2] = {Lin.ldu, Lin.ldv};
+        const int64_t x_off[2] = {0, foff_v_};
+        const int64_t out_off[2] = {0, nu_};
+        for (int bi = 0; bi < 2; ++bi) {
+          const Block& B = *blocks[bi];
+          float* X = fx_ + (int64_t)c * fstride_ + x_off[bi];
+          const int ldx = (B.nx + 15) & ~15;
+          FftJob& ja = fa.j[fa.count++];
+          ja.src = static_cast<const char*>(in) + ((int64_t)c * Lin.stride + in_off[bi]) * in_bytes;
+          ja.src_ld = in_ld[bi];
+          ja.dst = X;
+          ja.dst_ld = ldx;
+          ja.plan = B.mx->fft;
+          ja.rows = B.ny;
+          FftJob& jb = fb.j[fb.count++];
+          jb.dst = X;
+          jb.dst_ld = ldx;
+          jb.plan = B.my->fft;
+          jb.cols = B.nx;
+          jb.lam_row = B.lam_x;
+          jb.lam_col = B.lam_y;
+          FftJob& jc = fc.j[fc.count++];
+          jc.src = X;
+          jc.src_ld = ldx;
+          jc.dst = static_cast<char*>(out) + ((int64_t)c * out_stride + out_off[bi]) * out_bytes;
+          jc.dst_ld = B.nx;
+          jc.plan = B.mx->fft;
+          jc.rows = B.ny;
+        }
+      }
+      run_fft(fa, (double)in_bytes, 4.0);
+      run_fft(fb, 4.0, 4.0);
+      run_fft(fc, 4.0, out_type == DT_F64 ? 16.0 : 4.0);
+    }
+  }
+
+  // z = s0 * (D D^T)^+ r + s1 * r through the same three passes (eigenvalue division with
+  // the zero mode dropped; the last pass combines with the FP64 input).  in32 (optional):
+  // the input rounded to FP32, read by the first pass.
+  void pressure_inverse_fft(const double* in, double* out, const std::vector<int>& cols, double s0, double s1,
+                            const float* in32) {
+    const Block& B = pblk_;
+    const int ldx = (B.nx + 15) & ~15;
+    const size_t per_launch = kMaxFftJobs;
+    for (size_t base = 0; base < cols.size(); base += per_launch) {
+      const size_t count = std::min(per_launch, cols.size() - base);
+      FftBatch fa{}, fb{}, fc{};
+      fa.mode = FFT_ROWS_FWD;
+      fa.src_type = in32 != nullptr ? DT_F32 : DT_F64;
+      fb.mode = FFT_STRIPS;
+      fb.epi = EPI_LAMBDA;
+      fb.thresh = p_thresh_;
+      fc.mode = FFT_ROWS_INV;
+      fc.dst_type = DT_F64;
+      fc.epi = EPI_COMBINE;
+      fc.s0 = s0;
+      fc.s1 = s1;
+      for (size_t t = 0; t < count; ++t) {
+        const int c = cols[base + t];
+        float* X = fx_ + (int64_t)c * fstride_;
+        FftJob& ja = fa.j[fa.count++];
+        ja.src = in32 != nullptr ? static_cast<const void*>(in32 + (int64_t)c * np_)
+                                 : static_cast<const void*>(in + (int64_t)c * np_);
+        ja.src_ld = B.nx;
+        ja.dst = X;
+        ja.dst_ld = ldx;
+        ja.plan = B.mx->fft;
+        ja.rows = B.ny;
+        FftJob& jb = fb.j[fb.count++];
+        jb.dst = X;
+        jb.dst_ld = ldx;
+        jb.plan = B.my->fft;
+        jb.cols = B.nx;
+        jb.lam_row = B.lam_x;
+        jb.lam_col = B.lam_y;
+        FftJob& jc = fc.j[fc.count++];
+        jc.src = X;
+        jc.src_ld = ldx;
+        jc.dst = out + (int64_t)c * np_;
+        jc.aux = in + (int64_t)c * np_;
+        jc.dst_ld = B.nx;
+        jc.plan = B.mx->fft;
+        jc.rows = B.ny;
+      }
+      run_fft(fa, in32 != nullptr ? 4.0 : 8.0, 4.0);
+      run_fft(fb, 4.0, 4.0);
+      run_fft(fc, 4.0, 16.0);
+    }
+  }
+
+  // One FFT pass, event-timed when profiling: the transform work of the passes it computes
+  // (2 rows n^2 per length-n transform of `rows` rows, two transforms for a strip pass) and
+  // its algorithmic bytes (every element read once at in_b bytes, written once at out_b).
+  void run_fft(FftBatch& b, double in_b, double out_b) {
+    if (b.count <= 0) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (profiling_) {
+      e0 = take_event();
+      e1 = take_event();
+      set_fft_events(e0, e1);
+    }
+    fft_pass(b, stream_);
+    set_fft_events(nullptr, nullptr);
+    HIP_CHECK(hipGetLastError());
+    if (profiling_) {
+      double tflops = 0, bytes = 0;
+      for (int i = 0; i < b.count; ++i) {
+        const FftJob& j = b.j[i];
+        const double n = j.plan.n;
+        const double lines = b.mode == FFT_STRIPS ? j.cols : j.rows;
+        tflops += (b.mode == FFT_STRIPS ? 2.0 : 1.0) * 2.0 * lines * n * n;
+        bytes += lines * n * (in_b + out_b);
+      }
+      pending_fft_.push_back({e0, e1, 0.0, bytes, tflops});
+    }
+  }
+
+  // Bit 3: the split transforms run as FP32 FFT passes where every length allows.
+  bool fft_velocity() const { return (opts_.mixed_split & 8) != 0 && split_velocity() && fft_ok_vel_; }
+  bool fft_pressure() const { return (opts_.mixed_split & 8) != 0 && split_pressure() && fft_ok_p_; }
   bool split_velocity() const { return (opts_.mixed_split & 1) != 0; }
   bool split_pressure() const { return (opts_.mixed_split & 2) != 0; }
   // Bit 2: the split forward passes run as even/odd pairs of half contraction length.
@@ -349,6 +511,8 @@ class Solver {
     profiling_ = on;
     prof_ms_ = prof_flops_ = prof_bytes_ = prof_tflops_ = 0.0;
     prof_launches_ = 0;
+    pf_ms_ = pf_bytes_ = pf_tflops_ = 0.0;
+    pf_launches_ = 0;
   }
   void profile(double* ms, double* flops, double* bytes, int64_t* launches, double* tflops) {
     collect_profile();
@@ -357,6 +521,13 @@ class Solver {
     *bytes = prof_bytes_;
     *launches = prof_launches_;
     *tflops = prof_tflops_;
+  }
+  void profile_fft(double* ms, double* bytes, int64_t* launches, double* tflops) {
+    collect_profile();
+    *ms = pf_ms_;
+    *bytes = pf_bytes_;
+    *launches = pf_launches_;
+    *tflops = pf_tflops_;
   }
 
   void run_gemm(GemmBatch& b, int a_type = DT_F64, int c_type = DT_F64, bool bf16 = false, bool split = false) {
@@ -406,8 +577,19 @@ class Solver {
     return e;
   }
   void collect_profile() {
-    if (pending_.empty()) return;
+    if (pending_.empty() && pending_fft_.empty()) return;
     HIP_CHECK(hipStreamSynchronize(stream_));
+    for (const Pending& p : pending_fft_) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, p.e0, p.e1));
+      pf_ms_ += ms;
+      pf_bytes_ += p.bytes;
+      pf_tflops_ += p.tflops;
+      pf_launches_ += 1;
+      event_pool_.push_back(p.e0);
+      event_pool_.push_back(p.e1);
+    }
+    pending_fft_.clear();
     for (const Pending& p : pending_) {
       float ms = 0.f;
       HIP_CHECK(hipEventElapsedTime(&ms, p.e0, p.e1));
@@ -425,6 +607,9 @@ class Solver {
   double prof_ms_ = 0, prof_flops_ = 0, prof_bytes_ = 0, prof_tflops_ = 0;
   int64_t prof_launches_ = 0;
   std::vector<Pending> pending_;
+  double pf_ms_ = 0, pf_bytes_ = 0, pf_tflops_ = 0;  // FFT passes
+  int64_t pf_launches_ = 0;
+  std::vector<Pending> pending_fft_;
   std::vector<hipEvent_t> event_pool_, all_events_;
 
  public:
@@ -1098,6 +1283,8 @@ class Solver {
     set_lam(ublk_, KIND_DST2, KIND_DST1);
     set_lam(vblk_, KIND_DST1, KIND_DST2);
     set_lam(pblk_, KIND_DCT2, KIND_DCT2);
+    fft_ok_vel_ = ublk_.mx->fft_ok && ublk_.my->fft_ok && vblk_.mx->fft_ok && vblk_.my->fft_ok;
+    fft_ok_p_ = pblk_.mx->fft_ok && pblk_.my->fft_ok;
     double lmax = 0.0;
     for (double a : pblk_.host_lam_y)
       for (double b : pblk_.host_lam_x) lmax = std::max(lmax, a + b);
@@ -1174,6 +1361,18 @@ class Solver {
     m->qo_lo = static_cast<__bf16*>(upload_raw(qol.data(), qol.size() * sizeof(__bf16)));
     m->qtp = static_cast<__bf16*>(upload_raw(qtp.data(), qtp.size() * sizeof(__bf16)));
     m->qtp_lo = static_cast<__bf16*>(upload_raw(qtpl.data(), qtpl.size() * sizeof(__bf16)));
+    if (fft_length_ok(n, kind)) {
+      std::vector<float> ct;
+      std::vector<float2> cs;
+      int M = 0;
+      make_fft_tables(n, kind, &M, ct, cs);
+      m->fft.n = n;
+      m->fft.kind = kind;
+      m->fft.M = M;
+      m->fft.ct = static_cast<const float*>(upload_raw(ct.data(), ct.size() * sizeof(float)));
+      m->fft.cs = static_cast<const float2*>(upload_raw(cs.data(), cs.size() * sizeof(float2)));
+      m->fft_ok = true;
+    }
     mats_.push_back(std::move(m));
     return mats_.back().get();
   }
@@ -1246,6 +1445,7 @@ class Solver {
     tf1_ = (float*)take(sizeof(float) * tstride_ * ncols);
     tf2_ = (float*)take(sizeof(float) * tstride_ * ncols);
     tfb_ = (float*)take(sizeof(float) * tstride_ * ncols);
+    fx_ = (fft_ok_vel_ || fft_ok_p_) ? (float*)take(sizeof(float) * fstride_ * ncols) : nullptr;
     const int nb = std::max(defect_blocks(size_, ncols), reduce_blocks(np_) * ncols);
     partials_ = (double*)take(sizeof(double) * (4 * (size_t)nb + 16));
     sums_ = (double*)take(sizeof(double) * (ncols + 4));
@@ -1291,6 +1491,9 @@ class Solver {
   __bf16 *tb_ = nullptr, *tb1_ = nullptr, *tb2_ = nullptr;
   float *tf1_ = nullptr, *tf2_ = nullptr, *tfb_ = nullptr;
   int64_t tstride_ = 0, toff_v_ = 0;
+  float* fx_ = nullptr;              // FFT transform scratch (fstride_ floats per column)
+  int64_t fstride_ = 0, foff_v_ = 0;
+  bool fft_ok_vel_ = false, fft_ok_p_ = false;  // every length of the blocks has an FFT plan
   VelLayout lay_b_{};
   // During a mixed sweep's inner CG: r rounded to FP32 (np per column), kept in tfb_
   // between the kernels that update r and the next pressure inverse (tfb_ takes D^T p
@@ -1394,7 +1597,7 @@ void mac_stokes_default_opts(mac_stokes_opts* o) {
   o->refine_tolerance = 1e-10;
   o->inner_tolerance = 1e-3;
   o->max_sweeps = 30;
-  o->mixed_split = 7;
+  o->mixed_split = 15;
   o->device = 0;
 }
 
@@ -1410,7 +1613,7 @@ void* mac_stokes_create(int32_t nx, int32_t ny, double lx, double ly, const mac_
   if (!(o.tolerance > 0) || !(o.krylov_tolerance > 0) || !is_finite(o.tolerance) || !is_finite(o.krylov_tolerance) ||
       o.max_iterations < 1 || (o.precision != MAC_PRECISION_FP64 && o.precision != MAC_PRECISION_MIXED) ||
       !(o.refine_tolerance > 0) || !(o.inner_tolerance > 0) || o.max_sweeps < 1 || o.mixed_split < 0 ||
-      o.mixed_split > 7 || o.fp64_refine < 0 || o.fp64_refine > 30) {
+      o.mixed_split > 15 || o.fp64_refine < 0 || o.fp64_refine > 30) {
     g_create_error = "Invalid solver controls";
     return nullptr;
   }
@@ -1528,6 +1731,7 @@ int32_t mac_stokes_get_profile(void* h, mac_kernel_profile* out) {
   if (!s || !out) return MAC_INVALID_ARGUMENT;
   return guarded(s, [&] {
     s->profile(&out->gemm_ms, &out->gemm_flops, &out->gemm_bytes, &out->gemm_launches, &out->gemm_transform_flops);
+    s->profile_fft(&out->fft_ms, &out->fft_bytes, &out->fft_launches, &out->fft_transform_flops);
   });
 }
 

@@ -120,6 +120,10 @@ class KernelProfile(ctypes.Structure):
         ("gemm_bytes", ctypes.c_double),
         ("gemm_launches", ctypes.c_int64),
         ("gemm_transform_flops", ctypes.c_double),
+        ("fft_ms", ctypes.c_double),
+        ("fft_bytes", ctypes.c_double),
+        ("fft_launches", ctypes.c_int64),
+        ("fft_transform_flops", ctypes.c_double),
     ]
 
 

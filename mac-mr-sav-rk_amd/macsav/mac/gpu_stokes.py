@@ -46,7 +46,7 @@ class GpuTensorStokes:
                  max_iterations: int = 100, matrix_free_operators: bool = True,
                  reuse_transform_buffers: bool = True, *, precision: str = "fp64",
                  refine_tolerance: float = 1e-10, inner_tolerance: float = 1e-3, max_sweeps: int = 30,
-                 mixed_split: int = 7, fp64_refine: int = 0, device: int = 0) -> None:
+                 mixed_split: int = 15, fp64_refine: int = 0, device: int = 0) -> None:
         if (tolerance <= 0 or krylov_tolerance <= 0 or not np.isfinite([tolerance, krylov_tolerance]).all()
                 or max_iterations < 1):
             raise ValueError("Invalid solver controls")
@@ -258,7 +258,9 @@ class GpuTensorStokes:
         code = self._lib.mac_stokes_get_profile(self._handle, ctypes.byref(prof))
         _native.raise_for(code, self._lib.mac_stokes_last_error(self._handle).decode())
         return {"gemm_ms": prof.gemm_ms, "gemm_flops": prof.gemm_flops, "gemm_bytes": prof.gemm_bytes,
-                "gemm_launches": int(prof.gemm_launches), "gemm_transform_flops": prof.gemm_transform_flops}
+                "gemm_launches": int(prof.gemm_launches), "gemm_transform_flops": prof.gemm_transform_flops,
+                "fft_ms": prof.fft_ms, "fft_bytes": prof.fft_bytes, "fft_launches": int(prof.fft_launches),
+                "fft_transform_flops": prof.fft_transform_flops}
 
     # ----------------------------------------------- single operators (device)
     def apply_divergence(self, velocity: Any) -> Any:

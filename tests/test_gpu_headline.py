@@ -83,13 +83,14 @@ def test_mixed_meets_the_absolute_bar_and_matches_the_oracle(headline):
     assert got.residual <= 1e-10 and got.divergence_inf <= 1e-10
 
 
-@pytest.mark.parametrize("split", range(8))
+@pytest.mark.parametrize("split", range(16))
 @pytest.mark.parametrize("shape,visc", [((7, 5, 2.0, 1.0), 0.07), ((64, 64, 1.0, 1.0), 2.929e-6),
                                         ((33, 20, 1.3, 0.8), 1e-3)])
 def test_every_mixed_split_variant_converges_to_the_oracle(shape, visc, split):
     """mixed_split selects plain bf16 (0) or the split bf16x3 form for the velocity (bit 0)
-    and pressure (bit 1) transforms, and (bit 2) even/odd pairs for the split forward
-    passes; every variant must reach the same solution.  Odd and even sizes cover the
+    and pressure (bit 1) transforms, (bit 2) even/odd pairs for the split forward passes
+    and (bit 3) FP32 FFT passes in place of the split GEMMs where the lengths allow (64 x 64
+    here); every variant must reach the same solution.  Odd and even sizes cover the
     self-mirrored middle column of the pairs."""
     from tests.test_gpu_stokes import _stage_rhs
     g = mo.Grid(*shape)
