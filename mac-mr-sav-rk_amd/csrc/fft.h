@@ -17,7 +17,7 @@ namespace macs {
 constexpr int kFftMaxLength = 2048;
 struct FftPlan {
   int n, kind, M;       // transform length, Kind (host_common.h), complex FFT length
-  const float* ct;      // cos(2 pi j / M), j = 0 .. M/4: a quarter wave of the twiddles
+  const float2* ct;     // e^{-2 pi i e / M}, e < M/2: half a period of the twiddles
   const float2* cs;     // (cos, sin)(pi k / 2n), k < n: DCT-II / DST-II rotations
 };
 // Columns per block of the strip pass.
@@ -49,14 +49,14 @@ struct FftBatch {
   int epi;                  // FFT_STRIPS: EPI_SHIFT / EPI_LAMBDA; FFT_ROWS_INV: EPI_STORE /
                             // EPI_ACCUM / EPI_COMBINE (the last two with a DT_F64 dst)
   double s0, s1, thresh;    // as the GEMM epilogues use them
-  const float* ct;          // set by fft_pass: the largest plan's twiddle table (MT = 2^lg_mt)
+  const float2* ct;         // set by fft_pass: the largest plan's twiddle table (MT = 2^lg_mt)
   int lg_mt;
 };
 void fft_pass(FftBatch& b, hipStream_t s);
 bool fft_length_ok(int n, int kind);
-// Host tables of a plan: M, the quarter-wave twiddle table (M/4 + 1 floats) and the
-// DCT half-angle rotations (n entries; one for DST-I).
-void make_fft_tables(int n, int kind, int* M, std::vector<float>& ct, std::vector<float2>& cs);
+// Host tables of a plan: M, the half-period twiddle table (M/2 entries) and the DCT
+// half-angle rotations (n entries; one for DST-I).
+void make_fft_tables(int n, int kind, int* M, std::vector<float2>& th, std::vector<float2>& cs);
 // Events the next fft_pass launch records at its kernel's own start and end.
 void set_fft_events(hipEvent_t start, hipEvent_t stop);
 

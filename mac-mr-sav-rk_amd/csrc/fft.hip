@@ -57,15 +57,12 @@ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x
 // -i z (forward) or +i z (inverse)
 __device__ inline float2 rot(float2 z, bool inv) { return inv ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x); }
 
-// e^{-2 pi i e / MT} (0 <= e < MT) from the quarter-wave table ct[j] = cos(2 pi j / MT),
-// j = 0 .. MT/4 (in LDS): the quadrant's quarter turn applied to the first-quadrant angle.
-__device__ inline float2 twiddle(const float* ct, int e, int lg_mt) {
-  const int qt = 1 << (lg_mt - 2);
-  const int quad = e >> (lg_mt - 2), j = e & (qt - 1);
-  const float c = ct[j], s = ct[qt - j];
-  const float cr = quad == 0 ? c : (quad == 1 ? -s : (quad == 2 ? -c : s));
-  const float sr = quad == 0 ? s : (quad == 1 ? c : (quad == 2 ? -s : -c));
-  return make_float2(cr, -sr);
+// e^{-2 pi i e / MT} (0 <= e < MT) from the half-period table th[e] = e^{-2 pi i e / MT},
+// e < MT/2 (in LDS): the second half of the period is the first negated.
+__device__ inline float2 twiddle(const float2* th, int e, int lg_mt) {
+  const int half = 1 << (lg_mt - 1);
+  const float2 w = th[e & (half - 1)];
+  return e < half ? w : make_float2(-w.x, -w.y);
 }
 
 // LDS slot of FFT point i: one pad slot every 16 points, so the strided accesses of the
@@ -114,7 +111,7 @@ __device__ inline void dft(float2 (&x)[R], bool inv) {
 // every butterfly's inputs, synchronises, then writes).  Twiddles from the LDS table of
 // length MT = 2^lg_mt (a multiple of M).
 template <int R, int P, int Q>
-__device__ inline void fft_stage(float2* buf, int M, int MP, int lgM, int lgNs, const float* ct, int lg_mt, bool inv) {
+__device__ inline void fft_stage(float2* buf, int M, int MP, int lgM, int lgNs, const float2* ct, int lg_mt, bool inv) {
   constexpr int lgR = R == 2 ? 1 : (R == 4 ? 2 : 3);
   const int nb = M >> lgR;  // butterflies per buffer
   const int tid = threadIdx.x;
@@ -163,7 +160,7 @@ __device__ inline void fft_stage(float2* buf, int M, int MP, int lgM, int lgNs, 
 // The caller synchronises after filling the buffers; the result is visible to every
 // thread on return.
 template <int P, int MMAX>
-__device__ inline void fft_run(float2* buf, int M, int MP, const float* ct, int lg_mt, bool inv) {
+__device__ inline void fft_run(float2* buf, int M, int MP, const float2* ct, int lg_mt, bool inv) {
   const int lgM = 31 - __clz(M);
   int lgNs = 0;
   for (; lgNs + 3 <= lgM; lgNs += 3) fft_stage<8, P, cmax(1, MMAX / 2048)>(buf, M, MP, lgM, lgNs, ct, lg_mt, inv);
@@ -171,22 +168,22 @@ __device__ inline void fft_run(float2* buf, int M, int MP, const float* ct, int 
   else if (lgM - lgNs == 1) fft_stage<2, P, cmax(1, MMAX / 512)>(buf, M, MP, lgM, lgNs, ct, lg_mt, inv);
 }
 
-// The launch's quarter-wave table (MT/4 + 1 <= MMAX/4 + 1 floats): every thread's loads
-// issued at once (fetch), then stored to LDS (commit; the caller synchronises).
+// The launch's half-period twiddle table (MT/2 <= MMAX/2 complex floats): every thread's
+// loads issued at once (fetch), then stored to LDS (commit; the caller synchronises).
 template <int MMAX>
 struct Table {
-  static constexpr int E = MMAX / 4 / kThreads + 1;
-  float t[E];
+  static constexpr int E = cmax(1, MMAX / 2 / kThreads);
+  float2 t[E];
   __device__ inline void fetch(const FftBatch& b) {
-    const int q = (1 << (b.lg_mt - 2)) + 1;
+    const int q = 1 << (b.lg_mt - 1);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = threadIdx.x + e * kThreads;
-      t[e] = i < q ? b.ct[i] : 0.f;
+      t[e] = i < q ? b.ct[i] : make_float2(0.f, 0.f);
     }
   }
-  __device__ inline void commit(float* ct, const FftBatch& b) const {
-    const int q = (1 << (b.lg_mt - 2)) + 1;
+  __device__ inline void commit(float2* ct, const FftBatch& b) const {
+    const int q = 1 << (b.lg_mt - 1);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = threadIdx.x + e * kThreads;
@@ -296,7 +293,7 @@ __global__ __launch_bounds__(256) void k_fft_rows(const FftBatch b) {
   const int r0 = (bid - J.block_begin) * 2 * P;
   const int tid = threadIdx.x;
   const bool dct = pl.kind != KIND_DST1;
-  float* ct = reinterpret_cast<float*>(lds + P * lds_points(MMAX));
+  float2* ct = lds + P * lds_points(MMAX);
   // Rows a (2p) and b (2p + 1) of each pair: input rows (forward) or coefficient rows.
   float va[P][EPT], vb[P][EPT];
   const int stype = FWD ? b.src_type : DT_F32;
@@ -466,7 +463,7 @@ __global__ __launch_bounds__(256) void k_fft_strips(const FftBatch b) {
   const int kx0 = (bid - J.block_begin) * W;
   const int tid = threadIdx.x;
   const bool dct = pl.kind != KIND_DST1;
-  float* ct = reinterpret_cast<float*>(lds + P * lds_points(MMAX));
+  float2* ct = lds + P * lds_points(MMAX);
   float* X = static_cast<float*>(J.dst);
   const int64_t ld = J.dst_ld;
   float h[NPT][W];
@@ -633,7 +630,7 @@ void launch_rows(FftBatch& b, hipStream_t s) {
   }
   b.total_blocks = total;
   if (total == 0) return;
-  const size_t lds = sizeof(float2) * P * lds_points(MMAX) + sizeof(float) * (MMAX / 4 + 1);
+  const size_t lds = sizeof(float2) * (P * lds_points(MMAX) + MMAX / 2);
   if (b.mode == FFT_ROWS_FWD) launch(k_fft_rows<MMAX, EPT, true>, total, lds, s, b);
   else launch(k_fft_rows<MMAX, EPT, false>, total, lds, s, b);
 }
@@ -649,7 +646,7 @@ void launch_strips(FftBatch& b, hipStream_t s) {
   }
   b.total_blocks = total;
   if (total == 0) return;
-  launch(k_fft_strips<MMAX, NPT, W>, total, sizeof(float2) * P * lds_points(MMAX) + sizeof(float) * (MMAX / 4 + 1), s, b);
+  launch(k_fft_strips<MMAX, NPT, W>, total, sizeof(float2) * (P * lds_points(MMAX) + MMAX / 2), s, b);
 }
 
 int pow2_at_least(int v) {
@@ -679,8 +676,8 @@ void fft_pass(FftBatch& b, hipStream_t s) {
     mmax = mmax > b.j[i].plan.M ? mmax : b.j[i].plan.M;
     nmax = nmax > b.j[i].plan.n ? nmax : b.j[i].plan.n;
   }
-  // One quarter-wave twiddle table serves every plan of the launch: the largest M's
-  // (MT = M), of which every smaller power-of-two length takes every (MT / M')-th entry.
+  // One twiddle table serves every plan of the launch: the largest M's (MT = M), of
+  // which every smaller power-of-two length takes every (MT / M')-th entry.
   b.ct = b.j[big].plan.ct;
   b.lg_mt = 31 - __builtin_clz((unsigned)b.j[big].plan.M);
   mmax = pow2_at_least(mmax);
@@ -704,12 +701,15 @@ void fft_pass(FftBatch& b, hipStream_t s) {
   else launch_rows<4096, 8>(b, s);
 }
 
-void make_fft_tables(int n, int kind, int* M, std::vector<float>& ct, std::vector<float2>& cs) {
+void make_fft_tables(int n, int kind, int* M, std::vector<float2>& th, std::vector<float2>& cs) {
   const int m = kind == KIND_DST1 ? 2 * (n + 1) : n;
   *M = m;
-  // cos(2 pi j / m), j = 0 .. m/4: a quarter wave of the twiddles (m >= 4).
-  ct.resize(m / 4 + 1);
-  for (int j = 0; j <= m / 4; ++j) ct[j] = (float)std::cos(2.0 * M_PI * (double)j / (double)m);
+  // e^{-2 pi i e / m}, e < m/2: half a period of the twiddles.
+  th.resize(m / 2);
+  for (int e = 0; e < m / 2; ++e) {
+    const double a = 2.0 * M_PI * (double)e / (double)m;
+    th[e] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
   // (cos, sin)(pi k / 2n), k < n: the DCT family's half-angle rotations.
   cs.assign(kind == KIND_DST1 ? 1 : n, make_float2(1.f, 0.f));
   if (kind != KIND_DST1)

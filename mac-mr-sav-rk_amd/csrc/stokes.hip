@@ -1362,14 +1362,13 @@ class Solver {
     m->qtp = static_cast<__bf16*>(upload_raw(qtp.data(), qtp.size() * sizeof(__bf16)));
     m->qtp_lo = static_cast<__bf16*>(upload_raw(qtpl.data(), qtpl.size() * sizeof(__bf16)));
     if (fft_length_ok(n, kind)) {
-      std::vector<float> ct;
-      std::vector<float2> cs;
+      std::vector<float2> ct, cs;
       int M = 0;
       make_fft_tables(n, kind, &M, ct, cs);
       m->fft.n = n;
       m->fft.kind = kind;
       m->fft.M = M;
-      m->fft.ct = static_cast<const float*>(upload_raw(ct.data(), ct.size() * sizeof(float)));
+      m->fft.ct = static_cast<const float2*>(upload_raw(ct.data(), ct.size() * sizeof(float2)));
       m->fft.cs = static_cast<const float2*>(upload_raw(cs.data(), cs.size() * sizeof(float2)));
       m->fft_ok = true;
     }
