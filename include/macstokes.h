@@ -49,7 +49,10 @@ typedef struct mac_stokes_opts {
   int32_t precision;       /* MAC_PRECISION_*                                        */
   double refine_tolerance; /* MIXED: stop when the residual and max|D u| are both
                               <= this (default 1e-10)                               */
-  double inner_tolerance;  /* MIXED: inner Schur-CG rtol per sweep (default 1e-3)    */
+  double inner_tolerance;  /* MIXED: cap of the inner Schur-CG rtol of a sweep, which
+                              is 0.25 * refine_tolerance / (the sweep's defect) clamped
+                              to [min(1e-4, cap), cap]; FP64 refinement: its sweeps'
+                              rtol (default 1e-3)                                    */
   int32_t max_sweeps;      /* MIXED: outer defect-correction sweep cap (default 30)  */
   int32_t device;          /* HIP device ordinal                                      */
   int32_t mixed_split;     /* MIXED: bit 0 -> velocity inverses, bit 1 -> pressure

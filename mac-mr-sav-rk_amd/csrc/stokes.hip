@@ -1033,7 +1033,13 @@ class Solver {
       // divergence.  Besides the relative test, a column stops once max|r| is half the bar,
       // where the sweep is certain to meet it (the momentum defect it leaves is the split
       // operator's error on an already small correction).
-      cg_init(cg_, sums2_, sums3_, opts_.inner_tolerance, ncols, stream_, rr_, 0.5 * opts_.refine_tolerance);
+      // This sweep's inner tolerance: a sweep cuts the defect by about the inner tolerance,
+      // so 0.25 x bar / defect would reach the bar in one sweep; inner_tolerance caps it
+      // (a history start's small defect), 1e-4 floors it (a cold start's large one).
+      const double cap = opts_.inner_tolerance;
+      const double sweep_tol = std::min(cap, std::max(std::min(1e-4, cap), 0.25 * opts_.refine_tolerance /
+                                                                              std::max(out.residual, out.divergence)));
+      cg_init(cg_, sums2_, sums3_, sweep_tol, ncols, stream_, rr_, 0.5 * opts_.refine_tolerance);
       accumulate_ = true;
       pcg(ncols, mass, visc, inner, false);
       accumulate_ = false;

@@ -45,7 +45,7 @@ class GpuTensorStokes:
     def __init__(self, operators: Any, tolerance: float = 1e-9, krylov_tolerance: float = 1e-12,
                  max_iterations: int = 100, matrix_free_operators: bool = True,
                  reuse_transform_buffers: bool = True, *, precision: str = "fp64",
-                 refine_tolerance: float = 1e-10, inner_tolerance: float = 1e-3, max_sweeps: int = 30,
+                 refine_tolerance: float = 1e-10, inner_tolerance: float | None = None, max_sweeps: int = 30,
                  mixed_split: int = 15, fp64_refine: int = 0, device: int = 0) -> None:
         if (tolerance <= 0 or krylov_tolerance <= 0 or not np.isfinite([tolerance, krylov_tolerance]).all()
                 or max_iterations < 1):
@@ -73,6 +73,11 @@ class GpuTensorStokes:
         opts.max_iterations = max_iterations
         opts.precision = _PRECISIONS[precision]
         opts.refine_tolerance = refine_tolerance
+        # The mixed form's inner tolerance is the cap of its per-sweep tolerance (the sweep's
+        # defect sets it below that): 1e-2, where one inner CG iteration from a history start
+        # usually reaches the bar; refined FP64 keeps 1e-3 for its sweeps.
+        if inner_tolerance is None:
+            inner_tolerance = 1e-2 if precision == "mixed" else 1e-3
         opts.inner_tolerance = inner_tolerance
         opts.max_sweeps = max_sweeps
         opts.mixed_split = mixed_split

@@ -117,10 +117,11 @@ def test_adaptive_lid_run_mixed_history_equals_plain_fp64():
     sdirk3_mrsav under PI control (atol = rtol = 1e-6), device model.  The mixed solve with
     the same-stage history start (whose extrapolation weights follow the changing steps)
     makes the plain FP64 run's step decisions, ends on its field and needs fewer Schur-CG
-    iterations in total.  The embedded error is the norm of a difference of two fields
-    that agree to about 1e-6, so solve differences near 1e-11 reach it at about 1e-5
-    relative: the step proposals agree to 1e-4 and the final fields, whose time error
-    follows the steps, to 1e-8 (each solve itself meets 1e-9; those tests are separate)."""
+    iterations in total.  Each mixed sweep's inner tolerance aims the sweep at a quarter
+    of the 1e-10 bar, so the solves agree with FP64 to the bar.  The embedded error is the
+    norm of a difference of two fields that agree to about 1e-6, which amplifies
+    bar-level solve differences to about 1e-3 relative in the step proposals (7.1e-4
+    measured).  The decisions are identical, and the final fields agree to 1.5e-10."""
     from macsav import GpuTensorStokes, MACGrid
     from macsav.adaptivity import PIController, integrate_adaptive
     from macsav.fields import stream_function_velocity
@@ -143,7 +144,8 @@ def test_adaptive_lid_run_mixed_history_equals_plain_fp64():
     steps0 = [a["step"] for a in r0.attempts]
     steps1 = [a["step"] for a in r1.attempts]
     assert len(steps0) == len(steps1) and len(set(steps0[1:])) > 1
-    np.testing.assert_allclose(steps1, steps0, rtol=1e-4)
+    assert [a["accepted"] for a in r1.attempts] == [a["accepted"] for a in r0.attempts]
+    np.testing.assert_allclose(steps1, steps0, rtol=2e-3)
     assert np.linalg.norm(u1 - u0) <= 1e-8 * np.linalg.norm(u0)
     assert r1.diagnostics[-1]["divergence_inf"] < 1e-10
     assert it1 < it0
