@@ -106,6 +106,15 @@ int32_t mac_stokes_solve(void* handle, const double* rhs, int64_t ld_rhs, int32_
                          int64_t ld_pressure, const double* pressure_guess, int64_t ld_guess,
                          int32_t* iterations, mac_solve_report* report, int32_t device_pointers);
 
+/* No counterpart in the reference (its solve is a synchronous NumPy call): runs the
+ * handle's work on the caller's HIP stream (`use` = 1; `stream` may be the null stream)
+ * or on its own (`use` = 0, the default).  On the caller's stream a solve with device
+ * pointers is ordered like the caller's other work: it returns once the host has the
+ * solve's scalars, without waiting for the output copies queued after them, and needs no
+ * synchronisation of that stream before the call.  Pending work on the previous stream
+ * is finished first. */
+int32_t mac_stokes_set_stream(void* handle, void* stream, int32_t use);
+
 /* ---- measurement: HIP-event timing of the transform GEMM launches ---- */
 typedef struct mac_kernel_profile {
   double gemm_ms;        /* summed device time of the transform-GEMM launches       */

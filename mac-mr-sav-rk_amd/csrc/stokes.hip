@@ -86,6 +86,7 @@ class Solver {
     np_ = (int64_t)nx * ny;
     HIP_CHECK(hipSetDevice(o.device));
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = stream_;
     packed_.ldu = nx - 1;
     packed_.ldv = nx;
     packed_.voff = nu_;
@@ -125,13 +126,19 @@ class Solver {
     if (hist_u_) (void)hipFree(hist_u_);
     if (hist_b_) (void)hipFree(hist_b_);
     if (mailbox_) (void)hipHostFree(mailbox_);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (own_stream_) (void)hipStreamDestroy(own_stream_);
   }
 
   const GridDims& grid() const { return g_; }
   int64_t size() const { return size_; }
   int64_t np() const { return np_; }
   hipStream_t stream() const { return stream_; }
+  bool external_stream() const { return stream_ != own_stream_; }
+  // The stream the handle's work runs on: the caller's (use) or its own.
+  void set_stream(hipStream_t s, bool use) {
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    stream_ = use ? s : own_stream_;
+  }
   const mac_stokes_opts& opts() const { return opts_; }
   std::string error;
 
@@ -1477,6 +1484,7 @@ class Solver {
   GridDims g_{};
   int64_t nu_ = 0, nv_ = 0, size_ = 0, np_ = 0;
   hipStream_t stream_ = nullptr;
+  hipStream_t own_stream_ = nullptr;
   VelLayout packed_{};
   std::vector<std::unique_ptr<DevMatrix>> mats_;
   Block ublk_, vblk_, pblk_;
@@ -1691,7 +1699,8 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
         if (!is_finite(v)) throw macs::ArgFailure("Invalid Stokes RHS");
     }
     Solver::Outcome out = s->solve(f, lf, ncols, mass, viscosity, vel, lv, pr, lp, gs, lg);
-    HIP_CHECK(hipStreamSynchronize(st));
+    // On the caller's stream, device outputs are stream-ordered: no wait for the hand-out.
+    if (!device_pointers || !s->external_stream()) HIP_CHECK(hipStreamSynchronize(st));
     if (!device_pointers) {
       HIP_CHECK(hipMemcpy2D(velocity, ld_velocity * sizeof(double), vel, size * sizeof(double), size * sizeof(double),
                             ncols, hipMemcpyDeviceToHost));
@@ -1722,6 +1731,15 @@ int32_t mac_stokes_solve(void* h, const double* rhs, int64_t ld_rhs, int32_t nco
       std::snprintf(buf, sizeof buf, "Stokes residual=%.3e, divergence=%.3e", out.residual, out.divergence);
       throw macs::SolveFailure(MAC_GATE_FAILED, buf);
     }
+  });
+}
+
+int32_t mac_stokes_set_stream(void* h, void* stream, int32_t use) {
+  Solver* s = static_cast<Solver*>(h);
+  if (!s || (use != 0 && use != 1)) return MAC_INVALID_ARGUMENT;
+  return guarded(s, [&] {
+    HIP_CHECK(hipSetDevice(s->opts().device));
+    s->set_stream(static_cast<hipStream_t>(stream), use == 1);
   });
 }
 

@@ -34,6 +34,7 @@ EXPORTED = (
     "mac_stokes_destroy",
     "mac_stokes_last_error",
     "mac_stokes_solve",
+    "mac_stokes_set_stream",
     "mac_stokes_set_profiling",
     "mac_stokes_get_profile",
     "mac_cnn_create",
@@ -158,6 +159,8 @@ def load() -> ctypes.CDLL:
     lib.mac_stokes_solve.argtypes = [_P, _P, _I64, _I32, _D, _D, _P, _I64, _P, _I64, _P, _I64,
                                      ctypes.POINTER(_I32), ctypes.POINTER(SolveReport), _I32]
     lib.mac_stokes_solve.restype = _I32
+    lib.mac_stokes_set_stream.argtypes = [_P, _P, _I32]
+    lib.mac_stokes_set_stream.restype = _I32
     lib.mac_stokes_set_profiling.argtypes = [_P, _I32]
     lib.mac_stokes_set_profiling.restype = None
     lib.mac_stokes_get_profile.argtypes = [_P, ctypes.POINTER(KernelProfile)]

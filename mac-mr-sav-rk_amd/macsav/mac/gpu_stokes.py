@@ -91,6 +91,7 @@ class GpuTensorStokes:
             raise RuntimeError(f"GPU Stokes setup failed: {message}")
         self._lib = lib
         self._handle = handle
+        self._stream: int | None = None  # the stream handed to the library (torch's current one)
         self.device = device
         self.setup_seconds = perf_counter() - start
         self.solve_seconds = 0.0
@@ -156,7 +157,13 @@ class GpuTensorStokes:
             guess = torch.as_tensor(pressure_guess, device=rhs.device).to(torch.float64).reshape(grid.np, k)
             guess = guess.t().contiguous()
             guess_ptr, ld_guess = guess.data_ptr(), grid.np
-        torch.cuda.current_stream(rhs.device).synchronize()
+        # The solve runs on the caller's current stream, ordered after the work that made
+        # the right-hand side: no synchronisation here, and the outputs are stream-ordered.
+        stream = torch.cuda.current_stream(rhs.device).cuda_stream
+        if stream != self._stream:
+            _native.raise_for(self._lib.mac_stokes_set_stream(self._handle, stream, 1),
+                              self._lib.mac_stokes_last_error(self._handle).decode())
+            self._stream = stream
         report = self._call(cols.data_ptr(), grid.size, k, mass, viscosity, vel.data_ptr(), pres.data_ptr(),
                             guess_ptr, ld_guess, device=True)
         velocity = vel.t() if two_d else vel[0]

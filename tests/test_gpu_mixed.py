@@ -88,3 +88,24 @@ def test_mixed_matches_fp64_gpu_at_1024():
     assert mixed.residual <= 1e-10 and mixed.divergence_inf <= 1e-10
     assert _rel(mixed.velocity.cpu().numpy(), plain.velocity.cpu().numpy()) < 1e-9
     assert _rel(mixed.pressure.cpu().numpy(), plain.pressure.cpu().numpy()) < 1e-9
+
+
+def test_solves_run_on_the_callers_stream():
+    """The wrapper hands the library torch's current stream: a solve issued inside another
+    stream's context is ordered after that stream's work (the right-hand side is formed
+    there and not synchronised) and equals the same solve on the default stream; the
+    outputs are ready for the stream's later work without a host wait."""
+    g = mo.Grid(64, 64, 1.0, 1.0)
+    rhs = _stage_rhs(g, seed=5)
+    s = _backend(g, tolerance=1e-10, precision="mixed")
+    ref = s.solve(_dev(rhs), mass=1.0, viscosity=1e-4)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        f = _dev(rhs) * 3.0 - _dev(rhs) * 2.0       # queued on `side`, not synchronised
+        got = s.solve(f, mass=1.0, viscosity=1e-4)
+        vel_norm = torch.linalg.vector_norm(got.velocity - ref.velocity)  # also on `side`
+    side.synchronize()
+    assert float(vel_norm) <= 1e-12 * float(torch.linalg.vector_norm(ref.velocity))
+    again = s.solve(_dev(rhs), mass=1.0, viscosity=1e-4)     # back on the default stream
+    assert float(torch.linalg.vector_norm(again.pressure - ref.pressure)) <= 1e-12 * float(
+        torch.linalg.vector_norm(ref.pressure))
