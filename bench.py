@@ -810,8 +810,8 @@ def main() -> int:
     global HISTORY_STEPS
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed stage solves")
-    ap.add_argument("--warmup", type=int, default=5, help="untimed stage solves before them (after the fill)")
+    ap.add_argument("--steps", type=int, default=60, help="timed stage solves")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed stage solves before them (after the fill)")
     ap.add_argument("--run-steps", type=int, default=12, help="steps of the recorded run (at least the fill "
                                                              "plus the warm-up and timed solves)")
     ap.add_argument("--n", type=int, default=None, help="grid size (n x n)")
