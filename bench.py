@@ -609,7 +609,7 @@ def bench_run(args) -> dict:
     return {"metric": "sdirk3_mrsav steps/s end to end at 1024^2 MAC", "value": best["steps_per_s"],
             "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "bf16-gemm/f64-refine",
+            "vs_baseline": None, "dtype": "f32-fft/f64-refine (mixed form)",
             "data": f"seeded stream-function field (modes<={args.modes}), nu=1e-3, dt=2e-4",
             "config": {"workload": f"MAC {grid.nx}x{grid.ny} sdirk3_mrsav run, 4 two-RHS stage solves per step"},
             "forms": out, "speedup_vs_host_fp64": best["steps_per_s"] / out["host+fp64"]["steps_per_s"]}
@@ -678,7 +678,7 @@ def bench_combined(args) -> dict:
     return {"metric": f"stage solves/s, adaptive sdirk3_mrsav at {n}^2 MAC (lid + perturbation)",
             "value": best["stage_solves_per_s"], "unit": "stage solves/s", "n_gpus": 1,
             "steps": best["accepted_steps"], "warmup": 1, "ms_per_step": 1e3 * best["seconds"] / best["accepted_steps"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16-gemm/f64-refine",
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32-fft/f64-refine (mixed form)",
             "data": f"seeded stream-function perturbation (modes<={modes}, amplitude 0.05) on a cavity with lid "
                     f"velocity {lid:g}",
             "config": {"workload": f"MAC {n}x{n}, nu={nu:g}, lid {lid:g}, sdirk3_mrsav with PI control "
@@ -719,7 +719,7 @@ def bench_sizes(args) -> dict:
     return {"metric": "ms per two-RHS stage solve against grid size, plain FP64 and mixed",
             "value": rows["1024"]["mixed_speedup"], "unit": "x (mixed over FP64 at 1024^2)", "n_gpus": 1,
             "steps": 8, "warmup": 4, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "bf16-gemm/f64-refine", "data": "recorded sdirk3_mrsav stage right-hand sides per size",
+            "dtype": "f32-fft (power-of-two n) or bf16-gemm/f64-refine", "data": "recorded sdirk3_mrsav stage right-hand sides per size",
             "config": {"workload": "MAC n x n stage solves, n in 64..1024, zero start, residual 1e-10"},
             "sizes": rows}
 
