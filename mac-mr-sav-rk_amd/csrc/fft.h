@@ -12,13 +12,16 @@ namespace macs {
 
 // ------------------------------------------------------- FFT transforms (fft.hip)
 // Orthonormal DCT-II / DST-II / DST-I through FP32 complex FFTs (two real rows per FFT)
-// for lengths whose FFT length M is a power of two: M = n (DCT-II, DST-II) or 2(n + 1)
-// (DST-I), n <= kFftMaxLength.
+// for lengths whose FFT length M = n (DCT-II, DST-II) or n + 1 (DST-I) is a power of two
+// in [kFftMinLength, kFftMaxLength].  One launch holds one transform group of one M: the
+// velocity pair (DST-I and DST-II) or the pressure's DCT-II.
+constexpr int kFftMinLength = 16;
 constexpr int kFftMaxLength = 2048;
 struct FftPlan {
   int n, kind, M;       // transform length, Kind (host_common.h), complex FFT length
   const float2* ct;     // e^{-2 pi i e / M}, e < M/2: half a period of the twiddles
-  const float2* cs;     // (cos, sin)(pi k / 2n), k < n: DCT-II / DST-II rotations
+  const float2* cs;     // (cos, sin)(pi k / 2n), k < n: DCT-II / DST-II rotations;
+                        // DST-I: (sin(pi j / M), 0), j <= M/2, its input weights
 };
 // Columns per block of the strip pass.
 constexpr int kFftStrip = 4;
@@ -49,13 +52,11 @@ struct FftBatch {
   int epi;                  // FFT_STRIPS: EPI_SHIFT / EPI_LAMBDA; FFT_ROWS_INV: EPI_STORE /
                             // EPI_ACCUM / EPI_COMBINE (the last two with a DT_F64 dst)
   double s0, s1, thresh;    // as the GEMM epilogues use them
-  const float2* ct;         // set by fft_pass: the largest plan's twiddle table (MT = 2^lg_mt)
-  int lg_mt;
 };
 void fft_pass(FftBatch& b, hipStream_t s);
 bool fft_length_ok(int n, int kind);
 // Host tables of a plan: M, the half-period twiddle table (M/2 entries) and the DCT
-// half-angle rotations (n entries; one for DST-I).
+// half-angle rotations (n entries) or DST-I's input weights (M/2 + 1 entries).
 void make_fft_tables(int n, int kind, int* M, std::vector<float2>& th, std::vector<float2>& cs);
 // Events the next fft_pass launch records at its kernel's own start and end.
 void set_fft_events(hipEvent_t start, hipEvent_t stop);
