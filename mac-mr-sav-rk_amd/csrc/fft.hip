@@ -51,10 +51,10 @@ hipEvent_t f_ev_start = nullptr, f_ev_stop = nullptr;
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
-// Row pairs per block of the row passes: one pair from M = 1024 up, where a pair's
-// butterflies already fill most of the block (measured at 1024^2: one pair per block 8 %
-// faster than two).  The strip pass keeps two pairs per block: one round of forward and
-// inverse FFTs per four-column strip.
+// Row pairs per block of the row passes: one pair from M = 1024 up (at 1024^2 one pair
+// per block is 8-12 % faster than two: twice the blocks hide more of the memory latency,
+// though half the threads idle through the radix-8 stages).  The strip pass keeps two
+// pairs per block: one round of forward and inverse FFTs per four-column strip.
 constexpr int rows_pairs(int m) { return m <= 512 ? 4 : 1; }
 constexpr int kStripPairs = 2;
 static_assert(kFftStrip == 2 * kStripPairs, "one round of column-pair FFTs per strip");
@@ -134,30 +134,38 @@ __device__ inline float2 twiddle(const float2* th, int e) {
 // One radix-R Stockham stage (sub-transforms of length 2^LGNS before it) over P buffers of
 // M points each, in place: the block reads every butterfly's inputs, synchronises, then
 // writes.  Reads sit nb = M/R points apart and writes 2^LGNS apart; from 16 points on,
-// their padded slots are constant offsets from one padded base.
+// their padded slots are constant offsets from one padded base.  With nb >= 256 a thread
+// takes butterfly b of every buffer (its twiddles computed once); with fewer, the P nb
+// butterflies of all buffers spread over the block, so no wave idles through the stage.
 template <int R, int P, int M, int LGNS, bool INV>
 __device__ inline void fft_stage(float2* buf, const float2* th) {
   constexpr int lgR = ilog2(R);
   constexpr int nb = M >> lgR;  // butterflies per buffer
-  constexpr int Q = (nb + kThreads - 1) / kThreads;
-  constexpr bool ALL = nb % kThreads == 0;  // every thread has Q butterflies
+  constexpr bool SHARED = nb >= kThreads;
+  constexpr int PT = SHARED ? P : 1;                 // buffers per butterfly slot
+  constexpr int G = SHARED ? nb : P * nb;            // butterfly slots
+  constexpr int Q = (G + kThreads - 1) / kThreads;   // slots per thread
+  constexpr bool ALL = G % kThreads == 0;            // every thread has Q slots
   constexpr int NS = 1 << LGNS;
   constexpr int MP = lds_points(M);
   constexpr int TSH = ilog2(M) - LGNS - lgR;  // twiddle exponent k r in units of M / (NS R)
   const int tid = threadIdx.x;
-  float2 v[P][Q][R];
+  float2 v[PT][Q][R];
   float2 w[Q][R];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int b = tid + q * kThreads;
-    if (ALL || b < nb) {
+    const int g = tid + q * kThreads;
+    if (ALL || g < G) {
+      const int b = g & (nb - 1);
+      const int p0 = SHARED ? 0 : g >> ilog2(nb);
       const int lb = lp(b);
 #pragma unroll
-      for (int p = 0; p < P; ++p)
+      for (int pt = 0; pt < PT; ++pt)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          if constexpr (nb % 16 == 0) v[p][q][r] = buf[p * MP + lb + r * (nb + nb / 16)];
-          else v[p][q][r] = buf[p * MP + lp(b + r * nb)];
+          const int o = (p0 + pt) * MP;
+          if constexpr (nb % 16 == 0) v[pt][q][r] = buf[o + lb + r * (nb + nb / 16)];
+          else v[pt][q][r] = buf[o + lp(b + r * nb)];
         }
       const int k = b & (NS - 1);
 #pragma unroll
@@ -172,21 +180,24 @@ __device__ inline void fft_stage(float2* buf, const float2* th) {
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int b = tid + q * kThreads;
-    if (!ALL && b >= nb) continue;
+    const int g = tid + q * kThreads;
+    if (!ALL && g >= G) continue;
+    const int b = g & (nb - 1);
+    const int p0 = SHARED ? 0 : g >> ilog2(nb);
     const int k = b & (NS - 1);
     const int base = ((b >> LGNS) << (LGNS + lgR)) + k;
     const int lbase = lp(base);
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
+    for (int pt = 0; pt < PT; ++pt) {
+      const int o = (p0 + pt) * MP;
       float2 x[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[r] = r == 0 ? v[p][q][0] : cmul(v[p][q][r], w[q][r]);
+      for (int r = 0; r < R; ++r) x[r] = r == 0 ? v[pt][q][0] : cmul(v[pt][q][r], w[q][r]);
       dft<R, INV>(x);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if constexpr (NS % 16 == 0) buf[p * MP + lbase + r * (NS + NS / 16)] = x[r];
-        else buf[p * MP + lp(base + (r << LGNS))] = x[r];
+        if constexpr (NS % 16 == 0) buf[o + lbase + r * (NS + NS / 16)] = x[r];
+        else buf[o + lp(base + (r << LGNS))] = x[r];
       }
     }
   }
