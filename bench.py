@@ -198,8 +198,8 @@ def _stats(solver, before, n, elapsed, worst):
 
 def replay(grid, rec, fill: int, warmup: int, steps: int, form: str, net=None, profile: bool = False):
     """rec[:fill + warmup] untimed (the history fill, then the warm-up), then
-    rec[fill + warmup:][:steps] timed; optionally a second pass with HIP events on the
-    timed solves' transform GEMMs."""
+    rec[fill + warmup:][:steps]: once untimed, keeping the velocities, then timed;
+    optionally a further pass with HIP events on the timed solves' transform launches."""
     import torch
     solver = make_solver(grid, form, net, HISTORY_STEPS)
     first = fill + warmup
@@ -227,8 +227,13 @@ def replay(grid, rec, fill: int, warmup: int, steps: int, form: str, net=None, p
         resume()
         return _stats(solver, before, steps, elapsed, worst), elapsed
 
+    # The velocities the forms are compared on come from an untimed pass over the same
+    # records: kept inside the timed pass, every output would stay allocated, and the
+    # caching allocator would call hipMalloc (about 60 us) inside every timed solve.  In the
+    # timed pass each output is freed when the next replaces it, as in an integrator.
     velocities = []
-    stats, _ = run_once(velocities)
+    run_once(velocities)
+    stats, _ = run_once()
     prof = None
     if profile:
         _, prof_elapsed = run_once(profiled=True)
