@@ -170,6 +170,7 @@ __device__ inline void fft_stage(float2* buf, const float2* th) {
       const int k = b & (NS - 1);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
+        if constexpr (LGNS == 0) continue;  // k = 0: every twiddle is 1
         // e = k r M / (NS R) < r M / R: the second half of the period only for r > R/2.
         if (2 * r > R) w[q][r] = twiddle<M, true>(th, (k * r) << TSH);
         else w[q][r] = twiddle<M, false>(th, (k * r) << TSH);
@@ -192,7 +193,7 @@ __device__ inline void fft_stage(float2* buf, const float2* th) {
       const int o = (p0 + pt) * MP;
       float2 x[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[r] = r == 0 ? v[pt][q][0] : cmul(v[pt][q][r], w[q][r]);
+      for (int r = 0; r < R; ++r) x[r] = (r == 0 || LGNS == 0) ? v[pt][q][r] : cmul(v[pt][q][r], w[q][r]);
       dft<R, INV>(x);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
