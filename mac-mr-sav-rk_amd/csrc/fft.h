@@ -30,6 +30,11 @@ enum FftMode : int {
   FFT_STRIPS = 1,    // columns of the FP32 dst, in place: forward, divide, inverse
   FFT_ROWS_INV = 2,  // rows of FP32 coefficients in src -> inverse, dst_type with epi
 };
+// Source of the forward row pass's rows: the rows themselves (of the batch's src_type), or
+// the differences of FP64 pressure rows that D^T makes of them: element i of row r is
+// (p[r][i] - p[r][i+1]) / src_h (FFT_SRC_DX, the u faces) or (p[r][i] - p[r+1][i]) /
+// src_h (FFT_SRC_DY, the v faces), p[r] at src + r * src_ld, rounded to FP32 once.
+enum FftSource : int { FFT_SRC_ROWS = 0, FFT_SRC_DX = 1, FFT_SRC_DY = 2 };
 struct FftJob {
   const void* src;          // row r at src + r * src_ld (row modes)
   void* dst;                // row r at dst + r * dst_ld
@@ -42,6 +47,8 @@ struct FftJob {
                             // dst_ld a multiple of 16, 16-byte aligned rows)
   int src_ld, dst_ld;
   int block_begin;          // set by the launcher
+  int src_mode;             // FftSource (FFT_ROWS_FWD)
+  double src_h;             // FFT_SRC_DX / FFT_SRC_DY: the grid spacing
 };
 constexpr int kMaxFftJobs = 8;
 struct FftBatch {

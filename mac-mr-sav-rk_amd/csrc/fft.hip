@@ -374,6 +374,16 @@ __device__ inline float load_f(const void* p, int type, int64_t i) {
   return type == DT_F64 ? (float)static_cast<const double*>(p)[i] : static_cast<const float*>(p)[i];
 }
 
+// Element j of source row r of a forward row pass (FftSource): the row's own element, or
+// the FP64 pressure difference D^T makes of it, with D^T's arithmetic ((p0 - p1) / h in
+// FP64, then rounded).
+__device__ inline float load_row(const FftJob& J, int type, int r, int j) {
+  if (J.src_mode == FFT_SRC_ROWS) return load_f(J.src, type, (int64_t)r * J.src_ld + j);
+  const double* p = static_cast<const double*>(J.src) + (int64_t)r * J.src_ld + j;
+  const double q = J.src_mode == FFT_SRC_DX ? p[1] : p[J.src_ld];
+  return (float)((p[0] - q) / J.src_h);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs: give each XCD a contiguous run of
 // the launch's block indices, so neighbouring strips (which share cache lines) share an L2.
 __device__ inline int block_index(const FftBatch& b) {
@@ -436,8 +446,8 @@ __device__ inline void rows_body(const FftBatch& b, const FftJob& J, int r0, flo
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int j = tid + e * kThreads;
-      va[p][e] = (j < N && ra < rows) ? load_f(J.src, stype, (int64_t)ra * J.src_ld + j) : 0.f;
-      vb[p][e] = (j < N && rb < rows) ? load_f(J.src, stype, (int64_t)rb * J.src_ld + j) : 0.f;
+      va[p][e] = (j < N && ra < rows) ? load_row(J, stype, ra, j) : 0.f;
+      vb[p][e] = (j < N && rb < rows) ? load_row(J, stype, rb, j) : 0.f;
     }
   }
   float2 cs[EPT];
@@ -579,8 +589,8 @@ __device__ inline void rows_body_dst1(const FftBatch& b, const FftJob& J, int r0
       for (int e = 0; e < C; ++e) {
         const int j = 1 + tid + e * kThreads;
         const bool ok = j <= H && r < rows;
-        fl[p][e][h] = ok ? load_f(J.src, stype, (int64_t)r * J.src_ld + j - 1) : 0.f;
-        fh[p][e][h] = ok ? load_f(J.src, stype, (int64_t)r * J.src_ld + M - j - 1) : 0.f;
+        fl[p][e][h] = ok ? load_row(J, stype, r, j - 1) : 0.f;
+        fh[p][e][h] = ok ? load_row(J, stype, r, M - j - 1) : 0.f;
       }
     }
   float w[C];

@@ -356,9 +356,11 @@ class Solver {
   // The mixed form's velocity inverse as three FP32 FFT passes per block (fft.hip): rows
   // of the input to x-coefficients (scratch fx_, rows of round16 floats), the y-transform,
   // shift division and y-inverse in place on column strips, the x-inverse of each row into
-  // the output (FP32 store, or added into an FP64 output).  Input FP64 or FP32.
+  // the output (FP32 store, or added into an FP64 output).  Input FP64 or FP32, or with
+  // grad_p the FP64 pressure columns (stride np_) whose D^T the first pass forms itself.
   void velocity_inverse_fft(const void* in, int in_type, const VelLayout& Lin, void* out, int out_type,
-                            int64_t out_stride, const std::vector<int>& cols, double mass, double visc) {
+                            int64_t out_stride, const std::vector<int>& cols, double mass, double visc,
+                            const double* grad_p = nullptr) {
     const size_t in_bytes = in_type == DT_F64 ? 8 : 4;
     const size_t out_bytes = out_type == DT_F64 ? 8 : 4;
     const size_t per_launch = kMaxFftJobs / 2;
@@ -386,8 +388,15 @@ class Solver {
           float* X = fx_ + (int64_t)c * fstride_ + x_off[bi];
           const int ldx = (B.nx + 15) & ~15;
           FftJob& ja = fa.j[fa.count++];
-          ja.src = static_cast<const char*>(in) + ((int64_t)c * Lin.stride + in_off[bi]) * in_bytes;
-          ja.src_ld = in_ld[bi];
+          if (grad_p != nullptr) {
+            ja.src = grad_p + (int64_t)c * np_;
+            ja.src_ld = g_.nx;
+            ja.src_mode = bi == 0 ? FFT_SRC_DX : FFT_SRC_DY;
+            ja.src_h = bi == 0 ? g_.hx : g_.hy;
+          } else {
+            ja.src = static_cast<const char*>(in) + ((int64_t)c * Lin.stride + in_off[bi]) * in_bytes;
+            ja.src_ld = in_ld[bi];
+          }
           ja.dst = X;
           ja.dst_ld = ldx;
           ja.plan = B.mx->fft;
@@ -408,7 +417,7 @@ class Solver {
           jc.rows = B.ny;
         }
       }
-      run_fft(fa, (double)in_bytes, 4.0);
+      run_fft(fa, grad_p != nullptr ? 8.0 : (double)in_bytes, 4.0);
       run_fft(fb, 4.0, 4.0);
       run_fft(fc, 4.0, out_type == DT_F64 ? 16.0 : 4.0);
     }
@@ -1224,10 +1233,15 @@ class Solver {
         pressure_inverse_bf16(r_, z_, active, mass, visc, r32_);
         mixed_rho(z_, r_, np_, ncols, cg_, partials_, zmean, stream_);
         mixed_update_p(p_, z_, zmean, np_, ncols, cg_, stream_);
-        const int dt_type = split_velocity() ? DT_F32 : DT_BF16;
-        void* dt_buf = split_velocity() ? static_cast<void*>(tfb_) : static_cast<void*>(tb_);
-        divergence_transpose(p_, np_, nullptr, nullptr, 0, dt_type, dt_buf, lay_b_, g_, ncols, cg_, stream_);
-        velocity_inverse_bf16(dt_buf, dt_type, lay_b_, wf_, DT_F32, size_, active, mass, visc);
+        if (fft_velocity()) {
+          // D^T p formed by the velocity inverse's first pass from the pressure rows.
+          velocity_inverse_fft(nullptr, DT_F64, lay_b_, wf_, DT_F32, size_, active, mass, visc, p_);
+        } else {
+          const int dt_type = split_velocity() ? DT_F32 : DT_BF16;
+          void* dt_buf = split_velocity() ? static_cast<void*>(tfb_) : static_cast<void*>(tb_);
+          divergence_transpose(p_, np_, nullptr, nullptr, 0, dt_type, dt_buf, lay_b_, g_, ncols, cg_, stream_);
+          velocity_inverse_bf16(dt_buf, dt_type, lay_b_, wf_, DT_F32, size_, active, mass, visc);
+        }
         divergence_sums3(DT_F32, wf_, packed_, g_, q_, np_, 1.0, p_, ncols, partials_, cg_, stream_);
         mixed_alpha(partials_, np_, ncols, cg_, qmean, stream_);
         mixed_update_xr(x_, r_, p_, q_, qmean, np_, ncols, cg_, partials_, stream_, r32_);
