@@ -399,6 +399,37 @@ __global__ void k_cg_init(CgState* st, const double* bb, const double* bmax, dou
   st[c] = s;
 }
 
+// A mixed sweep's CG start after k_sweep_start, one block per column: (b, b) and max|b|
+// combined from the sweep start's partials (as k_finalize2 combines them), the column's
+// state (as k_cg_init) and the first convergence check (as k_cg_check).
+__global__ __launch_bounds__(kThreads) void k_sweep_cg_start(const double* partials, int gx, int ncols, double* bb,
+                                                             double* bmax, CgState* st, double rtol, double* rr_out,
+                                                             double mtol, int* flags) {
+  __shared__ double sh[4];
+  const int c = blockIdx.x;
+  const double b2 = combine_partials<RED_SUM>(partials + (int64_t)c * gx, gx, sh);
+  __syncthreads();  // sh is reused
+  const double bm = combine_partials<RED_MAXABS>(partials + (int64_t)(ncols + c) * gx, gx, sh);
+  if (threadIdx.x != 0) return;
+  bb[c] = b2;
+  bmax[c] = bm;
+  if (rr_out != nullptr) rr_out[c] = b2;
+  CgState v{};
+  v.bnorm = sqrt(b2);
+  v.tol = rtol * v.bnorm;
+  v.mtol = mtol;
+  v.rr = b2;
+  v.active = (bm != 0.0 && v.bnorm != 0.0) ? 1 : 0;
+  v.converged = v.active ? 0 : 1;
+  v.iters = 0;
+  if (v.active && sqrt(b2) < v.tol) {
+    v.active = 0;
+    v.converged = 1;
+  }
+  st[c] = v;
+  flags[c] = v.active;
+}
+
 __global__ void k_cg_check(CgState* st, const double* rr, int* flags, int ncols) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
@@ -871,6 +902,13 @@ void sweep_start(double* b, double* r, double* x, int64_t n, int ncols, const do
   k_finalize2<RED_SUM, RED_MAXABS><<<2 * ncols, kThreads, 0, s>>>(partials, (int)grid.x, ncols, bb,
                                                                   partials + (int64_t)ncols * grid.x,
                                                                   (int)grid.x, bmax);
+}
+void sweep_start_cg(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,
+                    double* bb, double* bmax, CgState* st, double rtol, double* rr_out, double mtol, int* flags,
+                    hipStream_t s, float* r32) {
+  const dim3 grid = col_grid(n, ncols);
+  k_sweep_start<<<grid, kThreads, 0, s>>>(b, r, x, n, sums, partials, r32);
+  k_sweep_cg_start<<<ncols, kThreads, 0, s>>>(partials, (int)grid.x, ncols, bb, bmax, st, rtol, rr_out, mtol, flags);
 }
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s) {
   k_cg_check<<<(ncols + 63) / 64, 64, 0, s>>>(st, rr, flags, ncols);

@@ -1042,9 +1042,6 @@ class Solver {
       if (!ahead) velocity_inverse_bf16(ru_buf, ru_type, lay_b_, du_, DT_F32, size_, all, mass, visc);
       divergence(DT_F32, du_, packed_, g_, b_, np_, -1.0, ncols, RED_SUM, partials_, nullptr, stream_, rc_,
                  fin(sums_));
-      // b -= mean, r = b, x = 0, with (b, b) and max|b| for the start test.
-      sweep_start(b_, r_, x_, np_, ncols, sums_, partials_, sums2_, sums3_, stream_, tfb_);
-      r32_ = tfb_;
       // The inner CG's recurrence residual is r = b - S~ x = -D (u + du): the sweep's new
       // divergence.  Besides the relative test, a column stops once max|r| is half the bar,
       // where the sweep is certain to meet it (the momentum defect it leaves is the split
@@ -1055,7 +1052,12 @@ class Solver {
       const double cap = opts_.inner_tolerance;
       const double sweep_tol = std::min(cap, std::max(std::min(1e-4, cap), 0.25 * opts_.refine_tolerance /
                                                                               std::max(out.residual, out.divergence)));
-      cg_init(cg_, sums2_, sums3_, sweep_tol, ncols, stream_, rr_, 0.5 * opts_.refine_tolerance);
+      // b -= mean, r = b, x = 0, with (b, b) and max|b|; the CG's state and its first
+      // check in the same launch sequence.
+      sweep_start_cg(b_, r_, x_, np_, ncols, sums_, partials_, sums2_, sums3_, cg_, sweep_tol, rr_,
+                     0.5 * opts_.refine_tolerance, flags_, stream_, tfb_);
+      r32_ = tfb_;
+      first_checked_ = true;
       accumulate_ = true;
       pcg(ncols, mass, visc, inner, false);
       accumulate_ = false;
@@ -1199,7 +1201,8 @@ class Solver {
         // so the first check runs on the device alone and the iteration is queued behind
         // it without a host round trip.  A column with b = 0 is inactive on the device and
         // every vector kernel skips it; its transform columns compute into scratch only.
-        cg_check(cg_, rr_, flags_, ncols, stream_);
+        if (!first_checked_) cg_check(cg_, rr_, flags_, ncols, stream_);
+        first_checked_ = false;
         active.clear();
         for (int c = 0; c < ncols; ++c) active.push_back(c);
       } else {
@@ -1515,6 +1518,7 @@ class Solver {
   static Fin fin(double* out) { return Fin{out}; }
   float* du_ = nullptr;       // mixed sweeps: velocity correction, accumulated in the CG
   bool accumulate_ = false;   // pcg / start_from add alpha * (L~^-1 D^T p) into du_
+  bool first_checked_ = false;  // the next pcg's first check is queued already (sweep_start_cg)
   __bf16 *tb_ = nullptr, *tb1_ = nullptr, *tb2_ = nullptr;
   float *tf1_ = nullptr, *tf2_ = nullptr, *tfb_ = nullptr;
   int64_t tstride_ = 0, toff_v_ = 0;
