@@ -49,6 +49,7 @@ struct FftJob {
   int block_begin;          // set by the launcher
   int src_mode;             // FftSource (FFT_ROWS_FWD)
   double src_h;             // FFT_SRC_DX / FFT_SRC_DY: the grid spacing
+  int col;                  // FFT_ROWS_INV with sums: the job's column
 };
 constexpr int kMaxFftJobs = 8;
 struct FftBatch {
@@ -59,8 +60,15 @@ struct FftBatch {
   int epi;                  // FFT_STRIPS: EPI_SHIFT / EPI_LAMBDA; FFT_ROWS_INV: EPI_STORE /
                             // EPI_ACCUM / EPI_COMBINE (the last two with a DT_F64 dst)
   double s0, s1, thresh;    // as the GEMM epilogues use them
+  // FFT_ROWS_INV, EPI_COMBINE (optional): per-block partial sums of z (the output), z aux
+  // and aux at sums[(q * sums_cols + col) * gx + block], q = 0, 1, 2, gx the blocks of a
+  // job (all jobs alike).
+  double* sums;
+  int sums_cols;
 };
 void fft_pass(FftBatch& b, hipStream_t s);
+// Blocks per job of a row pass over `rows` rows at FFT length M (the sums' gx).
+int fft_row_blocks(int rows, int M);
 bool fft_length_ok(int n, int kind);
 // Host tables of a plan: M, the half-period twiddle table (M/2 entries) and the DCT
 // half-angle rotations (n entries) or DST-I's input weights (M/2 + 1 entries).

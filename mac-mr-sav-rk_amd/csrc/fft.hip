@@ -532,6 +532,7 @@ __device__ inline void rows_body(const FftBatch& b, const FftJob& J, int r0, flo
     }
   } else {
     const float inv_n = 1.0f / (float)N;
+    double s3[3] = {0.0, 0.0, 0.0};  // with b.sums: z, z aux, aux of the block's elements
     // The FP64 epilogues read dst (accumulate) or aux (combine): those loads first too.
     double oa[P][EPT], ob[P][EPT];
 #pragma unroll
@@ -556,6 +557,34 @@ __device__ inline void rows_body(const FftBatch& b, const FftJob& J, int r0, flo
         const float s = (KIND == KIND_DST2 && (j & 1)) ? -inv_n : inv_n;
         if (ra < rows) store_out(b, J, (int64_t)ra * J.dst_ld + j, v.x * s, oa[p][e]);
         if (rb < rows) store_out(b, J, (int64_t)rb * J.dst_ld + j, v.y * s, ob[p][e]);
+        if (b.sums != nullptr) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if ((h == 0 ? ra : rb) >= rows) continue;
+            const double o = h == 0 ? oa[p][e] : ob[p][e];
+            const double z = b.s0 * (double)((h == 0 ? v.x : v.y) * s) + b.s1 * o;
+            s3[0] += z;
+            s3[1] += z * o;
+            s3[2] += o;
+          }
+        }
+      }
+    }
+    if (b.sums != nullptr) {
+      // The block's partial sums: wave sums, then the four waves' in a fixed order.
+      __shared__ double wsum[3][kWaves];
+      const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s3[q] += __shfl_down(s3[q], off, 64);
+        if (lane == 0) wsum[q][wave] = s3[q];
+      }
+      __syncthreads();
+      if (tid < 3) {
+        const double t = (wsum[tid][0] + wsum[tid][1]) + (wsum[tid][2] + wsum[tid][3]);
+        const int gx = (rows + 2 * P - 1) / (2 * P);  // the job's blocks
+        b.sums[((int64_t)tid * b.sums_cols + J.col) * gx + r0 / (2 * P)] = t;
       }
     }
   }
@@ -1017,6 +1046,8 @@ bool fft_length_ok(int n, int kind) {
   const int m = kind == KIND_DST1 ? n + 1 : n;
   return m >= kFftMinLength && m <= kFftMaxLength && (m & (m - 1)) == 0;
 }
+
+int fft_row_blocks(int rows, int M) { return (rows + 2 * rows_pairs(M) - 1) / (2 * rows_pairs(M)); }
 
 void set_fft_events(hipEvent_t start, hipEvent_t stop) {
   f_ev_start = start;

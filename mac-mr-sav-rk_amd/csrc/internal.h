@@ -234,6 +234,10 @@ void cg_init(CgState* st, const double* bb, const double* bmax, double rtol, int
 void sweep_start(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,
                  double* bb, double* bmax, hipStream_t s, float* r32 = nullptr);
 void cg_check(CgState* st, const double* rr, int* flags, int ncols, hipStream_t s);
+// mixed_rho's scalar step alone, from partial sums of z, z r and r that another pass
+// wrote (gx per column and quantity).
+void mixed_rho_partials(CgState* st, const double* partials, int gx, int64_t n, int ncols, double* zmean,
+                        hipStream_t s);
 // sweep_start, then in one launch per column what the finalize of its partials, cg_init
 // (rtol, rr_out, mtol) and the first cg_check do: the same values in the same order.
 void sweep_start_cg(double* b, double* r, double* x, int64_t n, int ncols, const double* sums, double* partials,

@@ -923,6 +923,10 @@ void mixed_rho(const double* z, const double* r, int64_t n, int ncols, CgState* 
   k_sums3<<<grid, kThreads, 0, s>>>(z, r, n, n, st, partials);
   k_cg_rho_mean<<<ncols, kThreads, 0, s>>>(st, partials, (int)grid.x, ncols, (double)n, zmean);
 }
+void mixed_rho_partials(CgState* st, const double* partials, int gx, int64_t n, int ncols, double* zmean,
+                        hipStream_t s) {
+  k_cg_rho_mean<<<ncols, kThreads, 0, s>>>(st, partials, gx, ncols, (double)n, zmean);
+}
 void mixed_update_p(double* p, const double* z, const double* zmean, int64_t n, int ncols, const CgState* st,
                     hipStream_t s) {
   k_update_p_mean<<<col_grid(n, ncols), kThreads, 0, s>>>(p, z, zmean, n, n, st);

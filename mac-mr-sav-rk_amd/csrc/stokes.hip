@@ -426,8 +426,10 @@ class Solver {
   // z = s0 * (D D^T)^+ r + s1 * r through the same three passes (eigenvalue division with
   // the zero mode dropped; the last pass combines with the FP64 input).  in32 (optional):
   // the input rounded to FP32, read by the first pass.
+  // sums (optional): the last pass also writes the partial sums of z, z in and in that
+  // rho needs (fft_row_blocks per column and quantity, sums_cols columns).
   void pressure_inverse_fft(const double* in, double* out, const std::vector<int>& cols, double s0, double s1,
-                            const float* in32) {
+                            const float* in32, double* sums = nullptr, int sums_cols = 0) {
     const Block& B = pblk_;
     const int ldx = (B.nx + 15) & ~15;
     const size_t per_launch = kMaxFftJobs;
@@ -444,6 +446,8 @@ class Solver {
       fc.epi = EPI_COMBINE;
       fc.s0 = s0;
       fc.s1 = s1;
+      fc.sums = sums;
+      fc.sums_cols = sums_cols;
       for (size_t t = 0; t < count; ++t) {
         const int c = cols[base + t];
         float* X = fx_ + (int64_t)c * fstride_;
@@ -467,6 +471,7 @@ class Solver {
         jc.src_ld = ldx;
         jc.dst = out + (int64_t)c * np_;
         jc.aux = in + (int64_t)c * np_;
+        jc.col = c;
         jc.dst_ld = B.nx;
         jc.plan = B.mx->fft;
         jc.rows = B.ny;
@@ -1233,8 +1238,15 @@ class Solver {
         // passes and four fewer launches).
         double* zmean = means_;
         double* qmean = means_ + kMaxCols;
-        pressure_inverse_bf16(r_, z_, active, mass, visc, r32_);
-        mixed_rho(z_, r_, np_, ncols, cg_, partials_, zmean, stream_);
+        const int gxs = fft_pressure() ? fft_row_blocks(pblk_.ny, pblk_.mx->fft.M) : 0;
+        if (fft_pressure() && (size_t)3 * ncols * gxs <= partials_cap_) {
+          // z, and the sums rho needs, from the pressure inverse's last pass.
+          pressure_inverse_fft(r_, z_, active, mass, visc, r32_, partials_, ncols);
+          mixed_rho_partials(cg_, partials_, gxs, np_, ncols, zmean, stream_);
+        } else {
+          pressure_inverse_bf16(r_, z_, active, mass, visc, r32_);
+          mixed_rho(z_, r_, np_, ncols, cg_, partials_, zmean, stream_);
+        }
         mixed_update_p(p_, z_, zmean, np_, ncols, cg_, stream_);
         if (fft_velocity()) {
           // D^T p formed by the velocity inverse's first pass from the pressure rows.
@@ -1476,7 +1488,8 @@ class Solver {
     tfb_ = (float*)take(sizeof(float) * tstride_ * ncols);
     fx_ = (fft_ok_vel_ || fft_ok_p_) ? (float*)take(sizeof(float) * fstride_ * ncols) : nullptr;
     const int nb = std::max(defect_blocks(size_, ncols), reduce_blocks(np_) * ncols);
-    partials_ = (double*)take(sizeof(double) * (4 * (size_t)nb + 16));
+    partials_cap_ = 4 * (size_t)nb + 16;
+    partials_ = (double*)take(sizeof(double) * partials_cap_);
     sums_ = (double*)take(sizeof(double) * (ncols + 4));
     means_ = (double*)take(sizeof(double) * 2 * kMaxCols);
     sums2_ = (double*)take(sizeof(double) * (ncols + 4));
@@ -1556,6 +1569,7 @@ class Solver {
   float *warm_x_ = nullptr, *warm_y_ = nullptr;
   double* warm_s_ = nullptr;
   double *partials_ = nullptr, *sums_ = nullptr, *sums2_ = nullptr, *sums3_ = nullptr, *rr_ = nullptr;
+  size_t partials_cap_ = 0;  // doubles at partials_
   double* means_ = nullptr;  // mixed CG: column means of z and q (kMaxCols each)
   CgState* cg_ = nullptr;
   double* gal_ = nullptr;          // two-term Galerkin start: five dot products per column
