@@ -23,6 +23,7 @@ its buffers, so they change nothing.  Warm starts: `set_warm_start`.
 from __future__ import annotations
 
 import ctypes
+import math
 from time import perf_counter
 from typing import Any
 
@@ -105,6 +106,7 @@ class GpuTensorStokes:
         self.network_starts = 0
         self.last_report: dict[str, float] = {}
         self.warm_start = None
+        self._outs: dict[int, tuple[Any, Any]] = {}
 
     def __del__(self) -> None:
         handle = getattr(self, "_handle", None)
@@ -115,7 +117,8 @@ class GpuTensorStokes:
     # ------------------------------------------------------------------ solve
     def solve(self, rhs: Any, *, mass: float, viscosity: float,
               pressure_guess: Any = None) -> StokesResult:
-        if (not np.isfinite([mass, viscosity]).all() or min(mass, viscosity) < 0
+        mass, viscosity = float(mass), float(viscosity)
+        if (not (math.isfinite(mass) and math.isfinite(viscosity)) or min(mass, viscosity) < 0
                 or mass + viscosity <= 0):
             raise ValueError("Require mass,viscosity >= 0 and nonzero total")
         grid = self.grid
@@ -173,14 +176,17 @@ class GpuTensorStokes:
     def _call(self, rhs_ptr: int, ld: int, k: int, mass: float, viscosity: float, vel_ptr: int, pres_ptr: int,
               guess_ptr: int | None, ld_guess: int, *, device: bool) -> dict[str, float]:
         grid = self.grid
-        iters = (ctypes.c_int32 * k)()
-        report = _native.SolveReport()
+        # The result structures are reused from call to call (one per column count).
+        out = self._outs.get(k)
+        if out is None:
+            out = self._outs[k] = ((ctypes.c_int32 * k)(), _native.SolveReport())
+        iters, report = out
         start = perf_counter()
         code = self._lib.mac_stokes_solve(self._handle, rhs_ptr, ld, k, mass, viscosity, vel_ptr, grid.size,
                                           pres_ptr, grid.np, guess_ptr, ld_guess, iters, ctypes.byref(report),
                                           1 if device else 0)
         elapsed = perf_counter() - start
-        message = self._lib.mac_stokes_last_error(self._handle).decode()
+        message = self._lib.mac_stokes_last_error(self._handle).decode() if code != 0 else ""
         if code == _native.MAC_NOT_CONVERGED or code in (_native.MAC_INVALID_ARGUMENT, _native.MAC_HIP_ERROR):
             _native.raise_for(code, message)
         self.solve_seconds += elapsed
