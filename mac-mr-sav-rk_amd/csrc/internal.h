@@ -300,6 +300,9 @@ void accum_velocity(float* du, const float* w, int64_t n, int64_t stride, int nc
 // x += y (FP64 += FP32), n elements.
 void add_f32(double* x, const float* y, int64_t n, hipStream_t s);
 void add_cols(double* x, const double* y, int64_t n, int ncols, hipStream_t s);
+// x += y over nx elements and u += w (FP64 += FP32) over nu, in one launch: the
+// arithmetic of add_cols and add_f32.
+void add_pair(double* x, const double* y, int64_t nx, double* u, const float* w, int64_t nu, hipStream_t s);
 
 // Learned warm start (cnn.hip): grid sizes it serves, input preparation (pool by 4,
 // normalise) and the guess (network forward, bilinear up by 4, rescale).

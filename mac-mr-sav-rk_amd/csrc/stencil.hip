@@ -1061,6 +1061,23 @@ void add_f32(double* x, const float* y, int64_t n, hipStream_t s) {
   k_add_f32<<<reduce_blocks(n), kThreads, 0, s>>>(x, y, n);
 }
 
+__global__ __launch_bounds__(kThreads) void k_add_pair(double* __restrict__ x, const double* __restrict__ y,
+                                                      int64_t nx, double* __restrict__ u,
+                                                      const float* __restrict__ w, int64_t nu) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < nx) {
+    x[i] = x[i] + y[i];
+  } else if (i < nx + nu) {
+    const int64_t j = i - nx;
+    u[j] = u[j] + (double)w[j];
+  }
+}
+
+void add_pair(double* x, const double* y, int64_t nx, double* u, const float* w, int64_t nu, hipStream_t s) {
+  const int64_t n = nx + nu;
+  if (n > 0) k_add_pair<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, s>>>(x, y, nx, u, w, nu);
+}
+
 void add_cols(double* x, const double* y, int64_t n, int ncols, hipStream_t s) {
   k_add_cols<<<reduce_blocks(n * ncols), kThreads, 0, s>>>(x, y, n * ncols);
 }

@@ -1068,9 +1068,9 @@ class Solver {
       accumulate_ = false;
       r32_ = nullptr;
       for (int c = 0; c < ncols; ++c) out.iterations[c] += inner[c];
-      add_cols(pacc_, x_, np_, ncols, stream_);
-      // du = L~^-1 (r_u + D^T dp) = W + sum_k alpha_k L~^-1 D^T p_k, already summed in du_.
-      add_f32(uacc_, du_, size_ * ncols, stream_);
+      // p += dp, and u += du with du = L~^-1 (r_u + D^T dp) = W + sum_k alpha_k L~^-1 D^T p_k,
+      // already summed in du_.
+      add_pair(pacc_, x_, np_ * ncols, uacc_, du_, size_ * ncols, stream_);
       mixed_ = false;
       out.sweeps = sweep + 1;
     }
